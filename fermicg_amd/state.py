@@ -1,0 +1,113 @@
+"""Sparse-index algebra and the TPSCI vector container.
+
+FockConfig   = tuple of (nα, nβ) per cluster     (type_FockConfig.jl:10-12)
+ClusterConfig= tuple of 1-based state indices     (type_ClusterConfig.jl:7-10)
+TransferConfig = FockConfig - FockConfig          (Indexing.jl:5-23)
+
+`TPSCIstate` keeps the reference's flattening order (type_TPSCIstate.jl:
+139-229): Fock blocks in insertion order, configurations in insertion order
+inside each block, row = configuration, column = root.  It is stored flat:
+`fock` (n_fock, N, 2) int16, `fock_start` (n_fock+1) int64, `cfg` (n_cfg, N)
+int16 1-based, `coeffs` (n_cfg, R) float64.
+"""
+from itertools import product
+
+import numpy as np
+
+
+def fock_add(f, tc):
+    return tuple((a[0] + b[0], a[1] + b[1]) for a, b in zip(f, tc))
+
+
+def fock_sub(a, b):
+    return tuple((x[0] - y[0], x[1] - y[1]) for x, y in zip(a, b))
+
+
+def n_transferred(fock, ref):
+    """Electrons gained by clusters relative to `ref` (sum of positive
+    per-cluster, per-spin changes)."""
+    return sum(max(0, f[0] - r[0]) + max(0, f[1] - r[1]) for f, r in zip(fock, ref))
+
+
+def fock_configs_within(bases, ref_fock, max_transfer):
+    """Fock configurations with the reference's total (Nα, Nβ), every sector
+    present in the cluster bases, and at most `max_transfer` electrons moved
+    between clusters.  Order: Iterators.product over each cluster's sectors
+    (first cluster fastest), as expand_to_full_space! (tpsci_outer.jl:830-857)."""
+    na = sum(f[0] for f in ref_fock)
+    nb = sum(f[1] for f in ref_fock)
+    secs = [cb.sectors() for cb in bases]
+    out = []
+    for combo in product(*reversed(secs)):
+        f = tuple(reversed(combo))
+        if sum(x[0] for x in f) != na or sum(x[1] for x in f) != nb:
+            continue
+        if n_transferred(f, ref_fock) <= max_transfer:
+            out.append(tuple((int(a), int(b)) for a, b in f))
+    return out
+
+
+class TPSCIstate:
+    def __init__(self, n_clusters, R=1):
+        self.N = n_clusters
+        self.R = R
+        self.fock = np.zeros((0, n_clusters, 2), np.int16)
+        self.fock_start = np.zeros(1, np.int64)
+        self.cfg = np.zeros((0, n_clusters), np.int16)
+        self.coeffs = np.zeros((0, R))
+
+    @classmethod
+    def from_blocks(cls, n_clusters, blocks, R=1):
+        """`blocks` = list of (fock, [configs]) in insertion order."""
+        s = cls(n_clusters, R)
+        focks, starts, cfgs = [], [0], []
+        for f, confs in blocks:
+            focks.append(f)
+            cfgs.extend(confs)
+            starts.append(len(cfgs))
+        s.fock = np.array(focks, np.int16).reshape(len(focks), n_clusters, 2)
+        s.fock_start = np.array(starts, np.int64)
+        s.cfg = np.array(cfgs, np.int16).reshape(len(cfgs), n_clusters)
+        s.coeffs = np.zeros((len(cfgs), R))
+        return s
+
+    @classmethod
+    def full_blocks(cls, bases, focks, R=1):
+        """Every ClusterConfig of every Fock block (expand_each_fock_space!,
+        tpsci_outer.jl:796-819: Iterators.product, first cluster fastest)."""
+        N = len(bases)
+        blocks = []
+        for f in focks:
+            dims = [bases[c].n_states(f[c]) for c in range(N)]
+            confs = [tuple(reversed(t)) for t in product(*[range(1, d + 1) for d in reversed(dims)])]
+            blocks.append((f, confs))
+        return cls.from_blocks(N, blocks, R)
+
+    def __len__(self):
+        return self.cfg.shape[0]
+
+    @property
+    def n_fock(self):
+        return self.fock.shape[0]
+
+    def fock_of_cfg(self):
+        return np.repeat(np.arange(self.n_fock), np.diff(self.fock_start))
+
+    def get_vector(self):
+        return self.coeffs.copy()
+
+    def set_vector(self, v):
+        v = np.asarray(v, dtype=np.float64)
+        if v.ndim == 1:
+            v = v[:, None]
+        if v.shape[0] != len(self):
+            raise ValueError("dimension mismatch")
+        self.R = v.shape[1]
+        self.coeffs = v.copy()
+
+    def focks(self):
+        return [tuple((int(a), int(b)) for a, b in f) for f in self.fock]
+
+    def blocks(self):
+        for fi in range(self.n_fock):
+            yield fi, self.fock_start[fi], self.fock_start[fi + 1]

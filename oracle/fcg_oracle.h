@@ -1,0 +1,66 @@
+/* CPU oracle: FP64 C restatement of the reference's TPSCI / BST sigma build.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, the smoke check and bench.py's
+ * cpu_baseline leg load this library, and only as the checker / the CPU
+ * baseline; the product path never calls it.
+ *
+ * Input arrays use the same wire format as the HIP library (see
+ * fermicg_amd/marshal.py): sector code = na*S + nb, Γ blocks column-major
+ * (np, m_bra, m_ket), terms grouped by TransferConfig, configurations grouped
+ * by Fock block with 1-based int16 state indices, vectors column-major
+ * (n_cfg x R, element (i,r) at r*n_cfg + i).
+ */
+#ifndef FCG_ORACLE_H
+#define FCG_ORACLE_H
+#include <stdint.h>
+
+typedef struct {
+  int n_clusters, S, n_ops;
+  const int32_t *gblock;   /* [n_clusters][n_ops][S*S][S*S] */
+  const int64_t *gb_off;   /* [n_blocks] */
+  const int32_t *gb_dims;  /* [n_blocks][3] np, mb, mk */
+  const double *gdata;
+} orc_ops;
+
+typedef struct {
+  int n_terms;
+  const int32_t *nbody, *clusters, *ops, *parity, *dims; /* [T], [T][4] x4 */
+  const int16_t *delta;                                   /* [T][4][2] */
+  const int64_t *ioff;
+  const double *ints;
+  int n_tc;
+  const int16_t *tc;        /* [n_tc][N][2] */
+  const int32_t *tc_start;  /* [n_tc+1] */
+} orc_ham;
+
+typedef struct {
+  int64_t n_fock;
+  const int16_t *fock;        /* [n_fock][N][2] */
+  const int64_t *fock_start;  /* [n_fock+1] config ranges */
+  int64_t n_cfg;
+  const int16_t *cfg;         /* [n_cfg][N], 1-based */
+} orc_space;
+
+/* Reference tps_ci_matvec (tpsci_outer.jl:462-527): one job per bra config,
+ * loop over TransferConfigs, ket configs of the ket Fock block, terms;
+ * check_term then contract_matrix_element.  Returns 0, or -3 when a needed
+ * Γ block is missing. */
+int orc_tpsci_sigma(const orc_ops *ops, const orc_ham *h, const orc_space *sp,
+                    int R, const double *c, double *sigma, int nthreads);
+
+/* Reference build_full_H (tpsci_outer.jl:10-50): dense symmetric H. */
+int orc_build_full_H(const orc_ops *ops, const orc_ham *h, const orc_space *sp,
+                     double *H, int nthreads);
+
+/* Reference compute_diagonal (tpsci_outer.jl:694-715): terms of the zero
+ * TransferConfig with bra == ket.  `zero_tc` is that group's index. */
+int orc_tpsci_diagonal(const orc_ops *ops, const orc_ham *h, const orc_space *sp,
+                       int zero_tc, double *hd);
+
+/* Sampled sigma: only the bra rows listed in `rows` (for CPU timing of a
+ * bounded fraction of one sigma build).  sigma_rows is [n_rows x R]. */
+int orc_tpsci_sigma_rows(const orc_ops *ops, const orc_ham *h, const orc_space *sp,
+                         int R, const double *c, const int64_t *rows, int64_t n_rows,
+                         double *sigma_rows, int nthreads);
+
+#endif
