@@ -1,0 +1,279 @@
+"""ctypes binding of libfcg_hip.so (the C ABI declared in include/fcg.h).
+
+The product path always runs through this library; if it is missing or no
+GPU is visible, every entry point raises -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfcg_hip.so")
+
+FCG_FP64, FCG_FP32, FCG_BF16X2 = 0, 1, 2
+PRECISION = {"fp64": FCG_FP64, "fp32": FCG_FP32, "bf16x2": FCG_BF16X2}
+ERRORS = {-1: "FCG_EINVAL", -2: "FCG_ENOMEM", -3: "FCG_EMISSING_BLOCK", -4: "FCG_EOVERFLOW",
+          -5: "FCG_EDEVICE", -6: "FCG_ENOTCONV"}
+
+P = ctypes.POINTER
+c_i16, c_i32, c_i64, c_f64 = ctypes.c_int16, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+VP = ctypes.c_void_p
+
+EXPORTS = [
+    "fcg_create", "fcg_destroy", "fcg_last_error", "fcg_ops_upload", "fcg_ops_destroy",
+    "fcg_ham_upload", "fcg_ham_destroy", "fcg_tpsci_space_upload", "fcg_bst_space_upload",
+    "fcg_space_nvar", "fcg_space_destroy", "fcg_plan_create", "fcg_plan_destroy",
+    "fcg_plan_stats", "fcg_sigma", "fcg_sigma_device", "fcg_diagonal", "fcg_davidson",
+    "fcg_last_sigma_time",
+]
+
+
+class DavidsonOpts(ctypes.Structure):
+    _fields_ = [("conv_thresh", c_f64), ("max_iter", ctypes.c_int), ("max_ss_vecs", ctypes.c_int),
+                ("lindep_thresh", c_f64), ("precision", ctypes.c_int), ("refresh_k", ctypes.c_int),
+                ("precond", ctypes.c_int)]
+
+
+class DavidsonStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("sigma_fp64", ctypes.c_int),
+                ("sigma_low", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", c_f64 * 16)]
+
+
+class FCGError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load the library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FCGError("libfcg_hip.so not built: run `make -C fermicg_amd/csrc` "
+                       "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.fcg_create.argtypes = [ctypes.c_int, P(VP)]
+    L.fcg_destroy.argtypes = [VP]
+    L.fcg_destroy.restype = None
+    L.fcg_last_error.argtypes = [VP]
+    L.fcg_last_error.restype = ctypes.c_char_p
+    L.fcg_ops_upload.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(c_i32), c_i64,
+                                 P(c_i64), P(c_i32), P(c_f64), c_i64, P(VP)]
+    L.fcg_ops_destroy.argtypes = [VP]
+    L.fcg_ops_destroy.restype = None
+    L.fcg_ham_upload.argtypes = [VP, ctypes.c_int, ctypes.c_int, P(c_i32), P(c_i32), P(c_i32),
+                                 P(c_i32), P(c_i32), P(c_i16), P(c_i64), P(c_f64), c_i64,
+                                 ctypes.c_int, P(c_i16), P(c_i32), P(VP)]
+    L.fcg_ham_destroy.argtypes = [VP]
+    L.fcg_ham_destroy.restype = None
+    L.fcg_tpsci_space_upload.argtypes = [VP, ctypes.c_int, c_i64, P(c_i16), P(c_i64), c_i64,
+                                         P(c_i16), P(VP)]
+    L.fcg_bst_space_upload.argtypes = [VP, ctypes.c_int, c_i64, P(c_i16), c_i64, P(c_i32),
+                                       P(c_i16), P(c_i16), P(c_i16), P(c_i64), P(c_f64), c_i64,
+                                       P(VP)]
+    L.fcg_space_nvar.argtypes = [VP]
+    L.fcg_space_nvar.restype = c_i64
+    L.fcg_space_destroy.argtypes = [VP]
+    L.fcg_space_destroy.restype = None
+    L.fcg_plan_create.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
+    L.fcg_plan_destroy.argtypes = [VP]
+    L.fcg_plan_destroy.restype = None
+    L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
+    L.fcg_sigma.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(c_f64), ctypes.c_int]
+    L.fcg_sigma_device.argtypes = [VP, VP, ctypes.c_int, VP, VP, ctypes.c_int]
+    L.fcg_diagonal.argtypes = [VP, VP, P(c_f64)]
+    L.fcg_davidson.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(DavidsonOpts), P(c_f64),
+                               P(c_f64), P(DavidsonStats)]
+    L.fcg_last_sigma_time.argtypes = [VP, P(c_f64), P(ctypes.c_int)]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+class Context:
+    """One device context (stream, events, BLAS handle) of the library."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = VP()
+        rc = self.L.fcg_create(device, ctypes.byref(h))
+        if rc:
+            raise FCGError("fcg_create(%d) failed: %s (is a GPU visible?)" % (device, ERRORS.get(rc, rc)))
+        self.h = h
+
+    def check(self, rc, what):
+        if rc:
+            msg = self.L.fcg_last_error(self.h).decode(errors="replace")
+            raise FCGError("%s: %s %s" % (what, ERRORS.get(rc, rc), msg))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self, ctx, h, keep=()):
+        self.ctx, self.h, self._keep = ctx, h, keep
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                getattr(self.ctx.L, self._destroy)(self.h)
+        except Exception:
+            pass
+        self.h = None
+
+
+class DeviceOps(_Handle):
+    _destroy = "fcg_ops_destroy"
+
+    @classmethod
+    def upload(cls, ctx, fops):
+        gb = np.ascontiguousarray(fops.gblock, np.int32)
+        off = np.ascontiguousarray(fops.gb_off, np.int64)
+        dims = np.ascontiguousarray(fops.gb_dims, np.int32)
+        data = np.ascontiguousarray(fops.gdata, np.float64)
+        h = VP()
+        ctx.check(ctx.L.fcg_ops_upload(ctx.h, fops.n_clusters, fops.S, fops.n_ops, _p(gb, c_i32),
+                                       len(off), _p(off, c_i64), _p(dims, c_i32), _p(data, c_f64),
+                                       data.size, ctypes.byref(h)), "fcg_ops_upload")
+        return cls(ctx, h)
+
+
+class DeviceHam(_Handle):
+    _destroy = "fcg_ham_destroy"
+
+    @classmethod
+    def upload(cls, ctx, fham, n_clusters):
+        types = {"nbody": np.int32, "clusters": np.int32, "ops": np.int32, "parity": np.int32,
+                 "dims": np.int32, "delta": np.int16, "ioff": np.int64, "ints": np.float64,
+                 "tc": np.int16, "tc_start": np.int32}
+        a = {k: np.ascontiguousarray(getattr(fham, k), dtype=t) for k, t in types.items()}
+        h = VP()
+        ctx.check(ctx.L.fcg_ham_upload(
+            ctx.h, n_clusters, fham.n_terms, _p(a["nbody"], c_i32), _p(a["clusters"], c_i32),
+            _p(a["ops"], c_i32), _p(a["parity"], c_i32), _p(a["dims"], c_i32),
+            _p(a["delta"], c_i16), _p(a["ioff"], c_i64), _p(a["ints"], c_f64), a["ints"].size,
+            fham.n_tc, _p(a["tc"], c_i16), _p(a["tc_start"], c_i32), ctypes.byref(h)),
+            "fcg_ham_upload")
+        return cls(ctx, h)
+
+
+class DeviceSpace(_Handle):
+    _destroy = "fcg_space_destroy"
+
+    @classmethod
+    def tpsci(cls, ctx, state):
+        fock = np.ascontiguousarray(state.fock, np.int16)
+        fs = np.ascontiguousarray(state.fock_start, np.int64)
+        cfg = np.ascontiguousarray(state.cfg, np.int16)
+        h = VP()
+        ctx.check(ctx.L.fcg_tpsci_space_upload(ctx.h, state.N, state.n_fock, _p(fock, c_i16),
+                                               _p(fs, c_i64), len(state), _p(cfg, c_i16),
+                                               ctypes.byref(h)), "fcg_tpsci_space_upload")
+        return cls(ctx, h)
+
+    @classmethod
+    def bst(cls, ctx, flat):
+        """`flat` is fermicg_amd.bst.FlatBST."""
+        h = VP()
+        ctx.check(ctx.L.fcg_bst_space_upload(
+            ctx.h, flat.N, flat.n_fock, _p(flat.fock, c_i16), flat.n_box,
+            _p(flat.box_fock, c_i32), _p(flat.lo, c_i16), _p(flat.hi, c_i16),
+            _p(flat.rank, c_i16), _p(flat.foff, c_i64), _p(flat.factors, c_f64),
+            flat.factors.size, ctypes.byref(h)), "fcg_bst_space_upload")
+        return cls(ctx, h)
+
+    def nvar(self):
+        return int(self.ctx.L.fcg_space_nvar(self.h))
+
+
+class Plan(_Handle):
+    _destroy = "fcg_plan_destroy"
+
+    @classmethod
+    def create(cls, ctx, ham, ops, space, nbody=4, precisions=("fp64",)):
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISION[p]
+        h = VP()
+        ctx.check(ctx.L.fcg_plan_create(ctx.h, ham.h, ops.h, space.h, nbody, mask,
+                                        ctypes.byref(h)), "fcg_plan_create")
+        pl = cls(ctx, h, keep=(ham, ops, space))
+        pl.nvar = space.nvar()
+        return pl
+
+    def stats(self):
+        out = np.zeros(5)
+        self.ctx.check(self.ctx.L.fcg_plan_stats(self.h, _p(out, c_f64)), "fcg_plan_stats")
+        return {"pairs": int(out[0]), "tasks": int(out[1]), "block_elems": int(out[2]),
+                "build_flops": out[3], "build_s": out[4] * 1e-6}
+
+    def sigma(self, c, precision="fp64"):
+        c = np.asarray(c, dtype=np.float64)
+        one = c.ndim == 1
+        if one:
+            c = c[:, None]
+        if c.shape[0] != self.nvar:
+            raise ValueError("vector length %d != space dimension %d" % (c.shape[0], self.nvar))
+        R = c.shape[1]
+        cc = np.ascontiguousarray(c.T)  # (R, n) == column-major (n, R)
+        out = np.zeros((R, self.nvar))
+        self.ctx.check(self.ctx.L.fcg_sigma(self.ctx.h, self.h, R, _p(cc, c_f64), _p(out, c_f64),
+                                            PRECISION[precision]), "fcg_sigma")
+        s = out.T.copy()
+        return s[:, 0] if one else s
+
+    def sigma_device(self, c_ptr, s_ptr, R, precision="fp64"):
+        self.ctx.check(self.ctx.L.fcg_sigma_device(self.ctx.h, self.h, R, VP(c_ptr), VP(s_ptr),
+                                                   PRECISION[precision]), "fcg_sigma_device")
+
+    def last_sigma_ms(self):
+        ms = c_f64()
+        n = ctypes.c_int()
+        self.ctx.check(self.ctx.L.fcg_last_sigma_time(self.ctx.h, ctypes.byref(ms), ctypes.byref(n)),
+                       "fcg_last_sigma_time")
+        return ms.value
+
+    def diagonal(self):
+        out = np.zeros(self.nvar)
+        self.ctx.check(self.ctx.L.fcg_diagonal(self.ctx.h, self.h, _p(out, c_f64)), "fcg_diagonal")
+        return out
+
+    def davidson(self, v0, conv_thresh=1e-8, max_iter=200, max_ss_vecs=12, lindep_thresh=1e-12,
+                 precision="fp64", refresh_k=4, precond=True):
+        v0 = np.asarray(v0, dtype=np.float64)
+        if v0.ndim == 1:
+            v0 = v0[:, None]
+        R = v0.shape[1]
+        vv = np.ascontiguousarray(v0.T)
+        ev = np.zeros(R)
+        vec = np.zeros((R, self.nvar))
+        o = DavidsonOpts(conv_thresh, max_iter, max_ss_vecs, lindep_thresh, PRECISION[precision],
+                         refresh_k, 1 if precond else 0)
+        st = DavidsonStats()
+        rc = self.ctx.L.fcg_davidson(self.ctx.h, self.h, R, _p(vv, c_f64), ctypes.byref(o),
+                                     _p(ev, c_f64), _p(vec, c_f64), ctypes.byref(st))
+        if rc and rc != -6:
+            self.ctx.check(rc, "fcg_davidson")
+        stats = {"iterations": st.iterations, "sigma_fp64": st.sigma_fp64,
+                 "sigma_low": st.sigma_low, "converged": bool(st.converged),
+                 "residual": list(st.residual)[:R]}
+        return ev, vec.T.copy(), stats

@@ -1,0 +1,477 @@
+// C ABI of libfcg_hip.so: handles, uploads, plan creation, sigma, diagonal.
+#include <chrono>
+#include <cstring>
+
+#include "fcg_internal.h"
+
+namespace {
+
+template <typename T>
+int dev_upload(fcg_ctx *ctx, T **dst, const T *src, int64_t n) {
+  *dst = nullptr;
+  if (n <= 0) return FCG_OK;
+  if (hipMalloc((void **)dst, sizeof(T) * n) != hipSuccess) {
+    *dst = nullptr;
+    ctx->err = "hipMalloc failed (" + std::to_string(sizeof(T) * n) + " bytes)";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream), ctx);
+  return FCG_OK;
+}
+
+template <typename T>
+int dev_alloc(fcg_ctx *ctx, T **dst, int64_t n) {
+  *dst = nullptr;
+  if (n <= 0) return FCG_OK;
+  if (hipMalloc((void **)dst, sizeof(T) * n) != hipSuccess) {
+    *dst = nullptr;
+    ctx->err = "hipMalloc failed (" + std::to_string(sizeof(T) * n) + " bytes)";
+    return FCG_ENOMEM;
+  }
+  return FCG_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+#define TRY(x)              \
+  do {                      \
+    int rc_ = (x);          \
+    if (rc_) return rc_;    \
+  } while (0)
+
+int ensure_scratch(fcg_ctx *ctx, int64_t n) {
+  if (n <= ctx->scratch_elems) return FCG_OK;
+  dev_free(ctx->d_c);
+  dev_free(ctx->d_s);
+  ctx->scratch_elems = 0;
+  TRY(dev_alloc(ctx, &ctx->d_c, n));
+  TRY(dev_alloc(ctx, &ctx->d_s, n));
+  ctx->scratch_elems = n;
+  return FCG_OK;
+}
+
+int space_to_device(fcg_space *sp) {
+  fcg_ctx *ctx = sp->ctx;
+  const int64_t nb = sp->n_box * sp->N;
+  TRY(dev_upload(ctx, &sp->d_lo, sp->lo.data(), nb));
+  TRY(dev_upload(ctx, &sp->d_hi, sp->hi.data(), nb));
+  TRY(dev_upload(ctx, &sp->d_rank, sp->rank.data(), nb));
+  TRY(dev_upload(ctx, &sp->d_foff, sp->foff.data(), nb));
+  TRY(dev_upload(ctx, &sp->d_var, sp->var_off.data(), sp->n_box + 1));
+  if (!sp->factors.empty())
+    TRY(dev_upload(ctx, &sp->d_factors, sp->factors.data(), (int64_t)sp->factors.size()));
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  return FCG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcg_create(int device, fcg_ctx_t *out) {
+  if (!out) return FCG_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return FCG_EDEVICE;
+  fcg_ctx *ctx = new fcg_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return FCG_EDEVICE;
+  }
+  *out = ctx;
+  return FCG_OK;
+}
+
+void fcg_destroy(fcg_ctx_t ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  dev_free(ctx->d_c);
+  dev_free(ctx->d_s);
+  fcg_blas_destroy(ctx);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *fcg_last_error(fcg_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fcg_ops_upload(fcg_ctx_t ctx, int n_clusters, int S, int n_ops, const int32_t *gblock,
+                   int64_t n_blocks, const int64_t *gb_off, const int32_t *gb_dims,
+                   const double *gdata, int64_t n_gdata, fcg_ops_t *out) {
+  if (!ctx || !out || n_clusters <= 0 || n_clusters > FCG_MAXN || S <= 1 || n_ops <= 0)
+    return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  fcg_ops *o = new fcg_ops();
+  o->ctx = ctx;
+  o->N = n_clusters;
+  o->S = S;
+  o->n_ops = n_ops;
+  const int64_t SS = (int64_t)S * S;
+  o->gblock.assign(gblock, gblock + (int64_t)n_clusters * n_ops * SS * SS);
+  o->gb_off.assign(gb_off, gb_off + n_blocks);
+  o->gb_dims.assign(gb_dims, gb_dims + 3 * n_blocks);
+  for (int64_t b = 0; b < n_blocks; b++) {
+    const int64_t sz = (int64_t)gb_dims[3 * b] * gb_dims[3 * b + 1] * gb_dims[3 * b + 2];
+    if (gb_off[b] < 0 || gb_off[b] + sz > n_gdata) {
+      ctx->err = "Γ block " + std::to_string(b) + " outside gdata";
+      delete o;
+      return FCG_EINVAL;
+    }
+  }
+  o->nstates.assign(n_clusters * SS, 0);
+  for (int c = 0; c < n_clusters; c++)
+    for (int code = 0; code < SS; code++) {
+      const int blk = o->block(c, 0 /* "H" */, code, code);
+      if (blk >= 0) o->nstates[c * SS + code] = gb_dims[3 * blk + 1];
+    }
+  int rc = dev_upload(ctx, &o->d_gdata, gdata, n_gdata);
+  if (!rc) rc = dev_upload(ctx, &o->d_gb_off, gb_off, n_blocks);
+  if (!rc) rc = dev_upload(ctx, &o->d_gb_dims, gb_dims, 3 * n_blocks);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FCG_EDEVICE;
+  if (rc) {
+    fcg_ops_destroy(o);
+    return rc;
+  }
+  *out = o;
+  return FCG_OK;
+}
+
+void fcg_ops_destroy(fcg_ops_t o) {
+  if (!o) return;
+  dev_free(o->d_gdata);
+  dev_free(o->d_gb_off);
+  dev_free(o->d_gb_dims);
+  delete o;
+}
+
+int fcg_ham_upload(fcg_ctx_t ctx, int n_clusters, int n_terms, const int32_t *nbody,
+                   const int32_t *clusters, const int32_t *ops, const int32_t *parity,
+                   const int32_t *dims, const int16_t *delta, const int64_t *ioff,
+                   const double *ints, int64_t n_ints, int n_tc, const int16_t *tc,
+                   const int32_t *tc_start, fcg_ham_t *out) {
+  if (!ctx || !out || n_clusters <= 0 || n_clusters > FCG_MAXN || n_terms < 0 || n_tc < 0)
+    return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  fcg_ham *h = new fcg_ham();
+  h->ctx = ctx;
+  h->N = n_clusters;
+  h->n_terms = n_terms;
+  h->n_tc = n_tc;
+  h->nbody.assign(nbody, nbody + n_terms);
+  h->clusters.assign(clusters, clusters + 4 * n_terms);
+  h->ops.assign(ops, ops + 4 * n_terms);
+  h->parity.assign(parity, parity + 4 * n_terms);
+  h->dims.assign(dims, dims + 4 * n_terms);
+  h->delta.assign(delta, delta + 8 * n_terms);
+  h->ioff.assign(ioff, ioff + n_terms);
+  h->tc.assign(tc, tc + (int64_t)n_tc * 2 * n_clusters);
+  h->tc_start.assign(tc_start, tc_start + n_tc + 1);
+  std::vector<TermDev> td(n_terms);
+  for (int t = 0; t < n_terms; t++) {
+    const int nb = nbody[t];
+    if (nb < 1 || nb > 4 || tc_start[n_tc] != n_terms) {
+      ctx->err = "bad term table";
+      delete h;
+      return FCG_EINVAL;
+    }
+    int64_t sz = 1;
+    td[t].nbody = nb;
+    for (int a = 0; a < 4; a++) {
+      td[t].cl[a] = clusters[4 * t + a];
+      td[t].dims[a] = dims[4 * t + a];
+      if (a < nb) {
+        if (clusters[4 * t + a] < 0 || clusters[4 * t + a] >= n_clusters ||
+            (a > 0 && clusters[4 * t + a] <= clusters[4 * t + a - 1])) {
+          ctx->err = "term clusters must be ascending and in range";
+          delete h;
+          return FCG_EINVAL;
+        }
+        if (nb > 1) sz *= dims[4 * t + a];
+      }
+    }
+    td[t].pad = 0;
+    td[t].ioff = ioff[t];
+    if (nb > 1 && (ioff[t] < 0 || ioff[t] + sz > n_ints)) {
+      ctx->err = "term integrals outside ints";
+      delete h;
+      return FCG_EINVAL;
+    }
+  }
+  int rc = dev_upload(ctx, &h->d_terms, td.data(), n_terms);
+  if (!rc) rc = dev_upload(ctx, &h->d_ints, ints, n_ints);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = FCG_EDEVICE;
+  if (rc) {
+    fcg_ham_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return FCG_OK;
+}
+
+void fcg_ham_destroy(fcg_ham_t h) {
+  if (!h) return;
+  dev_free(h->d_terms);
+  dev_free(h->d_ints);
+  delete h;
+}
+
+int fcg_tpsci_space_upload(fcg_ctx_t ctx, int n_clusters, int64_t n_fock, const int16_t *fock,
+                           const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
+                           fcg_space_t *out) {
+  // The space needs ClusterOps state counts to recognise full blocks; they
+  // are taken at plan time, so here the caller's arrays are only kept.
+  if (!ctx || !out || n_clusters <= 0 || n_clusters > FCG_MAXN || n_fock < 0 || n_cfg < 0)
+    return FCG_EINVAL;
+  fcg_space *sp = new fcg_space();
+  sp->ctx = ctx;
+  sp->N = n_clusters;
+  sp->tpsci = true;
+  sp->n_fock = n_fock;
+  sp->fock.assign(fock, fock + n_fock * 2 * n_clusters);
+  // stash configurations in lo/hi until boxes are formed against an ops table
+  sp->var_off.assign(fock_start, fock_start + n_fock + 1);
+  sp->lo.assign(cfg, cfg + n_cfg * n_clusters);
+  sp->nvar = n_cfg;
+  sp->n_box = -1;  // boxes not formed yet
+  *out = sp;
+  return FCG_OK;
+}
+
+int fcg_bst_space_upload(fcg_ctx_t ctx, int n_clusters, int64_t n_fock, const int16_t *fock,
+                         int64_t n_box, const int32_t *box_fock, const int16_t *box_lo,
+                         const int16_t *box_hi, const int16_t *box_rank, const int64_t *box_foff,
+                         const double *factors, int64_t n_factors, fcg_space_t *out) {
+  if (!ctx || !out || n_clusters <= 0 || n_clusters > FCG_MAXN || n_fock < 0 || n_box < 0)
+    return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  const int N = n_clusters;
+  fcg_space *sp = new fcg_space();
+  sp->ctx = ctx;
+  sp->N = N;
+  sp->n_fock = n_fock;
+  sp->n_box = n_box;
+  sp->fock.assign(fock, fock + n_fock * 2 * N);
+  sp->box_fock.assign(box_fock, box_fock + n_box);
+  sp->lo.assign(box_lo, box_lo + n_box * N);
+  sp->hi.assign(box_hi, box_hi + n_box * N);
+  sp->rank.assign(box_rank, box_rank + n_box * N);
+  sp->foff.assign(box_foff, box_foff + n_box * N);
+  sp->factors.assign(factors, factors + n_factors);
+  sp->var_off.assign(1, 0);
+  for (int64_t b = 0; b < n_box; b++) {
+    if (box_fock[b] < 0 || box_fock[b] >= n_fock) {
+      ctx->err = "box Fock index out of range";
+      delete sp;
+      return FCG_EINVAL;
+    }
+    int64_t d = 1;
+    for (int c = 0; c < N; c++) {
+      const int64_t i = b * N + c;
+      const int nrg = box_hi[i] - box_lo[i] + 1;
+      if (box_lo[i] < 1 || nrg < 1 || box_rank[i] < 1 ||
+          (box_foff[i] < 0 && box_rank[i] != nrg) ||
+          (box_foff[i] >= 0 && (box_rank[i] > nrg ||
+                                box_foff[i] + (int64_t)nrg * box_rank[i] > n_factors))) {
+        ctx->err = "bad box " + std::to_string(b) + " cluster " + std::to_string(c);
+        delete sp;
+        return FCG_EINVAL;
+      }
+      d *= box_rank[i];
+    }
+    sp->var_off.push_back(sp->var_off.back() + d);
+  }
+  sp->nvar = sp->var_off.back();
+  int rc = space_to_device(sp);
+  if (rc) {
+    fcg_space_destroy(sp);
+    return rc;
+  }
+  *out = sp;
+  return FCG_OK;
+}
+
+int64_t fcg_space_nvar(fcg_space_t sp) { return sp ? sp->nvar : -1; }
+
+void fcg_space_destroy(fcg_space_t sp) {
+  if (!sp) return;
+  dev_free(sp->d_lo);
+  dev_free(sp->d_hi);
+  dev_free(sp->d_rank);
+  dev_free(sp->d_foff);
+  dev_free(sp->d_var);
+  dev_free(sp->d_dim);
+  dev_free(sp->d_factors);
+  delete sp;
+}
+
+int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int nbody,
+                    int precisions, fcg_plan_t *out) {
+  if (!ctx || !h || !o || !sp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  auto t0 = std::chrono::steady_clock::now();
+  if (sp->tpsci && sp->n_box < 0) {
+    // form boxes of the TPSCI space against this ClusterOps table
+    std::vector<int64_t> fstart = sp->var_off;
+    std::vector<int16_t> cfg = sp->lo;
+    std::vector<int16_t> fock = sp->fock;
+    sp->lo.clear();
+    sp->var_off.clear();
+    int rc = fcg_tpsci_boxes(o, sp->N, sp->n_fock, fock.data(), fstart.data(),
+                             (int64_t)cfg.size() / sp->N, cfg.data(), sp, &ctx->err);
+    if (rc) {
+      sp->lo = cfg;
+      sp->var_off = fstart;
+      return rc;
+    }
+    TRY(space_to_device(sp));
+  }
+  HostPlan hp;
+  int rc = fcg_build_host_plan(h, o, sp, nbody, &hp, &ctx->err);
+  if (rc) return rc;
+  fcg_plan *pl = new fcg_plan();
+  pl->ctx = ctx;
+  pl->sp = sp;
+  pl->n_pair = (int64_t)hp.pair_B.size();
+  pl->n_task = (int64_t)hp.tasks.size();
+  pl->n_kel = hp.n_kel;
+  pl->n_cols = (int64_t)hp.colidx.size();
+  pl->build_flops = hp.flops;
+  pl->precisions = precisions | (1 << FCG_FP64);
+#define PTRY(x)                 \
+  do {                          \
+    int rc_ = (x);              \
+    if (rc_) {                  \
+      fcg_plan_destroy(pl);     \
+      return rc_;               \
+    }                           \
+  } while (0)
+  PTRY(dev_upload(ctx, &pl->d_row_ptr, hp.row_ptr.data(), sp->n_box + 1));
+  PTRY(dev_upload(ctx, &pl->d_pair_B, hp.pair_B.data(), pl->n_pair));
+  PTRY(dev_upload(ctx, &pl->d_pair_K, hp.pair_K.data(), pl->n_pair));
+  PTRY(dev_upload(ctx, &pl->d_pair_t0, hp.pair_t0.data(), pl->n_pair + 1));
+  PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
+  PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
+  PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
+  PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
+  PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
+  PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
+  PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
+  if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
+  if (pl->precisions & (1 << FCG_BF16X2)) PTRY(dev_alloc(ctx, &pl->d_Kbf, pl->n_kel));
+  if (pl->n_pair > 0) {
+    hipError_t e = fcg_launch_build(pl, h, o, ctx->stream);
+    if (e == hipSuccess) e = fcg_launch_convert(pl, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      ctx->err = std::string("block build: ") + hipGetErrorString(e);
+      fcg_plan_destroy(pl);
+      return FCG_EDEVICE;
+    }
+  }
+  // device copies of the task bookkeeping are only needed for the build
+  dev_free(pl->d_tasks);
+  dev_free(pl->d_pair_t0);
+  pl->build_us =
+      std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  *out = pl;
+  return FCG_OK;
+#undef PTRY
+}
+
+void fcg_plan_destroy(fcg_plan_t pl) {
+  if (!pl) return;
+  dev_free(pl->d_row_ptr);
+  dev_free(pl->d_pair_B);
+  dev_free(pl->d_pair_K);
+  dev_free(pl->d_pair_t0);
+  dev_free(pl->d_pair_koff);
+  dev_free(pl->d_tasks);
+  dev_free(pl->d_diag_koff);
+  dev_free(pl->d_row_koff);
+  dev_free(pl->d_col_ptr);
+  dev_free(pl->d_colidx);
+  dev_free(pl->d_K64);
+  dev_free(pl->d_K32);
+  dev_free(pl->d_Kbf);
+  dev_free(pl->d_c32);
+  delete pl;
+}
+
+int fcg_plan_stats(fcg_plan_t pl, double *o5) {
+  if (!pl || !o5) return FCG_EINVAL;
+  o5[0] = (double)pl->n_pair;
+  o5[1] = (double)pl->n_task;
+  o5[2] = (double)pl->n_kel;
+  o5[3] = pl->build_flops;
+  o5[4] = pl->build_us;
+  return FCG_OK;
+}
+
+int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, double *s_dev,
+                     int precision) {
+  if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c_dev || !s_dev) return FCG_EINVAL;
+  if (precision < FCG_FP64 || precision > FCG_BF16X2 || !(pl->precisions & (1 << precision))) {
+    ctx->err = "precision not built into this plan";
+    return FCG_EINVAL;
+  }
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = (int64_t)R * pl->sp->nvar;
+  if (precision != FCG_FP64 && pl->c32_elems < n) {
+    dev_free(pl->d_c32);
+    pl->c32_elems = 0;
+    TRY(dev_alloc(ctx, &pl->d_c32, n));
+    pl->c32_elems = n;
+  }
+  FCG_HIP(hipEventRecord(ctx->ev0, ctx->stream), ctx);
+  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream), ctx);
+  FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
+  ctx->last_launches = precision == FCG_FP64 ? 1 : 2;
+  return FCG_OK;
+}
+
+int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c, double *sigma,
+              int precision) {
+  if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c || !sigma) return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = (int64_t)R * pl->sp->nvar;
+  TRY(ensure_scratch(ctx, n));
+  FCG_HIP(hipMemcpyAsync(ctx->d_c, c, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream),
+          ctx);
+  TRY(fcg_sigma_device(ctx, pl, R, ctx->d_c, ctx->d_s, precision));
+  FCG_HIP(hipMemcpyAsync(sigma, ctx->d_s, sizeof(double) * n, hipMemcpyDeviceToHost,
+                         ctx->stream),
+          ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  return FCG_OK;
+}
+
+int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t pl, double *hd) {
+  if (!ctx || !pl || !hd) return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  const int64_t n = pl->sp->nvar;
+  TRY(ensure_scratch(ctx, n));
+  FCG_HIP(fcg_launch_diag(pl, ctx->d_s, ctx->stream), ctx);
+  FCG_HIP(hipMemcpyAsync(hd, ctx->d_s, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
+          ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  return FCG_OK;
+}
+
+int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches) {
+  if (!ctx) return FCG_EINVAL;
+  FCG_HIP(hipEventSynchronize(ctx->ev1), ctx);
+  float t = 0.f;
+  FCG_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1), ctx);
+  if (ms) *ms = t;
+  if (launches) *launches = ctx->last_launches;
+  return FCG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Internal types of libfcg_hip.so (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fcg.h"
+
+// Limits of the block engine.  A "box" is a tensor-product block of the
+// basis: one Fock configuration plus, per cluster, a range of local states
+// and a Tucker factor (identity for TPSCI).  Hamiltonian blocks between two
+// boxes are built whole in one workgroup, so their sizes are bounded by LDS.
+constexpr int FCG_MAXN = 16;          // clusters
+constexpr int FCG_MAX_BOX = 64;       // rows/cols of one Hamiltonian block
+constexpr int FCG_LDS_X = 4096;       // chain intermediates (doubles)
+constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
+constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
+constexpr int FCG_MAX_R = 16;         // roots per sigma launch
+
+#define FCG_HIP(call, ctx)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);            \
+      return FCG_EDEVICE;                                                         \
+    }                                                                             \
+  } while (0)
+
+struct fcg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  void *blas = nullptr;  // rocblas_handle
+  // scratch for host-pointer sigma calls
+  double *d_c = nullptr, *d_s = nullptr;
+  int64_t scratch_elems = 0;
+};
+
+struct TermDev {
+  int32_t nbody;
+  int32_t cl[4];
+  int32_t dims[4];
+  int32_t pad;
+  int64_t ioff;
+};
+
+struct TaskDev {
+  int32_t term;
+  int32_t sign;
+  int32_t blk[4];
+};
+
+struct fcg_ops {
+  fcg_ctx *ctx = nullptr;
+  int N = 0, S = 0, n_ops = 0;
+  std::vector<int32_t> gblock;   // [N][n_ops][S*S][S*S]
+  std::vector<int64_t> gb_off;
+  std::vector<int32_t> gb_dims;  // [n_blocks][3]
+  std::vector<int32_t> nstates;  // [N][S*S] from the "H" blocks
+  double *d_gdata = nullptr;
+  int64_t *d_gb_off = nullptr;
+  int32_t *d_gb_dims = nullptr;
+  int32_t block(int c, int op, int code_b, int code_k) const {
+    int SS = S * S;
+    return gblock[(((int64_t)c * n_ops + op) * SS + code_b) * SS + code_k];
+  }
+};
+
+struct fcg_ham {
+  fcg_ctx *ctx = nullptr;
+  int N = 0, n_terms = 0, n_tc = 0;
+  std::vector<int32_t> nbody, clusters, ops, parity, dims, tc_start;
+  std::vector<int16_t> delta, tc;
+  std::vector<int64_t> ioff;
+  TermDev *d_terms = nullptr;
+  double *d_ints = nullptr;
+};
+
+struct fcg_space {
+  fcg_ctx *ctx = nullptr;
+  int N = 0;
+  bool tpsci = false;
+  int64_t n_fock = 0, n_box = 0, nvar = 0;
+  std::vector<int16_t> fock;                  // [n_fock][N][2]
+  std::vector<int32_t> box_fock;              // [n_box]
+  std::vector<int16_t> lo, hi, rank;          // [n_box][N] (1-based ranges)
+  std::vector<int64_t> foff;                  // [n_box][N] (-1 = identity)
+  std::vector<int64_t> var_off;               // [n_box+1]
+  std::vector<double> factors;
+  int16_t *d_lo = nullptr, *d_hi = nullptr, *d_rank = nullptr;
+  int64_t *d_foff = nullptr, *d_var = nullptr;
+  int32_t *d_dim = nullptr;
+  double *d_factors = nullptr;
+};
+
+struct fcg_plan {
+  fcg_ctx *ctx = nullptr;
+  fcg_space *sp = nullptr;
+  int64_t n_pair = 0, n_task = 0, n_kel = 0;
+  double build_flops = 0.0, build_us = 0.0;
+  int precisions = 1;
+  // device
+  int64_t *d_row_ptr = nullptr;   // [n_box+1] pairs of each bra box
+  int32_t *d_pair_B = nullptr, *d_pair_K = nullptr;
+  int64_t *d_pair_t0 = nullptr;   // [n_pair+1]
+  int64_t *d_pair_koff = nullptr; // [n_pair]
+  TaskDev *d_tasks = nullptr;
+  int64_t *d_diag_koff = nullptr; // [n_box] offset of (B,B) block or -1
+  int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
+  int64_t *d_col_ptr = nullptr;   // [n_box+1] columns of each bra row
+  int32_t *d_colidx = nullptr;    // [n_cols] column -> variable index
+  int64_t n_cols = 0;
+  double *d_K64 = nullptr;
+  float *d_K32 = nullptr;
+  uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
+  float *d_c32 = nullptr;         // FP32 copy of c for reduced precision
+  int64_t c32_elems = 0;
+};
+
+// host plan construction (fcg_plan_host.cpp)
+struct HostPlan {
+  std::vector<int32_t> pair_B, pair_K;
+  std::vector<int64_t> pair_t0, pair_koff, row_ptr, diag_koff, row_koff, col_ptr;
+  std::vector<int32_t> colidx;
+  std::vector<TaskDev> tasks;
+  int64_t n_kel = 0;
+  double flops = 0.0;
+};
+int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp, int nbody,
+                        HostPlan *out, std::string *err);
+int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
+                    const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
+                    fcg_space *sp, std::string *err);
+
+// kernels (fcg_kernels.hip)
+hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
+                            hipStream_t s);
+hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
+hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
+                            int precision, hipStream_t s);
+void fcg_blas_destroy(fcg_ctx *ctx);  // fcg_davidson.hip
+hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s);

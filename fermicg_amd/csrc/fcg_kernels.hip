@@ -1,0 +1,429 @@
+// Device kernels of libfcg_hip.so (gfx950, wave64).
+//
+// k_build_blocks  one workgroup per (bra box, ket box) pair: builds the dense
+//                 Hamiltonian block sum_t s_t op_t (x) S_inactive in FP64.
+//                 op_t is the reference's build_dense_H_term chain
+//                 (src/tucker_build_dense_H_term.jl:19-172: compress each
+//                 active cluster's Γ with the Tucker factors, then contract
+//                 the integrals one cluster at a time), S are the inactive
+//                 clusters' factor overlaps U_bra' U_ket
+//                 (src/tucker_contract_dense_H_with_state.jl:82-158), or, with
+//                 identity factors, a Kronecker delta of the absolute state
+//                 indices (the inactive-state test of check_term).  With
+//                 identity factors (TPSCI) every element equals the
+//                 reference's contract_matrix_element (src/tpsci_inner.jl:
+//                 10-216).
+// k_apply         sigma = H c: one wavefront per bra box streams that box's
+//                 row of blocks, stored back to back as one column-major
+//                 panel (db x sum dk), with a column -> variable gather of c.
+//                 HBM-bound; owner-computes with a fixed-order butterfly
+//                 reduction, so results are bit-reproducible.
+#include "fcg_internal.h"
+
+namespace {
+
+struct BuildArgs {
+  const int32_t *pair_B, *pair_K;
+  const int64_t *pair_t0, *pair_koff;
+  const TaskDev *tasks;
+  const TermDev *terms;
+  const double *ints;
+  const int64_t *gb_off;
+  const int32_t *gb_dims;
+  const double *gdata;
+  const int16_t *lo, *hi, *rank;
+  const int64_t *foff;
+  const double *factors;
+  int N;
+  int64_t pair_base;
+  double *K;
+};
+
+__device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int x, int i) {
+  return off < 0 ? (x == i ? 1.0 : 0.0) : F[off + x + (int64_t)nrg * i];
+}
+
+constexpr int BT = 256;                       // threads of the build kernel
+constexpr int KPT = FCG_MAX_BOX * FCG_MAX_BOX / BT;  // block elements per thread
+
+__global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
+  __shared__ double sX[2][FCG_LDS_X];
+  __shared__ double sG[FCG_LDS_X];
+  __shared__ double sT[FCG_LDS_T];
+  __shared__ double sS[FCG_LDS_S];
+  __shared__ int sLoB[FCG_MAXN], sLoK[FCG_MAXN], sNB[FCG_MAXN], sNK[FCG_MAXN];
+  __shared__ int sRB[FCG_MAXN], sRK[FCG_MAXN], sKind[FCG_MAXN], sSoff[FCG_MAXN];
+  __shared__ int64_t sFB[FCG_MAXN], sFK[FCG_MAXN];
+  __shared__ int sAct[FCG_MAXN];
+
+  const int tid = threadIdx.x;
+  const int64_t p = a.pair_base + blockIdx.x;
+  const int64_t B = a.pair_B[p], K = a.pair_K[p];
+  const int N = a.N;
+  if (tid < N) {
+    const int c = tid;
+    sLoB[c] = a.lo[B * N + c] - 1;
+    sLoK[c] = a.lo[K * N + c] - 1;
+    sNB[c] = a.hi[B * N + c] - a.lo[B * N + c] + 1;
+    sNK[c] = a.hi[K * N + c] - a.lo[K * N + c] + 1;
+    sRB[c] = a.rank[B * N + c];
+    sRK[c] = a.rank[K * N + c];
+    sFB[c] = a.foff[B * N + c];
+    sFK[c] = a.foff[K * N + c];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int off = 0;
+    for (int c = 0; c < N; c++) {
+      const bool same = sLoB[c] == sLoK[c] && sNB[c] == sNK[c];
+      if (sFB[c] < 0 && sFK[c] < 0) sKind[c] = 0;  // identity: equal states
+      else if (same) { sKind[c] = 1; sSoff[c] = off; off += sRB[c] * sRK[c]; }
+      else sKind[c] = 2;
+    }
+  }
+  __syncthreads();
+  int db = 1, dk = 1;
+  for (int c = 0; c < N; c++) { db *= sRB[c]; dk *= sRK[c]; }
+  // overlaps of inactive-capable clusters: S = U_bra' U_ket
+  for (int c = 0; c < N; c++) {
+    if (sKind[c] != 1) continue;
+    const int rb = sRB[c], rk = sRK[c], nrg = sNB[c];
+    for (int e = tid; e < rb * rk; e += BT) {
+      const int i = e % rb, j = e / rb;
+      double s = 0.0;
+      for (int x = 0; x < nrg; x++)
+        s += fac(a.factors, sFB[c], nrg, x, i) * fac(a.factors, sFK[c], nrg, x, j);
+      sS[sSoff[c] + e] = s;
+    }
+  }
+  double acc[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; k++) acc[k] = 0.0;
+  __syncthreads();
+
+  for (int64_t t = a.pair_t0[p]; t < a.pair_t0[p + 1]; t++) {
+    const TaskDev td = a.tasks[t];
+    const TermDev tm = a.terms[td.term];
+    const int nb = tm.nbody;
+    double *cur = sX[0], *nxt = sX[1];
+    int rest = 1;
+    if (nb == 1) {
+      if (tid == 0) cur[0] = 1.0;
+    } else {
+      for (int s = 0; s < nb; s++) rest *= tm.dims[s];
+      for (int e = tid; e < rest; e += BT) cur[e] = a.ints[tm.ioff + e];
+    }
+    if (tid < N) sAct[tid] = -1;
+    __syncthreads();
+    if (tid < nb) sAct[tm.cl[tid]] = tid;
+    for (int s = 0; s < nb; s++) {
+      const int c = tm.cl[s];
+      const int blk = td.blk[s];
+      const int np = (nb == 1) ? 1 : tm.dims[s];
+      const int mb = a.gb_dims[3 * blk + 1];
+      const double *G = a.gdata + a.gb_off[blk];
+      const int rb = sRB[c], rk = sRK[c], lb = sLoB[c], lk = sLoK[c];
+      const int nI = sNB[c], nJ = sNK[c];
+      const int64_t fB = sFB[c], fK = sFK[c];
+      if (fB < 0 && fK < 0) {
+        for (int e = tid; e < np * rb * rk; e += BT) {
+          const int pp = e % np, ij = e / np, i = ij % rb, j = ij / rb;
+          sG[e] = G[pp + (int64_t)np * ((lb + i) + (int64_t)mb * (lk + j))];
+        }
+        __syncthreads();
+      } else {
+        int Pc = FCG_LDS_T / (nI * rk);
+        if (Pc > np) Pc = np;
+        for (int p0 = 0; p0 < np; p0 += Pc) {
+          const int pc = min(Pc, np - p0);
+          for (int e = tid; e < pc * nI * rk; e += BT) {
+            const int pp = e % pc, r2 = e / pc, I = r2 % nI, j = r2 / nI;
+            double v = 0.0;
+            const double *Gp = G + (p0 + pp) + (int64_t)np * (lb + I);
+            if (fK < 0) {
+              v = Gp[(int64_t)np * mb * (lk + j)];
+            } else {
+              for (int J = 0; J < nJ; J++)
+                v += Gp[(int64_t)np * mb * (lk + J)] * a.factors[fK + J + (int64_t)nJ * j];
+            }
+            sT[e] = v;
+          }
+          __syncthreads();
+          for (int e = tid; e < pc * rb * rk; e += BT) {
+            const int pp = e % pc, r2 = e / pc, i = r2 % rb, j = r2 / rb;
+            double v = 0.0;
+            if (fB < 0) {
+              v = sT[pp + pc * (i + nI * j)];
+            } else {
+              for (int I = 0; I < nI; I++)
+                v += a.factors[fB + I + (int64_t)nI * i] * sT[pp + pc * (I + nI * j)];
+            }
+            sG[(p0 + pp) + np * (i + rb * j)] = v;
+          }
+          __syncthreads();
+        }
+      }
+      // contract the leading index: nxt[q, P] = sum_p cur[p, q] g[p, P]
+      const int rest2 = rest / np, PP = rb * rk, osz = rest2 * PP;
+      for (int e = tid; e < osz; e += BT) {
+        const int q = e % rest2, P = e / rest2;
+        double v = 0.0;
+        for (int pp = 0; pp < np; pp++) v += cur[pp + np * q] * sG[pp + np * P];
+        nxt[e] = v;
+      }
+      __syncthreads();
+      double *tmp = cur; cur = nxt; nxt = tmp;
+      rest = osz;
+    }
+    // Kronecker accumulate into this thread's block elements
+    const double sign = (double)td.sign;
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+      const int e = tid + BT * k;
+      if (e >= db * dk) break;
+      int r = e % db, q = e / db;
+      double val = sign;
+      int opidx = 0, stride = 1;
+      for (int c = 0; c < N; c++) {
+        const int rb = sRB[c], rk = sRK[c];
+        const int ic = r % rb, jc = q % rk;
+        r /= rb; q /= rk;
+        if (sAct[c] >= 0) {
+          opidx += (ic + rb * jc) * stride;
+          stride *= rb * rk;
+        } else if (sKind[c] == 0) {
+          if (sLoB[c] + ic != sLoK[c] + jc) { val = 0.0; break; }
+        } else if (sKind[c] == 1) {
+          val *= sS[sSoff[c] + ic + rb * jc];
+        } else {
+          val = 0.0;
+          break;
+        }
+      }
+      if (val != 0.0) acc[k] += val * cur[opidx];
+    }
+    __syncthreads();
+  }
+  double *Kout = a.K + a.pair_koff[p];
+#pragma unroll
+  for (int k = 0; k < KPT; k++) {
+    const int e = tid + BT * k;
+    if (e < db * dk) Kout[e] = acc[k];
+  }
+}
+
+// ---- sigma apply ----
+template <typename TK>
+__device__ __forceinline__ float kval(TK v);
+template <>
+__device__ __forceinline__ float kval<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float kval<uint32_t>(uint32_t u) {
+  return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
+}
+
+template <int RMAX>
+__global__ __launch_bounds__(256) void k_apply_f64(int64_t n_box, const int64_t *__restrict__ var,
+                                                   const int64_t *__restrict__ row_koff,
+                                                   const int64_t *__restrict__ col_ptr,
+                                                   const int32_t *__restrict__ colidx,
+                                                   const double *__restrict__ K,
+                                                   const double *__restrict__ c, int R,
+                                                   int64_t nvar, double *__restrict__ sigma) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
+  if (b >= n_box) return;
+  const int db = (int)(var[b + 1] - var[b]);
+  int dbp = 1;
+  while (dbp < db) dbp <<= 1;
+  const int sl = 64 / dbp, i = lane & (dbp - 1), s = lane / dbp;
+  double acc[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; r++) acc[r] = 0.0;
+  const int64_t c0 = col_ptr[b], ncol = col_ptr[b + 1] - c0;
+  const double *Kr = K + row_koff[b];
+  if (i < db) {
+    for (int64_t jj = s; jj < ncol; jj += sl) {
+      const double kv = Kr[i + (int64_t)db * jj];
+      const int64_t col = colidx[c0 + jj];
+#pragma unroll
+      for (int r = 0; r < RMAX; r++)
+        if (r < R) acc[r] += kv * c[r * nvar + col];
+    }
+  }
+  for (int off = dbp; off < 64; off <<= 1)
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) acc[r] += __shfl_xor(acc[r], off, 64);
+  if (s == 0 && i < db) {
+#pragma unroll
+    for (int r = 0; r < RMAX; r++)
+      if (r < R) sigma[r * nvar + var[b] + i] = acc[r];
+  }
+}
+
+template <typename TK, int RMAX>
+__global__ __launch_bounds__(256) void k_apply_f32(int64_t n_box, const int64_t *__restrict__ var,
+                                                   const int64_t *__restrict__ row_koff,
+                                                   const int64_t *__restrict__ col_ptr,
+                                                   const int32_t *__restrict__ colidx,
+                                                   const TK *__restrict__ K,
+                                                   const float *__restrict__ c, int R,
+                                                   int64_t nvar, double *__restrict__ sigma) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
+  if (b >= n_box) return;
+  const int db = (int)(var[b + 1] - var[b]);
+  int dbp = 1;
+  while (dbp < db) dbp <<= 1;
+  const int sl = 64 / dbp, i = lane & (dbp - 1), s = lane / dbp;
+  float acc[RMAX];
+#pragma unroll
+  for (int r = 0; r < RMAX; r++) acc[r] = 0.0f;
+  const int64_t c0 = col_ptr[b], ncol = col_ptr[b + 1] - c0;
+  const TK *Kr = K + row_koff[b];
+  if (i < db) {
+    for (int64_t jj = s; jj < ncol; jj += sl) {
+      const float kv = kval<TK>(Kr[i + (int64_t)db * jj]);
+      const int64_t col = colidx[c0 + jj];
+#pragma unroll
+      for (int r = 0; r < RMAX; r++)
+        if (r < R) acc[r] = fmaf(kv, c[r * nvar + col], acc[r]);
+    }
+  }
+  for (int off = dbp; off < 64; off <<= 1)
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) acc[r] += __shfl_xor(acc[r], off, 64);
+  if (s == 0 && i < db) {
+#pragma unroll
+    for (int r = 0; r < RMAX; r++)
+      if (r < R) sigma[r * nvar + var[b] + i] = (double)acc[r];
+  }
+}
+
+__device__ __forceinline__ uint32_t bf16_rne_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u & 0xffff0000u;
+}
+
+__global__ void k_convert(int64_t n, const double *__restrict__ K64, float *__restrict__ K32,
+                          uint32_t *__restrict__ Kbf) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float f = (float)K64[e];
+    if (K32) K32[e] = f;
+    if (Kbf) {
+      const uint32_t hi = bf16_rne_bits(f);
+      const float r = f - __uint_as_float(hi);
+      const uint32_t lo = bf16_rne_bits(r);
+      Kbf[e] = hi | (lo >> 16);
+    }
+  }
+}
+
+__global__ void k_to_f32(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = (float)x[e];
+}
+
+__global__ void k_diag(int64_t n_box, const int64_t *__restrict__ var,
+                       const int64_t *__restrict__ dkoff, const double *__restrict__ K,
+                       double *__restrict__ hd) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_box) return;
+  const int64_t v0 = var[b];
+  const int db = (int)(var[b + 1] - v0);
+  for (int i = 0; i < db; i++)
+    hd[v0 + i] = dkoff[b] >= 0 ? K[dkoff[b] + i + (int64_t)db * i] : 0.0;
+}
+
+}  // namespace
+
+hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
+                            hipStream_t s) {
+  const fcg_space *sp = pl->sp;
+  BuildArgs a;
+  a.pair_B = pl->d_pair_B;
+  a.pair_K = pl->d_pair_K;
+  a.pair_t0 = pl->d_pair_t0;
+  a.pair_koff = pl->d_pair_koff;
+  a.tasks = pl->d_tasks;
+  a.terms = h->d_terms;
+  a.ints = h->d_ints;
+  a.gb_off = o->d_gb_off;
+  a.gb_dims = o->d_gb_dims;
+  a.gdata = o->d_gdata;
+  a.lo = sp->d_lo;
+  a.hi = sp->d_hi;
+  a.rank = sp->d_rank;
+  a.foff = sp->d_foff;
+  a.factors = sp->d_factors;
+  a.N = sp->N;
+  a.K = pl->d_K64;
+  const int64_t chunk = 1 << 20;
+  for (int64_t base = 0; base < pl->n_pair; base += chunk) {
+    a.pair_base = base;
+    const int64_t n = std::min(chunk, pl->n_pair - base);
+    hipLaunchKernelGGL(k_build_blocks, dim3((unsigned)n), dim3(BT), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
+  if (!pl->d_K32 && !pl->d_Kbf) return hipSuccess;
+  hipLaunchKernelGGL(k_convert, dim3(4096), dim3(256), 0, s, pl->n_kel, pl->d_K64, pl->d_K32,
+                     pl->d_Kbf);
+  return hipGetLastError();
+}
+
+template <template <int> class F>
+struct Dummy {};
+
+hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
+                            int precision, hipStream_t s) {
+  const fcg_space *sp = pl->sp;
+  const dim3 grid((unsigned)((sp->n_box + 3) / 4)), block(256);
+  if (precision == FCG_FP64) {
+#define L64(RM)                                                                              \
+  hipLaunchKernelGGL(k_apply_f64<RM>, grid, block, 0, s, sp->n_box, sp->d_var, pl->d_row_koff, \
+                     pl->d_col_ptr, pl->d_colidx, pl->d_K64, c, R, sp->nvar, sigma)
+    if (R <= 1) L64(1);
+    else if (R <= 2) L64(2);
+    else if (R <= 4) L64(4);
+    else if (R <= 8) L64(8);
+    else L64(16);
+#undef L64
+    return hipGetLastError();
+  }
+  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * sp->nvar, c,
+                     pl->d_c32);
+#define L32(TK, KP, RM)                                                                       \
+  hipLaunchKernelGGL((k_apply_f32<TK, RM>), grid, block, 0, s, sp->n_box, sp->d_var,          \
+                     pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, pl->d_c32, R, sp->nvar, \
+                     sigma)
+  if (precision == FCG_FP32) {
+    if (R <= 1) L32(float, pl->d_K32, 1);
+    else if (R <= 2) L32(float, pl->d_K32, 2);
+    else if (R <= 4) L32(float, pl->d_K32, 4);
+    else if (R <= 8) L32(float, pl->d_K32, 8);
+    else L32(float, pl->d_K32, 16);
+  } else {
+    if (R <= 1) L32(uint32_t, pl->d_Kbf, 1);
+    else if (R <= 2) L32(uint32_t, pl->d_Kbf, 2);
+    else if (R <= 4) L32(uint32_t, pl->d_Kbf, 4);
+    else if (R <= 8) L32(uint32_t, pl->d_Kbf, 8);
+    else L32(uint32_t, pl->d_Kbf, 16);
+  }
+#undef L32
+  return hipGetLastError();
+}
+
+hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s) {
+  const fcg_space *sp = pl->sp;
+  hipLaunchKernelGGL(k_diag, dim3((unsigned)((sp->n_box + 255) / 256)), dim3(256), 0, s,
+                     sp->n_box, sp->d_var, pl->d_diag_koff, pl->d_K64, hd);
+  return hipGetLastError();
+}

@@ -1,0 +1,410 @@
+// Host side of plan construction: which (bra box, ket box, term) triples
+// are connected, and the layout of the Hamiltonian blocks in HBM.
+//
+// The connection rule is the reference's check_term
+// (src/type_ClusteredTerm.jl:92-176) lifted from single configurations to
+// boxes: clusters a term does not touch must carry the same Fock sector and
+// the same state range on both sides, and the active clusters' Fock sectors
+// must differ by the term's delta, i.e. fock_ket = fock_bra - TransferConfig
+// (src/Indexing.jl:21-23, the lookup in tps_ci_matvec,
+// src/tpsci_outer.jl:479-481).  The fermionic sign is the reference's
+// compute_terms_state_sign (src/type_ClusteredTerm.jl:181-197), taken from
+// the ket Fock configuration.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+
+#include "fcg_internal.h"
+
+namespace {
+
+std::string fock_key(const int16_t *f, int N) {
+  std::string k(2 * N, '\0');
+  for (int i = 0; i < 2 * N; i++) k[i] = (char)(f[i] + 1);
+  return k;
+}
+
+int term_sign(const fcg_ham *h, int t, const int16_t *fk) {
+  int s = 0;
+  for (int a = 0; a < h->nbody[t]; a++) {
+    if (h->parity[4 * t + a] != 1) continue;
+    int ne = 0;
+    for (int c = 0; c < h->clusters[4 * t + a]; c++) ne += fk[2 * c] + fk[2 * c + 1];
+    s ^= ne & 1;
+  }
+  return s ? -1 : 1;
+}
+
+struct LocalTask {
+  int32_t B, K;
+  TaskDev t;
+};
+
+}  // namespace
+
+int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
+                    const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
+                    fcg_space *sp, std::string *err) {
+  const int S = o->S;
+  sp->N = N;
+  sp->tpsci = true;
+  sp->n_fock = n_fock;
+  sp->fock.assign(fock, fock + n_fock * 2 * N);
+  if (fock_start[0] != 0 || fock_start[n_fock] != n_cfg) {
+    *err = "fock_start must cover [0, n_cfg)";
+    return FCG_EINVAL;
+  }
+  auto push_box = [&](int32_t f, const int16_t *lo, const int16_t *hi) {
+    sp->box_fock.push_back(f);
+    int64_t d = 1;
+    for (int c = 0; c < N; c++) {
+      sp->lo.push_back(lo[c]);
+      sp->hi.push_back(hi[c]);
+      sp->rank.push_back((int16_t)(hi[c] - lo[c] + 1));
+      sp->foff.push_back(-1);
+      d *= hi[c] - lo[c] + 1;
+    }
+    sp->var_off.push_back(sp->var_off.back() + d);
+  };
+  sp->var_off.assign(1, 0);
+  std::vector<int> M(N);
+  for (int64_t f = 0; f < n_fock; f++) {
+    const int16_t *fk = fock + f * 2 * N;
+    int64_t prod = 1;
+    for (int c = 0; c < N; c++) {
+      if (fk[2 * c] < 0 || fk[2 * c + 1] < 0 || fk[2 * c] >= S || fk[2 * c + 1] >= S) {
+        *err = "Fock sector outside the cluster";
+        return FCG_EINVAL;
+      }
+      M[c] = o->nstates[(int64_t)c * S * S + fk[2 * c] * S + fk[2 * c + 1]];
+      if (M[c] <= 0) {
+        *err = "Fock block uses a sector missing from ClusterOps";
+        return FCG_EMISSING_BLOCK;
+      }
+      prod *= M[c];
+    }
+    int64_t s0 = fock_start[f], s1 = fock_start[f + 1];
+    for (int64_t i = s0; i < s1; i++)
+      for (int c = 0; c < N; c++)
+        if (cfg[i * N + c] < 1 || cfg[i * N + c] > M[c]) {
+          *err = "state index out of range";
+          return FCG_EINVAL;
+        }
+    bool full = (s1 - s0 == prod);
+    if (full) {  // configurations must enumerate the product, first cluster fastest
+      std::vector<int> idx(N, 1);
+      for (int64_t i = s0; i < s1 && full; i++) {
+        for (int c = 0; c < N; c++)
+          if (cfg[i * N + c] != idx[c]) { full = false; break; }
+        for (int c = 0; c < N; c++) {
+          if (++idx[c] <= M[c]) break;
+          idx[c] = 1;
+        }
+      }
+    }
+    int16_t lo[FCG_MAXN], hi[FCG_MAXN];
+    if (!full) {
+      for (int64_t i = s0; i < s1; i++) {
+        for (int c = 0; c < N; c++) lo[c] = hi[c] = cfg[i * N + c];
+        push_box((int32_t)f, lo, hi);
+      }
+      continue;
+    }
+    // split the full block into boxes of at most FCG_MAX_BOX configurations,
+    // each a contiguous run of the caller's order
+    int k = 0;
+    int64_t pre = 1;
+    while (k < N && pre * M[k] <= FCG_MAX_BOX) pre *= M[k++];
+    int chunk = (k < N) ? (int)(FCG_MAX_BOX / pre) : 1;
+    std::vector<int> slow(N, 1);
+    while (true) {
+      if (k < N) {
+        for (int a = 1; a <= M[k]; a += chunk) {
+          for (int c = 0; c < N; c++) {
+            if (c < k) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
+            else if (c == k) { lo[c] = (int16_t)a; hi[c] = (int16_t)std::min(M[k], a + chunk - 1); }
+            else { lo[c] = hi[c] = (int16_t)slow[c]; }
+          }
+          push_box((int32_t)f, lo, hi);
+        }
+      } else {
+        for (int c = 0; c < N; c++) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
+        push_box((int32_t)f, lo, hi);
+        break;
+      }
+      int c = k + 1;
+      for (; c < N; c++) {
+        if (++slow[c] <= M[c]) break;
+        slow[c] = 1;
+      }
+      if (c >= N) break;
+    }
+    if (sp->var_off.back() != s1) {
+      *err = "internal: box partition does not match the configuration order";
+      return FCG_EINVAL;
+    }
+  }
+  sp->n_box = (int64_t)sp->box_fock.size();
+  sp->nvar = sp->var_off.back();
+  return FCG_OK;
+}
+
+int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp, int nbody,
+                        HostPlan *out, std::string *err) {
+  const int N = sp->N, S = o->S;
+  if (h->N != N || o->N != N) {
+    *err = "cluster counts of ham/ops/space differ";
+    return FCG_EINVAL;
+  }
+  std::unordered_map<std::string, int64_t> fmap;
+  fmap.reserve(sp->n_fock * 2);
+  for (int64_t f = 0; f < sp->n_fock; f++) fmap.emplace(fock_key(&sp->fock[f * 2 * N], N), f);
+  // boxes of each Fock block
+  std::vector<std::vector<int32_t>> fboxes(sp->n_fock);
+  std::vector<int64_t> dim(sp->n_box);
+  for (int64_t b = 0; b < sp->n_box; b++) {
+    fboxes[sp->box_fock[b]].push_back((int32_t)b);
+    dim[b] = sp->var_off[b + 1] - sp->var_off[b];
+    if (dim[b] > FCG_MAX_BOX) {
+      *err = "block dimension exceeds " + std::to_string(FCG_MAX_BOX);
+      return FCG_EOVERFLOW;
+    }
+    const int16_t *F = &sp->fock[(int64_t)sp->box_fock[b] * 2 * N];
+    for (int c = 0; c < N; c++) {
+      if (F[2 * c] < 0 || F[2 * c + 1] < 0 || F[2 * c] >= S || F[2 * c + 1] >= S) {
+        *err = "Fock sector outside the cluster";
+        return FCG_EINVAL;
+      }
+      const int M = o->nstates[(int64_t)c * S * S + F[2 * c] * S + F[2 * c + 1]];
+      if (sp->hi[b * N + c] > M || sp->lo[b * N + c] < 1) {
+        *err = "box " + std::to_string(b) + " range outside the sector's " + std::to_string(M) +
+               " states on cluster " + std::to_string(c);
+        return FCG_EINVAL;
+      }
+    }
+  }
+  // terms of each TransferConfig grouped by active-cluster mask
+  struct Group { uint32_t mask; std::vector<int> terms; };
+  std::vector<std::vector<Group>> groups(h->n_tc);
+  for (int g = 0; g < h->n_tc; g++) {
+    for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++) {
+      if (h->nbody[t] > nbody) continue;
+      uint32_t m = 0;
+      for (int a = 0; a < h->nbody[t]; a++) m |= 1u << h->clusters[4 * t + a];
+      auto it = std::find_if(groups[g].begin(), groups[g].end(),
+                             [&](const Group &x) { return x.mask == m; });
+      if (it == groups[g].end()) groups[g].push_back({m, {t}});
+      else it->terms.push_back(t);
+    }
+  }
+  const int nthr = std::max(1, omp_get_max_threads());
+  std::vector<std::vector<LocalTask>> per_fock(sp->n_fock);
+  int errcode = FCG_OK;
+  std::string errmsg;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthr)
+  for (int64_t fb = 0; fb < sp->n_fock; fb++) {
+    if (errcode) continue;
+    const int16_t *F = &sp->fock[fb * 2 * N];
+    std::vector<LocalTask> loc;
+    int16_t fk[2 * FCG_MAXN];
+    for (int g = 0; g < h->n_tc && !errcode; g++) {
+      const int16_t *tc = &h->tc[(int64_t)g * 2 * N];
+      bool ok = true;
+      for (int i = 0; i < 2 * N; i++) {
+        fk[i] = F[i] - tc[i];
+        if (fk[i] < 0 || fk[i] >= S) ok = false;
+      }
+      if (!ok) continue;
+      auto it = fmap.find(fock_key(fk, N));
+      if (it == fmap.end()) continue;
+      int64_t fki = it->second;
+      for (const Group &G : groups[g]) {
+        // Γ blocks and sign per term (depend only on the Fock pair)
+        std::vector<TaskDev> tt;
+        for (int t : G.terms) {
+          TaskDev d{};
+          d.term = t;
+          d.sign = term_sign(h, t, fk);
+          for (int a = 0; a < 4; a++) d.blk[a] = -1;
+          for (int a = 0; a < h->nbody[t]; a++) {
+            int c = h->clusters[4 * t + a];
+            int blk = o->block(c, h->ops[4 * t + a], F[2 * c] * S + F[2 * c + 1],
+                               fk[2 * c] * S + fk[2 * c + 1]);
+            if (blk < 0) {
+#pragma omp critical
+              {
+                errcode = FCG_EMISSING_BLOCK;
+                errmsg = "missing Γ block for term " + std::to_string(t) + " cluster " +
+                         std::to_string(c);
+              }
+            } else {
+              const int32_t *gd = &o->gb_dims[3 * (int64_t)blk];
+              const int np_t = h->nbody[t] == 1 ? 1 : h->dims[4 * t + a];
+              const int Mb = o->nstates[(int64_t)c * S * S + F[2 * c] * S + F[2 * c + 1]];
+              const int Mk = o->nstates[(int64_t)c * S * S + fk[2 * c] * S + fk[2 * c + 1]];
+              if (gd[0] != np_t || gd[1] != Mb || gd[2] != Mk) {
+#pragma omp critical
+                {
+                  errcode = FCG_EINVAL;
+                  errmsg = "Γ block shape disagrees with term " + std::to_string(t) +
+                           " / sector sizes on cluster " + std::to_string(c);
+                }
+              }
+            }
+            d.blk[a] = blk;
+          }
+          tt.push_back(d);
+        }
+        for (int32_t B : fboxes[fb])
+          for (int32_t K : fboxes[fki]) {
+            // inactive clusters: equal TuckerConfig range (reference
+            // check_term); with identity factors on both sides (TPSCI boxes)
+            // overlapping ranges suffice -- the states must then be equal,
+            // which the block builder enforces element by element
+            bool eq = true;
+            for (int c = 0; c < N && eq; c++) {
+              if (G.mask >> c & 1u) continue;
+              const int64_t ib = (int64_t)B * N + c, ik = (int64_t)K * N + c;
+              if (sp->foff[ib] < 0 && sp->foff[ik] < 0)
+                eq = std::max(sp->lo[ib], sp->lo[ik]) <= std::min(sp->hi[ib], sp->hi[ik]);
+              else
+                eq = sp->lo[ib] == sp->lo[ik] && sp->hi[ib] == sp->hi[ik];
+            }
+            if (!eq) continue;
+            for (const TaskDev &d : tt) loc.push_back({B, K, d});
+          }
+      }
+    }
+    std::stable_sort(loc.begin(), loc.end(), [](const LocalTask &a, const LocalTask &b) {
+      return a.B != b.B ? a.B < b.B : a.K < b.K;
+    });
+    per_fock[fb].swap(loc);
+  }
+  if (errcode) {
+    *err = errmsg;
+    return errcode;
+  }
+  // assemble pairs ordered by bra box then ket box
+  std::vector<int64_t> order(sp->n_fock);
+  std::iota(order.begin(), order.end(), 0);
+  out->pair_B.clear(); out->pair_K.clear(); out->pair_t0.clear(); out->tasks.clear();
+  std::vector<int64_t> cnt(sp->n_box + 1, 0);
+  size_t ntask = 0;
+  for (auto &v : per_fock) ntask += v.size();
+  out->tasks.reserve(ntask);
+  struct PairRec { int32_t B, K; int64_t t0, t1; };
+  std::vector<PairRec> pairs;
+  for (int64_t fb = 0; fb < sp->n_fock; fb++) {
+    auto &v = per_fock[fb];
+    size_t i = 0;
+    while (i < v.size()) {
+      size_t j = i;
+      int64_t t0 = (int64_t)out->tasks.size();
+      while (j < v.size() && v[j].B == v[i].B && v[j].K == v[i].K) out->tasks.push_back(v[j++].t);
+      pairs.push_back({v[i].B, v[i].K, t0, (int64_t)out->tasks.size()});
+      i = j;
+    }
+    std::vector<LocalTask>().swap(v);
+  }
+  std::stable_sort(pairs.begin(), pairs.end(), [](const PairRec &a, const PairRec &b) {
+    return a.B != b.B ? a.B < b.B : a.K < b.K;
+  });
+  // tasks must be contiguous in pair order: rebuild the task array
+  std::vector<TaskDev> tasks2;
+  tasks2.reserve(out->tasks.size());
+  const int64_t np = (int64_t)pairs.size();
+  out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.resize(np + 1);
+  out->pair_koff.resize(np);
+  out->row_ptr.assign(sp->n_box + 1, 0);
+  out->diag_koff.assign(sp->n_box, -1);
+  int64_t koff = 0;
+  double flops = 0.0;
+  for (int64_t p = 0; p < np; p++) {
+    const PairRec &r = pairs[p];
+    out->pair_B[p] = r.B;
+    out->pair_K[p] = r.K;
+    out->pair_t0[p] = (int64_t)tasks2.size();
+    for (int64_t t = r.t0; t < r.t1; t++) tasks2.push_back(out->tasks[t]);
+    out->pair_koff[p] = koff;
+    if (r.B == r.K) out->diag_koff[r.B] = koff;
+    out->row_ptr[r.B + 1]++;
+    const int64_t db = dim[r.B], dk = dim[r.K];
+    koff += db * dk;
+    // limits of the block-build kernel and its flop count
+    int64_t ssum = 0;
+    for (int c = 0; c < N; c++) {
+      int64_t B = r.B, K = r.K;
+      if (sp->lo[B * N + c] == sp->lo[K * N + c] && sp->hi[B * N + c] == sp->hi[K * N + c] &&
+          !(sp->foff[B * N + c] < 0 && sp->foff[K * N + c] < 0))
+        ssum += (int64_t)sp->rank[B * N + c] * sp->rank[K * N + c];
+    }
+    if (ssum > FCG_LDS_S) {
+      *err = "inactive-cluster overlaps exceed LDS budget";
+      return FCG_EOVERFLOW;
+    }
+    for (int64_t t = r.t0; t < r.t1; t++) {
+      const TaskDev &d = out->tasks[t];
+      const int tm = d.term, nb = h->nbody[tm];
+      int64_t rest = 1;
+      for (int a = 0; a < nb; a++) rest *= (nb == 1) ? 1 : h->dims[4 * tm + a];
+      if (rest > FCG_LDS_X) {
+        *err = "term integrals exceed LDS budget";
+        return FCG_EOVERFLOW;
+      }
+      for (int a = 0; a < nb; a++) {
+        int c = h->clusters[4 * tm + a];
+        int64_t npd = (nb == 1) ? 1 : h->dims[4 * tm + a];
+        int64_t rb = sp->rank[(int64_t)r.B * N + c], rk = sp->rank[(int64_t)r.K * N + c];
+        int64_t nI = sp->hi[(int64_t)r.B * N + c] - sp->lo[(int64_t)r.B * N + c] + 1;
+        int64_t nJ = sp->hi[(int64_t)r.K * N + c] - sp->lo[(int64_t)r.K * N + c] + 1;
+        if (npd * rb * rk > FCG_LDS_X || nI * rk > FCG_LDS_T) {
+          *err = "Γ slice exceeds LDS budget";
+          return FCG_EOVERFLOW;
+        }
+        bool ident = sp->foff[(int64_t)r.B * N + c] < 0 && sp->foff[(int64_t)r.K * N + c] < 0;
+        if (!ident) flops += 2.0 * npd * (double)(nI * nJ * rk + nI * rb * rk);
+        int64_t out_el = (rest / npd) * rb * rk;
+        if (out_el > FCG_LDS_X) {
+          *err = "chain intermediate exceeds LDS budget";
+          return FCG_EOVERFLOW;
+        }
+        flops += 2.0 * npd * (double)out_el;
+        rest = out_el;
+      }
+      flops += 2.0 * db * dk;
+    }
+  }
+  out->pair_t0[np] = (int64_t)tasks2.size();
+  out->tasks.swap(tasks2);
+  for (int64_t b = 0; b < sp->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
+  // row panels: the blocks of one bra row are contiguous, so the row is a
+  // column-major db x (sum dk) panel; colidx maps its columns to variables
+  out->row_koff.assign(sp->n_box, 0);
+  out->col_ptr.assign(sp->n_box + 1, 0);
+  int64_t ncol = 0;
+  for (int64_t b = 0; b < sp->n_box; b++) {
+    out->col_ptr[b] = ncol;
+    if (out->row_ptr[b + 1] > out->row_ptr[b]) out->row_koff[b] = out->pair_koff[out->row_ptr[b]];
+    for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) ncol += dim[out->pair_K[q]];
+  }
+  out->col_ptr[sp->n_box] = ncol;
+  if (sp->nvar > INT32_MAX) {
+    *err = "more than 2^31 variables";
+    return FCG_EOVERFLOW;
+  }
+  out->colidx.resize(ncol);
+  for (int64_t b = 0; b < sp->n_box; b++) {
+    int64_t at = out->col_ptr[b];
+    for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) {
+      const int32_t K = out->pair_K[q];
+      for (int64_t j = 0; j < dim[K]; j++) out->colidx[at++] = (int32_t)(sp->var_off[K] + j);
+    }
+  }
+  out->n_kel = koff;
+  out->flops = flops;
+  return FCG_OK;
+}

@@ -1,0 +1,162 @@
+/* fcg.h -- C ABI of the MI355X sigma-build library (libfcg_hip.so).
+ *
+ * Drop-in boundary for FermiCG's sigma-build hot path.  Julia host code
+ * would ccall these entry points from the same methods the reference
+ * exposes; the Python package fermicg_amd binds them with ctypes.
+ *
+ * Conventions
+ *   - return codes: 0 ok, negative FCG_E* on error; fcg_last_error(ctx)
+ *     returns a message.  Nothing throws or aborts across the ABI.
+ *   - host pointers are read during the call only; outputs are
+ *     caller-allocated host memory unless the name ends in _device.
+ *   - handles are device-resident and owned by the library; free them with
+ *     fcg_destroy_*; destroy dependants (plans) before their inputs.
+ *   - one fcg_ctx_t per host thread; calls on one ctx are serialised on its
+ *     HIP stream.
+ *   - layouts are column-major: Γ blocks (np, m_bra, m_ket), integrals as
+ *     the ClusteredTerm ints array, vectors (n_var x R) with element (i,r)
+ *     at r*n_var + i.  State indices and ranges are 1-based (Julia).
+ *   - sector code of a cluster Fock sector (na, nb) is na*S + nb.
+ */
+#ifndef FCG_H
+#define FCG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FCG_OK = 0,
+  FCG_EINVAL = -1,         /* bad argument or inconsistent shapes */
+  FCG_ENOMEM = -2,         /* device allocation failed */
+  FCG_EMISSING_BLOCK = -3, /* a term needs a Γ block that was not uploaded */
+  FCG_EOVERFLOW = -4,      /* a size exceeds a library limit */
+  FCG_EDEVICE = -5,        /* HIP runtime error */
+  FCG_ENOTCONV = -6        /* Davidson did not converge within max_iter */
+};
+
+enum { FCG_FP64 = 0, FCG_FP32 = 1, FCG_BF16X2 = 2 };
+
+typedef struct fcg_ctx *fcg_ctx_t;
+typedef struct fcg_ops *fcg_ops_t;
+typedef struct fcg_ham *fcg_ham_t;
+typedef struct fcg_space *fcg_space_t;
+typedef struct fcg_plan *fcg_plan_t;
+
+/* Context: device ordinal, its stream and workspace. */
+int fcg_create(int device, fcg_ctx_t *out);
+void fcg_destroy(fcg_ctx_t ctx);
+const char *fcg_last_error(fcg_ctx_t ctx);
+
+/* ClusterOps of all clusters (replaces Vector{ClusterOps} as built by
+ * compute_cluster_ops, reference src/build_local_quantities.jl:40-191, and
+ * read by contract_matrix_element, src/tpsci_inner.jl:10-216).
+ * gblock[c][op][code_bra][code_ket] = block id or -1; block b has dims
+ * gb_dims[3b..3b+2] = (np, m_bra, m_ket) and data at gdata[gb_off[b]]. */
+int fcg_ops_upload(fcg_ctx_t ctx, int n_clusters, int S, int n_ops, const int32_t *gblock,
+                   int64_t n_blocks, const int64_t *gb_off, const int32_t *gb_dims,
+                   const double *gdata, int64_t n_gdata, fcg_ops_t *out);
+void fcg_ops_destroy(fcg_ops_t ops);
+
+/* ClusteredOperator (replaces ClusteredOperator.trans with ClusteredTerm1B-4B,
+ * reference src/type_ClusteredOperator.jl:5-8, src/type_ClusteredTerm.jl:31-67).
+ * Terms ordered by TransferConfig group: group g holds terms
+ * [tc_start[g], tc_start[g+1]); tc[g] is its (Δnα,Δnβ) per cluster.
+ * Per term t: nbody[t]; clusters/ops/parity/dims [4t..4t+3] (0-based
+ * clusters, -1 padding); delta[8t..8t+7]; integrals at ints[ioff[t]]. */
+int fcg_ham_upload(fcg_ctx_t ctx, int n_clusters, int n_terms, const int32_t *nbody,
+                   const int32_t *clusters, const int32_t *ops, const int32_t *parity,
+                   const int32_t *dims, const int16_t *delta, const int64_t *ioff,
+                   const double *ints, int64_t n_ints, int n_tc, const int16_t *tc,
+                   const int32_t *tc_start, fcg_ham_t *out);
+void fcg_ham_destroy(fcg_ham_t ham);
+
+/* TPSCIstate basis (replaces TPSCIstate.data keys, reference
+ * src/type_TPSCIstate.jl:11-14; flattening order of get_vector :139-149).
+ * fock: [n_fock][N][2]; configurations of Fock block f are
+ * cfg[fock_start[f] .. fock_start[f+1]) with 1-based int16 state indices
+ * cfg[i*N + c].  The library keeps the caller's order. */
+int fcg_tpsci_space_upload(fcg_ctx_t ctx, int n_clusters, int64_t n_fock, const int16_t *fock,
+                           const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
+                           fcg_space_t *out);
+
+/* BSTstate basis (replaces BSTstate.data keys and Tucker factors, reference
+ * src/type_BSTstate.jl:13-18, src/hosvd.jl:15-19; flattening order of
+ * get_vector src/type_BSTstate.jl:499-515).  Block b: Fock block
+ * box_fock[b]; per cluster c the TuckerConfig range
+ * box_lo[b*N+c]..box_hi[b*N+c] (1-based, inclusive), Tucker rank
+ * box_rank[b*N+c] and factor (range x rank, column-major) at
+ * factors[box_foff[b*N+c]] or identity when box_foff is -1 (then rank must
+ * equal the range length).  Core b occupies variables
+ * [var_off, var_off + prod(rank)) in blocks' order, column-major. */
+int fcg_bst_space_upload(fcg_ctx_t ctx, int n_clusters, int64_t n_fock, const int16_t *fock,
+                         int64_t n_box, const int32_t *box_fock, const int16_t *box_lo,
+                         const int16_t *box_hi, const int16_t *box_rank, const int64_t *box_foff,
+                         const double *factors, int64_t n_factors, fcg_space_t *out);
+int64_t fcg_space_nvar(fcg_space_t sp);
+void fcg_space_destroy(fcg_space_t sp);
+
+/* Sigma plan: connects the space with itself through the Hamiltonian and
+ * builds every Hamiltonian block in the space's basis once (replaces
+ * cache_hamiltonian, src/tucker_inner.jl:90-159, and the term loop of
+ * tps_ci_matvec, src/tpsci_outer.jl:462-527).  nbody (1..4) drops terms on
+ * more clusters (build_sigma! nbody keyword).  precisions: bitmask of
+ * (1<<FCG_FP64)|(1<<FCG_FP32)|(1<<FCG_BF16X2) copies kept resident. */
+int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp, int nbody,
+                    int precisions, fcg_plan_t *out);
+void fcg_plan_destroy(fcg_plan_t plan);
+/* Plan statistics: [0] block pairs, [1] (term, block pair) tasks,
+ * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us). */
+int fcg_plan_stats(fcg_plan_t plan, double *out5);
+
+/* sigma = H c (replaces tps_ci_matvec, src/tpsci_outer.jl:462-527, and
+ * build_sigma!, src/tucker_inner.jl:165-245).  c, sigma: host (n_var x R). */
+int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *c, double *sigma,
+              int precision);
+/* Same with device-resident FP64 operands (n_var x R column-major). */
+int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *c_dev,
+                     double *sigma_dev, int precision);
+
+/* Diagonal of H in the space's basis (replaces compute_diagonal,
+ * src/tpsci_outer.jl:694-715). */
+int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t plan, double *hd);
+
+/* Davidson for the lowest R eigenpairs (replaces tps_ci_davidson,
+ * src/tpsci_outer.jl:364-453, and ci_solve, src/tucker_outer.jl:68-215,
+ * whose eigensolver is the external BlockDavidson package).  precision
+ * FCG_FP64 runs every sigma in FP64; FCG_FP32 / FCG_BF16X2 run the
+ * expansion sigmas in that precision and recompute the Ritz vectors'
+ * sigma and residual in FP64 every refresh_k iterations, converging on the
+ * FP64 residual.  v0: host (n_var x R) start vectors (orthonormalised
+ * inside); evals (R), evecs (n_var x R) host outputs. */
+typedef struct {
+  double conv_thresh;   /* residual norm threshold (reference conv_thresh) */
+  int max_iter;         /* reference max_iter */
+  int max_ss_vecs;      /* subspace vectors per root before restart */
+  double lindep_thresh; /* drop expansion vectors below this norm */
+  int precision;        /* FCG_FP64 / FCG_FP32 / FCG_BF16X2 */
+  int refresh_k;        /* FP64 refresh period for mixed precision */
+  int precond;          /* 1: diagonal (Davidson) preconditioner */
+} fcg_davidson_opts;
+
+typedef struct {
+  int iterations;
+  int sigma_fp64;       /* number of FP64 sigma vector products */
+  int sigma_low;        /* number of reduced-precision sigma vector products */
+  int converged;
+  double residual[16];
+} fcg_davidson_stats;
+
+int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *v0,
+                 const fcg_davidson_opts *opts, double *evals, double *evecs,
+                 fcg_davidson_stats *stats);
+
+/* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
+ * HIP events on the context's stream) and its launch count. */
+int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
