@@ -1,0 +1,211 @@
+"""Benchmark: sigma builds/s (FP64 vs mixed) and Davidson solves on one MI355X.
+
+Workload (default): the Tucker/BST sigma build of fermicg_amd.workloads.
+bst_medium -- 24 orbitals, 24 electrons, 6 clusters of 4, 16 states per Fock
+sector, Tucker blocks with ranks 1..8 over the 1,441 Fock configurations with
+<= 2 electrons transferred, 2 roots (464,878 variables).  Synthetic seeded
+integrals and random-init Tucker cores/factors; nothing is read from disk.
+
+One step = one sigma build H c over the whole space for all roots, inputs
+resident in HBM (device pointers, no PCIe in the timed region).  The headline
+`value` is the mixed-precision path (FP32 Hamiltonian blocks, FP32
+accumulation); FP64 and split-BF16 rates, Davidson solves (FP64 vs mixed with
+FP64 residual refresh) and |dE| are reported beside it.
+
+Prints ONE JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
+    ap.add_argument("--cpu-rows", type=int, default=2000)
+    ap.add_argument("--clusters", type=int, default=6, help="bst workload: number of clusters")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the sigma build would shard); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fermicg_amd import _lib
+    from fermicg_amd.marshal import flatten_ham, flatten_ops
+    from fermicg_amd.tpsci import Engine, cluster_sizes_of
+
+    t0 = time.time()
+    if args.workload == "bst":
+        from fermicg_amd.bst import cache_hamiltonian, ci_solve
+        from fermicg_amd.workloads import bst_medium
+        p, st = bst_medium(n_clusters=args.clusters)
+    else:
+        from fermicg_amd.workloads import tpsci_small
+        p, st = tpsci_small(R=1)
+    meta = dict(p.meta)
+    log("workload", meta, "built in %.1fs" % (time.time() - t0))
+
+    eng = Engine(0)
+    t1 = time.time()
+    precs = ("fp64", "fp32", "bf16x2")
+    if args.workload == "bst":
+        pl = cache_hamiltonian(st, st, p.cluster_ops, p.clustered_ham, precisions=precs, engine=eng)
+    else:
+        pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, precisions=precs)
+    ps = pl.stats()
+    log("plan", ps, "total %.1fs" % (time.time() - t1))
+
+    n = pl.nvar
+    R = st.coeffs.shape[1]
+    dev = torch.device("cuda:0")
+    c_dev = torch.from_numpy(np.ascontiguousarray(st.coeffs.T)).to(dev)
+    s_dev = torch.empty_like(c_dev)
+    torch.cuda.synchronize()
+
+    def run_steps(prec, k):
+        for _ in range(k):
+            pl.sigma_device(c_dev.data_ptr(), s_dev.data_ptr(), R, prec)
+
+    rates = {}
+    kernel_ms = {}
+    for prec in precs:
+        run_steps(prec, args.warmup)
+        eng.ctx.synchronize()
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        eng.ctx.event(0)
+        run_steps(prec, args.steps)
+        eng.ctx.event(1)
+        eng.ctx.synchronize()
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        ms_wall = (tb - ta) * 1e3 / args.steps
+        ms_dev = eng.ctx.elapsed_ms(0, 1) / args.steps
+        # apply-kernel-only duration (events around the apply launch)
+        pl.sigma_device(c_dev.data_ptr(), s_dev.data_ptr(), R, prec)
+        k_ms = pl.last_sigma_ms()
+        rates[prec] = 1e3 / ms_wall
+        kernel_ms[prec] = {"step_ms": ms_wall, "stream_ms_per_step": ms_dev, "apply_kernel_ms": k_ms}
+        log(prec, kernel_ms[prec])
+
+    # parity on sampled bra blocks against the CPU oracle (and CPU baseline timing)
+    parity, cpu = {}, None
+    sig = {}
+    for prec in precs:
+        pl.sigma_device(c_dev.data_ptr(), s_dev.data_ptr(), R, prec)
+        eng.ctx.synchronize()
+        sig[prec] = s_dev.cpu().numpy().T.copy()
+    if not args.no_cpu:
+        from oracle.pyoracle import BstOracle, Oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        fo = flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops))
+        fh = flatten_ham(p.clustered_ham)
+        rng = np.random.Generator(np.random.PCG64(12345))
+        if args.workload == "bst":
+            orc = BstOracle(fo, fh, st)
+            nb = len(st.blocks)
+            rows = np.sort(rng.choice(nb, min(args.cpu_rows, nb), replace=False))
+            ta = time.perf_counter()
+            ref, idx, (tbuild, tcon) = orc.sigma_rows(st.coeffs, rows, nthreads=threads, timings=True)
+            wall = time.perf_counter() - ta
+            frac = len(rows) / nb
+            # reference ci_solve iterations read cached ops (cache_hamiltonian):
+            # the per-iteration CPU cost is the contraction share of the work
+            con_wall = wall * tcon / max(tbuild + tcon, 1e-30)
+            cpu = {"value": frac / con_wall, "unit": "sigma builds/s", "cores": threads,
+                   "kind": "port",
+                   "sample": "C restatement of build_sigma! (src/tucker_inner.jl:165) on %d of %d "
+                             "bra Tucker blocks (seeded), %d OpenMP threads, %.2f s wall; value "
+                             "uses the contraction share (cached ops, as in ci_solve iterations); "
+                             "with per-term op builds: %.4f sigma/s"
+                             % (len(rows), nb, threads, wall, frac / wall)}
+        else:
+            orc = Oracle(fo, fh, st)
+            idx = np.arange(n)
+            ta = time.perf_counter()
+            ref = orc.sigma(st.coeffs, nthreads=threads)
+            wall = time.perf_counter() - ta
+            cpu = {"value": 1.0 / wall, "unit": "sigma builds/s", "cores": threads, "kind": "port",
+                   "sample": "C restatement of tps_ci_matvec, full sigma, %d threads" % threads}
+        scale = np.abs(ref).max()
+        for prec in precs:
+            parity[prec] = float(np.abs(sig[prec][idx] - ref).max() / scale)
+        log("parity", parity, "cpu", cpu)
+
+    # Davidson solves: FP64 vs mixed (FP64 residual refresh)
+    solves = {}
+    if not args.no_solve:
+        for prec in ("fp64", args.precision if args.precision != "fp64" else "fp32"):
+            ta = time.perf_counter()
+            e, v, stt = pl.davidson(st.coeffs, conv_thresh=1e-8, max_iter=300, max_ss_vecs=12,
+                                    lindep_thresh=1e-12, precision=prec, refresh_k=4)
+            dt = time.perf_counter() - ta
+            solves[prec] = {"seconds": dt, "energies": [float(x) for x in e], **stt}
+            log("solve", prec, solves[prec])
+        lo = [k for k in solves if k != "fp64"][0]
+        dE = float(np.max(np.abs(np.array(solves[lo]["energies"]) - np.array(solves["fp64"]["energies"]))))
+    else:
+        dE = None
+
+    # roofline of the dominant kernel (the sigma apply of the headline precision)
+    prec = args.precision
+    esz = 8 if prec == "fp64" else 4
+    csz = 8 if prec == "fp64" else 4
+    bytes_alg = ps["block_elems"] * esz + 4.0 * ps["panel_cols"] + n * R * (csz + 8)
+    k_ms = kernel_ms[prec]["apply_kernel_ms"]
+    achieved = bytes_alg / (k_ms * 1e-3) / 1e9
+    out = {
+        "metric": "sigma builds/s (mixed FP32 vs FP64 GPU at |dE|<=1 uEh)",
+        "value": rates[prec],
+        "unit": "sigma builds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / rates[prec],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == "fp32" else ("f64" if prec == "fp64" else "bf16x2/f32"),
+        "data": "synthetic (seeded integrals, random Tucker cores and factors)",
+        "config": {"workload": meta["workload"], "n_var": int(n), "roots": int(R),
+                   "n_fock": meta.get("n_fock"), "n_blocks": meta.get("n_blocks"),
+                   "block_pairs": ps["pairs"], "terms_x_pairs": ps["tasks"],
+                   "hamiltonian_block_elems": ps["block_elems"],
+                   "plan_build_s": ps["build_s"], "parallelism": "single GPU"},
+        "sigma_per_s": rates,
+        "kernel_ms": kernel_ms,
+        "solves": solves,
+        "solves_per_s": {k: 1.0 / v["seconds"] for k, v in solves.items()},
+        "abs_dE_mixed_vs_fp64": dE,
+        "parity_rel_vs_cpu_oracle": parity,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_apply_f32 (block-sparse H c)" if prec != "fp64" else "k_apply_f64"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -25,7 +25,7 @@ EXPORTS = [
     "fcg_ham_upload", "fcg_ham_destroy", "fcg_tpsci_space_upload", "fcg_bst_space_upload",
     "fcg_space_nvar", "fcg_space_destroy", "fcg_plan_create", "fcg_plan_destroy",
     "fcg_plan_stats", "fcg_sigma", "fcg_sigma_device", "fcg_diagonal", "fcg_davidson",
-    "fcg_last_sigma_time",
+    "fcg_last_sigma_time", "fcg_event_record", "fcg_event_elapsed", "fcg_synchronize",
 ]
 
 
@@ -90,6 +90,9 @@ def load():
     L.fcg_davidson.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(DavidsonOpts), P(c_f64),
                                P(c_f64), P(DavidsonStats)]
     L.fcg_last_sigma_time.argtypes = [VP, P(c_f64), P(ctypes.c_int)]
+    L.fcg_event_record.argtypes = [VP, ctypes.c_int]
+    L.fcg_event_elapsed.argtypes = [VP, ctypes.c_int, ctypes.c_int, P(c_f64)]
+    L.fcg_synchronize.argtypes = [VP]
     _lib = L
     return L
 
@@ -113,6 +116,17 @@ class Context:
         if rc:
             msg = self.L.fcg_last_error(self.h).decode(errors="replace")
             raise FCGError("%s: %s %s" % (what, ERRORS.get(rc, rc), msg))
+
+    def event(self, slot):
+        self.check(self.L.fcg_event_record(self.h, slot), "fcg_event_record")
+
+    def elapsed_ms(self, a, b):
+        ms = c_f64()
+        self.check(self.L.fcg_event_elapsed(self.h, a, b, ctypes.byref(ms)), "fcg_event_elapsed")
+        return ms.value
+
+    def synchronize(self):
+        self.check(self.L.fcg_synchronize(self.h), "fcg_synchronize")
 
     def close(self):
         if getattr(self, "h", None):
@@ -221,10 +235,10 @@ class Plan(_Handle):
         return pl
 
     def stats(self):
-        out = np.zeros(5)
+        out = np.zeros(6)
         self.ctx.check(self.ctx.L.fcg_plan_stats(self.h, _p(out, c_f64)), "fcg_plan_stats")
         return {"pairs": int(out[0]), "tasks": int(out[1]), "block_elems": int(out[2]),
-                "build_flops": out[3], "build_s": out[4] * 1e-6}
+                "build_flops": out[3], "build_s": out[4] * 1e-6, "panel_cols": int(out[5])}
 
     def sigma(self, c, precision="fp64"):
         c = np.asarray(c, dtype=np.float64)

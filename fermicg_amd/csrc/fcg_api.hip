@@ -94,6 +94,8 @@ void fcg_destroy(fcg_ctx_t ctx) {
   dev_free(ctx->d_c);
   dev_free(ctx->d_s);
   fcg_blas_destroy(ctx);
+  for (auto &e : ctx->slots)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -404,13 +406,14 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   delete pl;
 }
 
-int fcg_plan_stats(fcg_plan_t pl, double *o5) {
-  if (!pl || !o5) return FCG_EINVAL;
-  o5[0] = (double)pl->n_pair;
-  o5[1] = (double)pl->n_task;
-  o5[2] = (double)pl->n_kel;
-  o5[3] = pl->build_flops;
-  o5[4] = pl->build_us;
+int fcg_plan_stats(fcg_plan_t pl, double *o6) {
+  if (!pl || !o6) return FCG_EINVAL;
+  o6[0] = (double)pl->n_pair;
+  o6[1] = (double)pl->n_task;
+  o6[2] = (double)pl->n_kel;
+  o6[3] = pl->build_flops;
+  o6[4] = pl->build_us;
+  o6[5] = (double)pl->n_cols;
   return FCG_OK;
 }
 
@@ -429,8 +432,7 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
     TRY(dev_alloc(ctx, &pl->d_c32, n));
     pl->c32_elems = n;
   }
-  FCG_HIP(hipEventRecord(ctx->ev0, ctx->stream), ctx);
-  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream), ctx);
+  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0), ctx);
   FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
   ctx->last_launches = precision == FCG_FP64 ? 1 : 2;
   return FCG_OK;
@@ -471,6 +473,30 @@ int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches) {
   FCG_HIP(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1), ctx);
   if (ms) *ms = t;
   if (launches) *launches = ctx->last_launches;
+  return FCG_OK;
+}
+
+int fcg_event_record(fcg_ctx_t ctx, int slot) {
+  if (!ctx || slot < 0 || slot >= 8) return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->slots[slot]) FCG_HIP(hipEventCreate(&ctx->slots[slot]), ctx);
+  FCG_HIP(hipEventRecord(ctx->slots[slot], ctx->stream), ctx);
+  return FCG_OK;
+}
+
+int fcg_event_elapsed(fcg_ctx_t ctx, int a, int b, double *ms) {
+  if (!ctx || !ms || a < 0 || a >= 8 || b < 0 || b >= 8 || !ctx->slots[a] || !ctx->slots[b])
+    return FCG_EINVAL;
+  FCG_HIP(hipEventSynchronize(ctx->slots[b]), ctx);
+  float t = 0.f;
+  FCG_HIP(hipEventElapsedTime(&t, ctx->slots[a], ctx->slots[b]), ctx);
+  *ms = t;
+  return FCG_OK;
+}
+
+int fcg_synchronize(fcg_ctx_t ctx) {
+  if (!ctx) return FCG_EINVAL;
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
   return FCG_OK;
 }
 

@@ -32,6 +32,7 @@ struct fcg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t slots[8] = {};
   std::string err;
   double last_ms = 0.0;
   int last_launches = 0;
@@ -49,11 +50,14 @@ struct TermDev {
   int64_t ioff;
 };
 
+// meta: bit 0 = negative sign; bits 8+4s.. = term position contracted at
+// chain stage s (the host picks the order that keeps intermediates small)
 struct TaskDev {
   int32_t term;
-  int32_t sign;
+  int32_t meta;
   int32_t blk[4];
 };
+__host__ __device__ inline int task_order(int32_t meta, int s) { return (meta >> (8 + 4 * s)) & 0xF; }
 
 struct fcg_ops {
   fcg_ctx *ctx = nullptr;
@@ -142,6 +146,6 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
                             hipStream_t s);
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
-                            int precision, hipStream_t s);
+                            int precision, hipStream_t s, hipEvent_t before_apply);
 void fcg_blas_destroy(fcg_ctx *ctx);  // fcg_davidson.hip
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s);

@@ -55,6 +55,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
   __shared__ int sRB[FCG_MAXN], sRK[FCG_MAXN], sKind[FCG_MAXN], sSoff[FCG_MAXN];
   __shared__ int64_t sFB[FCG_MAXN], sFK[FCG_MAXN];
   __shared__ int sAct[FCG_MAXN];
+  __shared__ int sStride[FCG_MAXN];
 
   const int tid = threadIdx.x;
   const int64_t p = a.pair_base + blockIdx.x;
@@ -106,20 +107,43 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
     const TermDev tm = a.terms[td.term];
     const int nb = tm.nbody;
     double *cur = sX[0], *nxt = sX[1];
+    int ord[4];
+    for (int s = 0; s < 4; s++) ord[s] = task_order(td.meta, s);
     int rest = 1;
     if (nb == 1) {
       if (tid == 0) cur[0] = 1.0;
     } else {
-      for (int s = 0; s < nb; s++) rest *= tm.dims[s];
-      for (int e = tid; e < rest; e += BT) cur[e] = a.ints[tm.ioff + e];
+      // integrals with their axes in chain order (stage s contracts term
+      // position ord[s]); original layout is column-major in term order
+      int ostr[4], pd[4];
+      ostr[0] = 1;
+      for (int s = 1; s < nb; s++) ostr[s] = ostr[s - 1] * tm.dims[s - 1];
+      for (int s = 0; s < nb; s++) { pd[s] = tm.dims[ord[s]]; rest *= pd[s]; }
+      for (int e = tid; e < rest; e += BT) {
+        int r = e, src = 0;
+        for (int s = 0; s < nb; s++) {
+          src += (r % pd[s]) * ostr[ord[s]];
+          r /= pd[s];
+        }
+        cur[e] = a.ints[tm.ioff + src];
+      }
     }
     if (tid < N) sAct[tid] = -1;
     __syncthreads();
-    if (tid < nb) sAct[tm.cl[tid]] = tid;
+    if (tid == 0) {
+      int st = 1;
+      for (int s = 0; s < nb; s++) {
+        const int c = tm.cl[ord[s]];
+        sAct[c] = s;
+        sStride[c] = st;
+        st *= sRB[c] * sRK[c];
+      }
+    }
     for (int s = 0; s < nb; s++) {
-      const int c = tm.cl[s];
-      const int blk = td.blk[s];
-      const int np = (nb == 1) ? 1 : tm.dims[s];
+      const int tp = ord[s];
+      const int c = tm.cl[tp];
+      const int blk = td.blk[tp];
+      const int np = (nb == 1) ? 1 : tm.dims[tp];
       const int mb = a.gb_dims[3 * blk + 1];
       const double *G = a.gdata + a.gb_off[blk];
       const int rb = sRB[c], rk = sRK[c], lb = sLoB[c], lk = sLoK[c];
@@ -176,21 +200,20 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
       rest = osz;
     }
     // Kronecker accumulate into this thread's block elements
-    const double sign = (double)td.sign;
+    const double sign = (td.meta & 1) ? -1.0 : 1.0;
 #pragma unroll
     for (int k = 0; k < KPT; k++) {
       const int e = tid + BT * k;
       if (e >= db * dk) break;
       int r = e % db, q = e / db;
       double val = sign;
-      int opidx = 0, stride = 1;
+      int opidx = 0;
       for (int c = 0; c < N; c++) {
         const int rb = sRB[c], rk = sRK[c];
         const int ic = r % rb, jc = q % rk;
         r /= rb; q /= rk;
         if (sAct[c] >= 0) {
-          opidx += (ic + rb * jc) * stride;
-          stride *= rb * rk;
+          opidx += (ic + rb * jc) * sStride[c];
         } else if (sKind[c] == 0) {
           if (sLoB[c] + ic != sLoK[c] + jc) { val = 0.0; break; }
         } else if (sKind[c] == 1) {
@@ -379,14 +402,12 @@ hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <template <int> class F>
-struct Dummy {};
-
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
-                            int precision, hipStream_t s) {
+                            int precision, hipStream_t s, hipEvent_t before_apply) {
   const fcg_space *sp = pl->sp;
   const dim3 grid((unsigned)((sp->n_box + 3) / 4)), block(256);
   if (precision == FCG_FP64) {
+    if (before_apply) (void)hipEventRecord(before_apply, s);
 #define L64(RM)                                                                              \
   hipLaunchKernelGGL(k_apply_f64<RM>, grid, block, 0, s, sp->n_box, sp->d_var, pl->d_row_koff, \
                      pl->d_col_ptr, pl->d_colidx, pl->d_K64, c, R, sp->nvar, sigma)
@@ -400,6 +421,7 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
   }
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * sp->nvar, c,
                      pl->d_c32);
+  if (before_apply) (void)hipEventRecord(before_apply, s);
 #define L32(TK, KP, RM)                                                                       \
   hipLaunchKernelGGL((k_apply_f32<TK, RM>), grid, block, 0, s, sp->n_box, sp->d_var,          \
                      pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, pl->d_c32, R, sp->nvar, \

@@ -228,7 +228,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
         for (int t : G.terms) {
           TaskDev d{};
           d.term = t;
-          d.sign = term_sign(h, t, fk);
+          d.meta = term_sign(h, t, fk) < 0 ? 1 : 0;
           for (int a = 0; a < 4; a++) d.blk[a] = -1;
           for (int a = 0; a < h->nbody[t]; a++) {
             int c = h->clusters[4 * t + a];
@@ -347,35 +347,54 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
       return FCG_EOVERFLOW;
     }
     for (int64_t t = r.t0; t < r.t1; t++) {
-      const TaskDev &d = out->tasks[t];
+      TaskDev &d = tasks2[out->pair_t0[p] + (t - r.t0)];
       const int tm = d.term, nb = h->nbody[tm];
-      int64_t rest = 1;
-      for (int a = 0; a < nb; a++) rest *= (nb == 1) ? 1 : h->dims[4 * tm + a];
-      if (rest > FCG_LDS_X) {
-        *err = "term integrals exceed LDS budget";
-        return FCG_EOVERFLOW;
-      }
+      int64_t np_[4], rr_[4];
+      int64_t rest0 = 1;
       for (int a = 0; a < nb; a++) {
-        int c = h->clusters[4 * tm + a];
-        int64_t npd = (nb == 1) ? 1 : h->dims[4 * tm + a];
-        int64_t rb = sp->rank[(int64_t)r.B * N + c], rk = sp->rank[(int64_t)r.K * N + c];
-        int64_t nI = sp->hi[(int64_t)r.B * N + c] - sp->lo[(int64_t)r.B * N + c] + 1;
-        int64_t nJ = sp->hi[(int64_t)r.K * N + c] - sp->lo[(int64_t)r.K * N + c] + 1;
-        if (npd * rb * rk > FCG_LDS_X || nI * rk > FCG_LDS_T) {
+        const int c = h->clusters[4 * tm + a];
+        np_[a] = (nb == 1) ? 1 : h->dims[4 * tm + a];
+        rr_[a] = (int64_t)sp->rank[(int64_t)r.B * N + c] * sp->rank[(int64_t)r.K * N + c];
+        rest0 *= np_[a];
+        const int64_t nI = sp->hi[(int64_t)r.B * N + c] - sp->lo[(int64_t)r.B * N + c] + 1;
+        const int64_t nJ = sp->hi[(int64_t)r.K * N + c] - sp->lo[(int64_t)r.K * N + c] + 1;
+        const int64_t rk = sp->rank[(int64_t)r.K * N + c];
+        if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) {
           *err = "Γ slice exceeds LDS budget";
           return FCG_EOVERFLOW;
         }
-        bool ident = sp->foff[(int64_t)r.B * N + c] < 0 && sp->foff[(int64_t)r.K * N + c] < 0;
-        if (!ident) flops += 2.0 * npd * (double)(nI * nJ * rk + nI * rb * rk);
-        int64_t out_el = (rest / npd) * rb * rk;
-        if (out_el > FCG_LDS_X) {
-          *err = "chain intermediate exceeds LDS budget";
-          return FCG_EOVERFLOW;
-        }
-        flops += 2.0 * npd * (double)out_el;
-        rest = out_el;
+        const bool ident = sp->foff[(int64_t)r.B * N + c] < 0 && sp->foff[(int64_t)r.K * N + c] < 0;
+        if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * sp->rank[(int64_t)r.B * N + c] * rk);
       }
-      flops += 2.0 * db * dk;
+      if (rest0 > FCG_LDS_X) {
+        *err = "term integrals exceed LDS budget";
+        return FCG_EOVERFLOW;
+      }
+      // chain order: the permutation with the smallest peak intermediate
+      int perm[4] = {0, 1, 2, 3}, best[4] = {0, 1, 2, 3};
+      int64_t best_peak = INT64_MAX;
+      double best_fl = 0.0;
+      do {
+        int64_t rest = rest0, peak = rest0;
+        double fl = 0.0;
+        for (int s2 = 0; s2 < nb; s2++) {
+          const int a = perm[s2];
+          rest = rest / np_[a] * rr_[a];
+          fl += 2.0 * np_[a] * (double)rest;
+          peak = std::max(peak, rest);
+        }
+        if (peak < best_peak || (peak == best_peak && fl < best_fl)) {
+          best_peak = peak;
+          best_fl = fl;
+          std::copy(perm, perm + 4, best);
+        }
+      } while (std::next_permutation(perm, perm + nb));
+      if (best_peak > FCG_LDS_X) {
+        *err = "chain intermediate exceeds LDS budget";
+        return FCG_EOVERFLOW;
+      }
+      for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? best[s2] : 0) << (8 + 4 * s2);
+      flops += best_fl + 2.0 * db * dk;
     }
   }
   out->pair_t0[np] = (int64_t)tasks2.size();

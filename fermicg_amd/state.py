@@ -36,14 +36,38 @@ def fock_configs_within(bases, ref_fock, max_transfer):
     (first cluster fastest), as expand_to_full_space! (tpsci_outer.jl:830-857)."""
     na = sum(f[0] for f in ref_fock)
     nb = sum(f[1] for f in ref_fock)
-    secs = [cb.sectors() for cb in bases]
+    secs = [sorted(cb.sectors()) for cb in bases]
+    N = len(bases)
+    # bounds on what clusters 0..c-1 can still contribute, for pruning
+    amin = [0] * (N + 1); amax = [0] * (N + 1); bmin = [0] * (N + 1); bmax = [0] * (N + 1)
+    for c in range(N):
+        amin[c + 1] = amin[c] + min(s[0] for s in secs[c])
+        amax[c + 1] = amax[c] + max(s[0] for s in secs[c])
+        bmin[c + 1] = bmin[c] + min(s[1] for s in secs[c])
+        bmax[c + 1] = bmax[c] + max(s[1] for s in secs[c])
+    order = [cb.sectors() for cb in bases]
     out = []
-    for combo in product(*reversed(secs)):
-        f = tuple(reversed(combo))
-        if sum(x[0] for x in f) != na or sum(x[1] for x in f) != nb:
-            continue
-        if n_transferred(f, ref_fock) <= max_transfer:
-            out.append(tuple((int(a), int(b)) for a, b in f))
+    chosen = [None] * N
+
+    def rec(c, ra, rb, tr):
+        # choose cluster c (slowest first); clusters < c remain
+        if c < 0:
+            if ra == 0 and rb == 0:
+                out.append(tuple((int(a), int(b)) for a, b in chosen))
+            return
+        for s in order[c]:
+            t = tr + max(0, s[0] - ref_fock[c][0]) + max(0, s[1] - ref_fock[c][1])
+            a2, b2 = ra - s[0], rb - s[1]
+            if t > max_transfer or a2 < amin[c] or a2 > amax[c] or b2 < bmin[c] or b2 > bmax[c]:
+                continue
+            chosen[c] = s
+            rec(c - 1, a2, b2, t)
+
+    rec(N - 1, na, nb, 0)
+    # product order with the first cluster fastest
+    key = {s: i for i, s in enumerate(order[0])}
+    idx = [{s: i for i, s in enumerate(order[c])} for c in range(N)]
+    out.sort(key=lambda f: tuple(idx[c][f[c]] for c in reversed(range(N))))
     return out
 
 

@@ -58,3 +58,16 @@ def tpsci_small(seed=0, R=1):
     st.set_vector(v)
     p.meta = {"workload": "tpsci_12o_3c_M4", "n_fock": len(p.focks), "n_cfg": len(st)}
     return p, st
+
+
+def bst_medium(seed=1, R=2, n_clusters=6, max_roots=16, max_rank=8):
+    """24 orbitals / 6 clusters / M=16 / Tucker P-Q blocks with <= 2 Q
+    clusters and ranks <= 8 over the Fock configurations with <= 2
+    electrons transferred."""
+    from .bst import bst_pq_state
+    p = build_problem(4 * n_clusters, n_clusters, max_roots, 8 * n_clusters, seed)
+    st = bst_pq_state(p.bases, p.focks, p.ref_fock, max_q=2, max_rank=max_rank, R=R,
+                      seed=seed + 2000)
+    p.meta = {"workload": "bst_%do_%dc_M%d_r%d" % (4 * n_clusters, n_clusters, max_roots, max_rank),
+              "n_fock": len(p.focks), "n_blocks": len(st.blocks), "n_var": len(st)}
+    return p, st

@@ -107,8 +107,9 @@ int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp,
                     int precisions, fcg_plan_t *out);
 void fcg_plan_destroy(fcg_plan_t plan);
 /* Plan statistics: [0] block pairs, [1] (term, block pair) tasks,
- * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us). */
-int fcg_plan_stats(fcg_plan_t plan, double *out5);
+ * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us),
+ * [5] panel columns (column-index entries read by one sigma). */
+int fcg_plan_stats(fcg_plan_t plan, double *out6);
 
 /* sigma = H c (replaces tps_ci_matvec, src/tpsci_outer.jl:462-527, and
  * build_sigma!, src/tucker_inner.jl:165-245).  c, sigma: host (n_var x R). */
@@ -155,6 +156,14 @@ int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *v0,
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */
 int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches);
+
+/* Stream-ordered timing on the context's stream: record event `slot`
+ * (0..7), then read the time between two recorded slots (waits for the
+ * later one).  No counterpart in the reference (its timers are host-side
+ * @time / TimerOutputs sections, e.g. src/tucker_outer.jl:135,185). */
+int fcg_event_record(fcg_ctx_t ctx, int slot);
+int fcg_event_elapsed(fcg_ctx_t ctx, int slot_a, int slot_b, double *ms);
+int fcg_synchronize(fcg_ctx_t ctx);
 
 #ifdef __cplusplus
 }

@@ -64,3 +64,34 @@ int orc_tpsci_sigma_rows(const orc_ops *ops, const orc_ham *h, const orc_space *
                          double *sigma_rows, int nthreads);
 
 #endif
+
+#ifndef FCG_ORACLE_BST
+#define FCG_ORACLE_BST
+/* BST blocks (Tucker): block b lives in Fock block box_fock[b]; per cluster
+ * c the TuckerConfig range lo..hi (1-based inclusive), rank, and factor
+ * (range x rank column-major) at factors[foff] or identity when foff < 0.
+ * Cores are concatenated in block order (var_off[b]..var_off[b+1]). */
+typedef struct {
+  int64_t n_fock;
+  const int16_t *fock;
+  int64_t n_box;
+  const int32_t *box_fock;
+  const int16_t *lo, *hi, *rank;
+  const int64_t *foff;
+  const double *factors;
+  const int64_t *var_off;
+} orc_bst;
+
+/* Reference build_sigma! (src/tucker_inner.jl:165-245) restricted to the
+ * bra blocks in `rows`: for each ket block of each connected Fock block and
+ * each term passing check_term, build_dense_H_term
+ * (src/tucker_build_dense_H_term.jl:5-174) then contract_dense_H_with_state
+ * (src/tucker_contract_dense_H_with_state.jl:4-329).  out holds the rows'
+ * cores back to back, column-major over roots (R columns of length
+ * sum(dim(rows))).  t_build/t_contract (seconds, summed over threads)
+ * split the work as the reference's cache_hamiltonian vs. its Davidson
+ * iterations. */
+int orc_bst_sigma_rows(const orc_ops *ops, const orc_ham *h, const orc_bst *b, int R,
+                       const double *c, int64_t nvar, const int64_t *rows, int64_t n_rows,
+                       double *out, int nthreads, double *t_build, double *t_contract);
+#endif

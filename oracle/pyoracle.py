@@ -129,3 +129,71 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle diagonal failed: %d" % rc)
         return hd
+
+
+class OrcBst(ctypes.Structure):
+    _fields_ = [("n_fock", i64), ("fock", P(i16)), ("n_box", i64), ("box_fock", P(i32)),
+                ("lo", P(i16)), ("hi", P(i16)), ("rank", P(i16)), ("foff", P(i64)),
+                ("factors", P(f64)), ("var_off", P(i64))]
+
+
+class BstOracle(Oracle):
+    """Reference build_sigma! restated in C, on a fermicg_amd.bst.BSTstate."""
+
+    def __init__(self, fops, fham, bst):
+        self._keep = []
+
+        def k(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        self.ops = OrcOps(fops.n_clusters, fops.S, fops.n_ops,
+                          _p(k(fops.gblock, np.int32), i32), _p(k(fops.gb_off, np.int64), i64),
+                          _p(k(fops.gb_dims, np.int32), i32), _p(k(fops.gdata, np.float64), f64))
+        self.ham = OrcHam(fham.n_terms, _p(k(fham.nbody, np.int32), i32),
+                          _p(k(fham.clusters, np.int32), i32), _p(k(fham.ops, np.int32), i32),
+                          _p(k(fham.parity, np.int32), i32), _p(k(fham.dims, np.int32), i32),
+                          _p(k(fham.delta, np.int16), i16), _p(k(fham.ioff, np.int64), i64),
+                          _p(k(fham.ints, np.float64), f64), fham.n_tc,
+                          _p(k(fham.tc, np.int16), i16), _p(k(fham.tc_start, np.int32), i32))
+        fl = bst.flat()
+        self.var_off = k(bst.var_offsets(), np.int64)
+        self.nvar = int(self.var_off[-1])
+        self.n_box = fl.n_box
+        self.bst = OrcBst(fl.n_fock, _p(k(fl.fock, np.int16), i16), fl.n_box,
+                          _p(k(fl.box_fock, np.int32), i32), _p(k(fl.lo, np.int16), i16),
+                          _p(k(fl.hi, np.int16), i16), _p(k(fl.rank, np.int16), i16),
+                          _p(k(fl.foff, np.int64), i64),
+                          _p(k(fl.factors if fl.factors.size else np.zeros(1), np.float64), f64),
+                          _p(self.var_off, i64))
+        L = lib()
+        L.orc_bst_sigma_rows.argtypes = [P(OrcOps), P(OrcHam), P(OrcBst), ctypes.c_int, P(f64),
+                                         i64, P(i64), i64, P(f64), ctypes.c_int, P(f64), P(f64)]
+
+    def sigma_rows(self, c, rows, nthreads=0, timings=False):
+        """sigma restricted to the bra blocks `rows`: returns (values, index
+        array of the variables they cover[, (t_build, t_contract)])."""
+        c = np.asarray(c, dtype=np.float64)
+        if c.ndim == 1:
+            c = c[:, None]
+        R = c.shape[1]
+        cc = np.ascontiguousarray(c.T)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        idx = np.concatenate([np.arange(self.var_off[b], self.var_off[b + 1]) for b in rows]) \
+            if len(rows) else np.zeros(0, np.int64)
+        out = np.zeros((R, len(idx)))
+        tb, tc = f64(), f64()
+        rc = lib().orc_bst_sigma_rows(ctypes.byref(self.ops), ctypes.byref(self.ham),
+                                      ctypes.byref(self.bst), R, _p(cc, f64), self.nvar,
+                                      _p(rows, i64), len(rows), _p(out, f64), nthreads,
+                                      ctypes.byref(tb), ctypes.byref(tc))
+        if rc:
+            raise RuntimeError("oracle bst sigma failed: %d" % rc)
+        if timings:
+            return out.T.copy(), idx, (tb.value, tc.value)
+        return out.T.copy(), idx
+
+    def sigma(self, c, nthreads=0):
+        s, idx = self.sigma_rows(c, np.arange(self.n_box), nthreads)
+        return s
