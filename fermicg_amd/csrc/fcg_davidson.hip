@@ -5,10 +5,12 @@
 // with the reference's options (conv_thresh on the residual norm,
 // max_ss_vecs per root, lindep_thresh, diagonal preconditioner).
 //
-// Mixed precision: expansion vectors get their sigma in FP32 or split-BF16;
-// every refresh_k iterations, and before convergence is accepted, the
-// subspace collapses to the current Ritz vectors whose sigma is recomputed in
-// FP64, so the residual that decides convergence is always an FP64 one.
+// Restarts are thick: the subspace collapses onto its lowest min(2R, 16) Ritz
+// vectors.  Mixed precision: expansion vectors get their sigma in FP32 or
+// split-BF16; every refresh_k iterations, and before convergence is
+// accepted, the subspace collapses onto those Ritz vectors and their sigma
+// is recomputed in FP64 (one launch), so the residual that decides
+// convergence is always an FP64 one.
 // Orthogonalisation (classical Gram-Schmidt, twice) and the Rayleigh-Ritz
 // projection are FP64; the small projected eigenproblem is solved on the host
 // by cyclic Jacobi.
@@ -187,13 +189,16 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
     ctx->err = "plan lacks the requested sigma precision";
     return FCG_EINVAL;
   }
-  const int mmax = std::max(3 * R, o->max_ss_vecs * R);
+  // thick restart keeps the lowest KR Ritz vectors (one sigma launch holds
+  // at most FCG_MAX_R columns)
+  const int KR = std::min(2 * R, FCG_MAX_R);
+  const int mmax = std::max(KR + R, o->max_ss_vecs * R);
   const int refresh = std::max(1, o->refresh_k);
   auto alloc = [&](double **p, int64_t cnt) {
     return hipMalloc((void **)p, sizeof(double) * std::max<int64_t>(cnt, 1)) == hipSuccess;
   };
-  if (!alloc(&d.V, n * mmax) || !alloc(&d.W, n * mmax) || !alloc(&d.X, n * R) ||
-      !alloc(&d.AX, n * R) || !alloc(&d.T, n * R) || !alloc(&d.hd, n) ||
+  if (!alloc(&d.V, n * mmax) || !alloc(&d.W, n * mmax) || !alloc(&d.X, n * KR) ||
+      !alloc(&d.AX, n * KR) || !alloc(&d.T, n * KR) || !alloc(&d.hd, n) ||
       !alloc(&d.small, (int64_t)mmax * mmax) || !alloc(&d.theta, R)) {
     ctx->err = "Davidson workspace allocation failed";
     return FCG_ENOMEM;
@@ -218,7 +223,7 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
   };
   if (int rc = sigma(d.V, d.W, R, FCG_FP64)) return rc;
   int m = R;
-  bool all_exact = true;
+  int ne = R;  // leading subspace columns whose sigma is FP64
   int since_refresh = 0;
   std::vector<double> G, w, Y, theta(R), res(R);
   const double one = 1.0, zero = 0.0;
@@ -232,22 +237,29 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
                            ctx->stream),
             ctx);
     FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+    // G[i,j] = v_i' w_j.  Columns below ne carry FP64 sigma: a coupling of
+    // an exact column i with a reduced-precision column j is taken as
+    // v_j' w_i (= v_i' H v_j exactly), the rest is symmetrised by averaging
     for (int j = 0; j < m; j++)
       for (int i = 0; i < j; i++) {
-        const double a = 0.5 * (G[i + (size_t)m * j] + G[j + (size_t)m * i]);
+        const double a = (i < ne && j >= ne)
+                             ? G[j + (size_t)m * i]
+                             : 0.5 * (G[i + (size_t)m * j] + G[j + (size_t)m * i]);
         G[i + (size_t)m * j] = G[j + (size_t)m * i] = a;
       }
     jacobi_eigh(m, G, w, Y);
     for (int r = 0; r < R; r++) theta[r] = w[r];
-    FCG_HIP(hipMemcpyAsync(d.small, Y.data(), sizeof(double) * m * R, hipMemcpyHostToDevice,
+    const int nk = std::min(KR, m);
+    FCG_HIP(hipMemcpyAsync(d.small, Y.data(), sizeof(double) * m * nk, hipMemcpyHostToDevice,
                            ctx->stream),
             ctx);
     FCG_HIP(hipMemcpyAsync(d.theta, theta.data(), sizeof(double) * R, hipMemcpyHostToDevice,
                            ctx->stream),
             ctx);
-    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, R, m, &one, d.V, n,
+    // lowest nk Ritz vectors X = V Y and their images AX = W Y
+    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, nk, m, &one, d.V, n,
                      d.small, m, &zero, d.X, n));
-    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, R, m, &one, d.W, n,
+    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, nk, m, &one, d.W, n,
                      d.small, m, &zero, d.AX, n));
     hipLaunchKernelGGL(k_residual, dim3(1024), dim3(256), 0, ctx->stream, n, R, d.X, d.AX,
                        d.theta, d.T);
@@ -257,38 +269,41 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
       rmax = std::max(rmax, res[r]);
     }
     since_refresh++;
-    const bool want_refresh =
-        mixed && !all_exact && (rmax < o->conv_thresh || since_refresh >= refresh);
-    if (rmax < o->conv_thresh && all_exact) {
+    if (rmax < o->conv_thresh && ne == m) {
       converged = 1;
       break;
     }
+    const bool want_refresh =
+        mixed && ne < m && (rmax < o->conv_thresh || since_refresh >= refresh);
     if (want_refresh || m + R > mmax) {
-      m = R;
-      if (want_refresh || !all_exact) {
-        // collapse to the re-orthonormalised Ritz vectors, sigma redone in FP64
-        FCG_HIP(hipMemcpyAsync(d.T, d.X, sizeof(double) * n * R, hipMemcpyDeviceToDevice,
+      const bool exact = ne == m;
+      m = nk;
+      if (!exact) {
+        // collapse onto the re-orthonormalised Ritz vectors; their sigma is
+        // recomputed in FP64 so the residual that decides convergence is exact
+        FCG_HIP(hipMemcpyAsync(d.T, d.X, sizeof(double) * n * nk, hipMemcpyDeviceToDevice,
                                ctx->stream),
                 ctx);
-        if (int rc = orthonormalize(d, 0, R, 1e-300, &kept)) return rc;
-        FCG_HIP(hipMemcpyAsync(d.V, d.T, sizeof(double) * n * R, hipMemcpyDeviceToDevice,
+        if (int rc = orthonormalize(d, 0, nk, 1e-300, &kept)) return rc;
+        m = kept;
+        FCG_HIP(hipMemcpyAsync(d.V, d.T, sizeof(double) * n * m, hipMemcpyDeviceToDevice,
                                ctx->stream),
                 ctx);
-        if (int rc = sigma(d.V, d.W, R, FCG_FP64)) return rc;
-        all_exact = true;
+        if (int rc = sigma(d.V, d.W, m, FCG_FP64)) return rc;
         since_refresh = 0;
       } else {
-        // exact subspace: V <- V Y, W <- W Y (X and AX) stay consistent
-        FCG_HIP(hipMemcpyAsync(d.V, d.X, sizeof(double) * n * R, hipMemcpyDeviceToDevice,
+        // exact subspace: thick restart V <- V Y, W <- W Y
+        FCG_HIP(hipMemcpyAsync(d.V, d.X, sizeof(double) * n * nk, hipMemcpyDeviceToDevice,
                                ctx->stream),
                 ctx);
-        FCG_HIP(hipMemcpyAsync(d.W, d.AX, sizeof(double) * n * R, hipMemcpyDeviceToDevice,
+        FCG_HIP(hipMemcpyAsync(d.W, d.AX, sizeof(double) * n * nk, hipMemcpyDeviceToDevice,
                                ctx->stream),
                 ctx);
       }
+      ne = m;
       continue;
     }
-    // correction vectors
+    // correction vectors from the residuals of the R lowest roots
     if (o->precond)
       hipLaunchKernelGGL(k_precond, dim3(1024), dim3(256), 0, ctx->stream, n, R, d.hd, d.theta,
                          d.T);
@@ -302,8 +317,8 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
             ctx);
     const int prec = mixed ? low : FCG_FP64;
     if (int rc = sigma(d.V + (int64_t)m * n, d.W + (int64_t)m * n, kept, prec)) return rc;
-    if (mixed) all_exact = false;
     m += kept;
+    if (!mixed) ne = m;
   }
   FCG_HIP(hipMemcpyAsync(evecs, d.X, sizeof(double) * n * R, hipMemcpyDeviceToHost, ctx->stream),
           ctx);

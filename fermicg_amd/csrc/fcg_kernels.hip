@@ -236,90 +236,87 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
 }
 
 // ---- sigma apply ----
-template <typename TK>
-__device__ __forceinline__ float kval(TK v);
-template <>
-__device__ __forceinline__ float kval<float>(float v) { return v; }
-template <>
-__device__ __forceinline__ float kval<uint32_t>(uint32_t u) {
+// Element conversion of the stored Hamiltonian blocks.
+__device__ __forceinline__ double kval(double v) { return v; }
+__device__ __forceinline__ float kval(float v) { return v; }
+__device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 
-template <int RMAX>
-__global__ __launch_bounds__(256) void k_apply_f64(int64_t n_box, const int64_t *__restrict__ var,
-                                                   const int64_t *__restrict__ row_koff,
-                                                   const int64_t *__restrict__ col_ptr,
-                                                   const int32_t *__restrict__ colidx,
-                                                   const double *__restrict__ K,
-                                                   const double *__restrict__ c, int R,
-                                                   int64_t nvar, double *__restrict__ sigma) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
-  if (b >= n_box) return;
-  const int db = (int)(var[b + 1] - var[b]);
-  int dbp = 1;
-  while (dbp < db) dbp <<= 1;
-  const int sl = 64 / dbp, i = lane & (dbp - 1), s = lane / dbp;
-  double acc[RMAX];
-#pragma unroll
-  for (int r = 0; r < RMAX; r++) acc[r] = 0.0;
-  const int64_t c0 = col_ptr[b], ncol = col_ptr[b + 1] - c0;
-  const double *Kr = K + row_koff[b];
-  if (i < db) {
-    for (int64_t jj = s; jj < ncol; jj += sl) {
-      const double kv = Kr[i + (int64_t)db * jj];
-      const int64_t col = colidx[c0 + jj];
-#pragma unroll
-      for (int r = 0; r < RMAX; r++)
-        if (r < R) acc[r] += kv * c[r * nvar + col];
-    }
-  }
-  for (int off = dbp; off < 64; off <<= 1)
-#pragma unroll
-    for (int r = 0; r < RMAX; r++) acc[r] += __shfl_xor(acc[r], off, 64);
-  if (s == 0 && i < db) {
-#pragma unroll
-    for (int r = 0; r < RMAX; r++)
-      if (r < R) sigma[r * nvar + var[b] + i] = acc[r];
-  }
-}
+constexpr int APPLY_UNROLL = 8;
 
-template <typename TK, int RMAX>
-__global__ __launch_bounds__(256) void k_apply_f32(int64_t n_box, const int64_t *__restrict__ var,
-                                                   const int64_t *__restrict__ row_koff,
-                                                   const int64_t *__restrict__ col_ptr,
-                                                   const int32_t *__restrict__ colidx,
-                                                   const TK *__restrict__ K,
-                                                   const float *__restrict__ c, int R,
-                                                   int64_t nvar, double *__restrict__ sigma) {
+// One wavefront per bra box b.  Lane l = i + db*s works on row i of the
+// box's panel and on every sl-th column starting at s (sl = 64/db slices),
+// so a wave reads 64 consecutive panel elements per column step.  The column
+// loop is unrolled APPLY_UNROLL times and the root count R is a template
+// constant, so each lane issues all block, column-index and vector loads of
+// an unrolled step before the first FMA (no per-load waits).  Offsets inside
+// one row panel are 32-bit (a panel holds fewer than 2^31 elements).  Slices
+// are summed in a fixed order through LDS.
+template <typename TK, typename TA, typename TC, int R>
+__global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__restrict__ var,
+                                               const int64_t *__restrict__ row_koff,
+                                               const int64_t *__restrict__ col_ptr,
+                                               const int32_t *__restrict__ colidx,
+                                               const TK *__restrict__ K, const TC *__restrict__ c,
+                                               int64_t nvar, double *__restrict__ sigma) {
+  __shared__ TA red[4][64 * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * 4 + wave;
-  if (b >= n_box) return;
-  const int db = (int)(var[b + 1] - var[b]);
-  int dbp = 1;
-  while (dbp < db) dbp <<= 1;
-  const int sl = 64 / dbp, i = lane & (dbp - 1), s = lane / dbp;
-  float acc[RMAX];
+  const bool live = b < n_box;
+  const int db = live ? (int)(var[b + 1] - var[b]) : 1;
+  const int sl = 64 / db;
+  const int i = lane % db, s = lane / db;
+  TA acc[R];
 #pragma unroll
-  for (int r = 0; r < RMAX; r++) acc[r] = 0.0f;
-  const int64_t c0 = col_ptr[b], ncol = col_ptr[b + 1] - c0;
-  const TK *Kr = K + row_koff[b];
-  if (i < db) {
-    for (int64_t jj = s; jj < ncol; jj += sl) {
-      const float kv = kval<TK>(Kr[i + (int64_t)db * jj]);
-      const int64_t col = colidx[c0 + jj];
+  for (int r = 0; r < R; r++) acc[r] = (TA)0;
+  if (live && s < sl) {
+    const int64_t c0 = col_ptr[b];
+    const int ncol = (int)(col_ptr[b + 1] - c0);
+    const TK *Kr = K + row_koff[b] + i;
+    const int32_t *ci = colidx + c0;
+    const TC *cr[R];
 #pragma unroll
-      for (int r = 0; r < RMAX; r++)
-        if (r < R) acc[r] = fmaf(kv, c[r * nvar + col], acc[r]);
+    for (int r = 0; r < R; r++) cr[r] = c + (int64_t)r * nvar;
+    int jj = s;
+    const int step = APPLY_UNROLL * sl;
+    for (; jj + (APPLY_UNROLL - 1) * sl < ncol; jj += step) {
+      TK kv[APPLY_UNROLL];
+      int32_t col[APPLY_UNROLL];
+#pragma unroll
+      for (int u = 0; u < APPLY_UNROLL; u++) {
+        kv[u] = Kr[(uint32_t)(db * (jj + u * sl))];
+        col[u] = ci[jj + u * sl];
+      }
+      TC cv[APPLY_UNROLL][R];
+#pragma unroll
+      for (int u = 0; u < APPLY_UNROLL; u++)
+#pragma unroll
+        for (int r = 0; r < R; r++) cv[u][r] = cr[r][col[u]];
+#pragma unroll
+      for (int u = 0; u < APPLY_UNROLL; u++) {
+        const TA k = kval(kv[u]);
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] += k * (TA)cv[u][r];
+      }
+    }
+    for (; jj < ncol; jj += sl) {
+      const TA k = kval(Kr[(uint32_t)(db * jj)]);
+      const int32_t cc = ci[jj];
+#pragma unroll
+      for (int r = 0; r < R; r++) acc[r] += k * (TA)cr[r][cc];
     }
   }
-  for (int off = dbp; off < 64; off <<= 1)
 #pragma unroll
-    for (int r = 0; r < RMAX; r++) acc[r] += __shfl_xor(acc[r], off, 64);
-  if (s == 0 && i < db) {
+  for (int r = 0; r < R; r++) red[wave][lane * R + r] = acc[r];
+  __syncthreads();
+  if (live && lane < db) {
 #pragma unroll
-    for (int r = 0; r < RMAX; r++)
-      if (r < R) sigma[r * nvar + var[b] + i] = (double)acc[r];
+    for (int r = 0; r < R; r++) {
+      TA v = (TA)0;
+      for (int s2 = 0; s2 < sl; s2++) v += red[wave][(lane + db * s2) * R + r];
+      sigma[(int64_t)r * nvar + var[b] + lane] = (double)v;
+    }
   }
 }
 
@@ -406,40 +403,37 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
                             int precision, hipStream_t s, hipEvent_t before_apply) {
   const fcg_space *sp = pl->sp;
   const dim3 grid((unsigned)((sp->n_box + 3) / 4)), block(256);
+#define LAUNCH(TK, TA, TC, KP, CP, RR)                                                        \
+  case RR:                                                                                    \
+    hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>), grid, block, 0, s, sp->n_box, sp->d_var,   \
+                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma); \
+    break;
+#define BY_R(TK, TA, TC, KP, CP)                                                              \
+  switch (R) {                                                                                \
+    LAUNCH(TK, TA, TC, KP, CP, 1) LAUNCH(TK, TA, TC, KP, CP, 2) LAUNCH(TK, TA, TC, KP, CP, 3)   \
+    LAUNCH(TK, TA, TC, KP, CP, 4) LAUNCH(TK, TA, TC, KP, CP, 5) LAUNCH(TK, TA, TC, KP, CP, 6)   \
+    LAUNCH(TK, TA, TC, KP, CP, 7) LAUNCH(TK, TA, TC, KP, CP, 8) LAUNCH(TK, TA, TC, KP, CP, 9)   \
+    LAUNCH(TK, TA, TC, KP, CP, 10) LAUNCH(TK, TA, TC, KP, CP, 11)                             \
+    LAUNCH(TK, TA, TC, KP, CP, 12) LAUNCH(TK, TA, TC, KP, CP, 13)                             \
+    LAUNCH(TK, TA, TC, KP, CP, 14) LAUNCH(TK, TA, TC, KP, CP, 15)                             \
+    LAUNCH(TK, TA, TC, KP, CP, 16)                                                            \
+    default: return hipErrorInvalidValue;                                                     \
+  }
   if (precision == FCG_FP64) {
     if (before_apply) (void)hipEventRecord(before_apply, s);
-#define L64(RM)                                                                              \
-  hipLaunchKernelGGL(k_apply_f64<RM>, grid, block, 0, s, sp->n_box, sp->d_var, pl->d_row_koff, \
-                     pl->d_col_ptr, pl->d_colidx, pl->d_K64, c, R, sp->nvar, sigma)
-    if (R <= 1) L64(1);
-    else if (R <= 2) L64(2);
-    else if (R <= 4) L64(4);
-    else if (R <= 8) L64(8);
-    else L64(16);
-#undef L64
+    BY_R(double, double, double, pl->d_K64, c);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * sp->nvar, c,
                      pl->d_c32);
   if (before_apply) (void)hipEventRecord(before_apply, s);
-#define L32(TK, KP, RM)                                                                       \
-  hipLaunchKernelGGL((k_apply_f32<TK, RM>), grid, block, 0, s, sp->n_box, sp->d_var,          \
-                     pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, pl->d_c32, R, sp->nvar, \
-                     sigma)
   if (precision == FCG_FP32) {
-    if (R <= 1) L32(float, pl->d_K32, 1);
-    else if (R <= 2) L32(float, pl->d_K32, 2);
-    else if (R <= 4) L32(float, pl->d_K32, 4);
-    else if (R <= 8) L32(float, pl->d_K32, 8);
-    else L32(float, pl->d_K32, 16);
+    BY_R(float, float, float, pl->d_K32, pl->d_c32);
   } else {
-    if (R <= 1) L32(uint32_t, pl->d_Kbf, 1);
-    else if (R <= 2) L32(uint32_t, pl->d_Kbf, 2);
-    else if (R <= 4) L32(uint32_t, pl->d_Kbf, 4);
-    else if (R <= 8) L32(uint32_t, pl->d_Kbf, 8);
-    else L32(uint32_t, pl->d_Kbf, 16);
+    BY_R(uint32_t, float, float, pl->d_Kbf, pl->d_c32);
   }
-#undef L32
+#undef BY_R
+#undef LAUNCH
   return hipGetLastError();
 }
 
