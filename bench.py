@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
     ap.add_argument("--cpu-rows", type=int, default=2000)
     ap.add_argument("--clusters", type=int, default=6, help="bst workload: number of clusters")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (known bytes) to scale PMC counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -79,6 +81,15 @@ def main():
     n = pl.nvar
     R = st.coeffs.shape[1]
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        # reads 1 GiB and writes 1 GiB per dispatch: the reference point that
+        # FETCH_SIZE / WRITE_SIZE of the sigma kernels are scaled against
+        xa = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        xb = torch.empty_like(xa)
+        for _ in range(3):
+            xb.copy_(xa)
+        torch.cuda.synchronize()
+        del xa, xb
     c_dev = torch.from_numpy(np.ascontiguousarray(st.coeffs.T)).to(dev)
     s_dev = torch.empty_like(c_dev)
     torch.cuda.synchronize()

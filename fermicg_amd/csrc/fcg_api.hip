@@ -93,7 +93,6 @@ void fcg_destroy(fcg_ctx_t ctx) {
   (void)hipSetDevice(ctx->device);
   dev_free(ctx->d_c);
   dev_free(ctx->d_s);
-  fcg_blas_destroy(ctx);
   for (auto &e : ctx->slots)
     if (e) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

@@ -11,25 +11,23 @@
 // accepted, the subspace collapses onto those Ritz vectors and their sigma
 // is recomputed in FP64 (one launch), so the residual that decides
 // convergence is always an FP64 one.
-// Orthogonalisation (classical Gram-Schmidt, twice) and the Rayleigh-Ritz
-// projection are FP64; the small projected eigenproblem is solved on the host
-// by cyclic Jacobi.
-#include <rocblas/rocblas.h>
-
+// Orthogonalisation (block Gram-Schmidt against the subspace, twice, then
+// symmetric (Loewdin) orthonormalisation of the new block) and the
+// Rayleigh-Ritz projection are FP64 on the GPU with this file's own
+// tall-skinny kernels: a split-K Gram product reduced in a fixed order (so
+// results are reproducible) and a row-parallel product with a small matrix.
+// The small projected eigenproblems are solved on the host by cyclic Jacobi.
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "fcg_internal.h"
 
-void fcg_blas_destroy(fcg_ctx *ctx) {
-  if (ctx && ctx->blas) {
-    rocblas_destroy_handle((rocblas_handle)ctx->blas);
-    ctx->blas = nullptr;
-  }
-}
-
 namespace {
+
+constexpr int GT = 16;            // Gram output tile (GT x GT per workgroup)
+constexpr int GROWS = 64;         // rows staged in LDS per step
+constexpr int64_t GCHUNK = 4096;  // rows per split-K chunk
 
 __global__ void k_residual(int64_t n, int R, const double *X, const double *AX,
                            const double *theta, double *res) {
@@ -49,6 +47,65 @@ __global__ void k_precond(int64_t n, int R, const double *hd, const double *thet
     double d = theta[r] - hd[e - (int64_t)r * n];
     if (fabs(d) < 1e-8) d = d < 0 ? -1e-8 : 1e-8;
     t[e] = t[e] / d;
+  }
+}
+
+// part[chunk][i + mv*j] = sum over the chunk's rows of V[r,i] W[r,j]
+__global__ __launch_bounds__(256) void k_gram_partial(int64_t n, const double *V, int mv,
+                                                      const double *W, int mw, double *part) {
+  __shared__ double sV[GROWS][GT + 1];
+  __shared__ double sW[GROWS][GT + 1];
+  const int tid = threadIdx.x, ti = tid % GT, tj = tid / GT;
+  const int i0 = blockIdx.y * GT, j0 = blockIdx.z * GT;
+  const int64_t r_begin = (int64_t)blockIdx.x * GCHUNK;
+  const int64_t r_end = std::min<int64_t>(n, r_begin + GCHUNK);
+  double acc = 0.0;
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += GROWS) {
+    for (int e = tid; e < GROWS * GT; e += 256) {
+      const int rr = e % GROWS, cc = e / GROWS;
+      const int64_t r = r0 + rr;
+      sV[rr][cc] = (r < r_end && i0 + cc < mv) ? V[r + n * (i0 + cc)] : 0.0;
+      sW[rr][cc] = (r < r_end && j0 + cc < mw) ? W[r + n * (j0 + cc)] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int rr = 0; rr < GROWS; rr++) acc += sV[rr][ti] * sW[rr][tj];
+    __syncthreads();
+  }
+  const int i = i0 + ti, j = j0 + tj;
+  if (i < mv && j < mw) part[(int64_t)blockIdx.x * mv * mw + i + (int64_t)mv * j] = acc;
+}
+
+__global__ void k_gram_reduce(int nchunk, int cnt, const double *part, double *out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= cnt) return;
+  double s = 0.0;
+  for (int c = 0; c < nchunk; c++) s += part[(int64_t)c * cnt + e];
+  out[e] = s;
+}
+
+// Y[r, c] = beta Y[r, c] + alpha sum_i A[r, i] B[i, c]  (B is k x nc, k <= 256, nc <= 16)
+__global__ __launch_bounds__(256) void k_gemm_small(int64_t n, int k, int nc, const double *A,
+                                                    const double *B, double alpha, double beta,
+                                                    double *Y) {
+  __shared__ double sB[FCG_MAX_R * 16 * 16];
+  for (int e = threadIdx.x; e < k * nc; e += blockDim.x) sB[e] = B[e];
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    double acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) acc[c] = 0.0;
+    for (int i = 0; i < k; i++) {
+      const double a = A[r + n * i];
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (c < nc) acc[c] += a * sB[i + k * c];
+    }
+    for (int c = 0; c < nc; c++) {
+      const double prev = beta == 0.0 ? 0.0 : beta * Y[r + n * c];
+      Y[r + n * c] = prev + alpha * acc[c];
+    }
   }
 }
 
@@ -99,65 +156,87 @@ void jacobi_eigh(int m, std::vector<double> &A, std::vector<double> &w, std::vec
 }
 
 struct Dav {
-  fcg_ctx *ctx;
-  rocblas_handle h;
-  int64_t n;
-  double *V = nullptr, *W = nullptr, *X = nullptr, *AX = nullptr, *T = nullptr;
-  double *hd = nullptr, *small = nullptr, *theta = nullptr;
+  fcg_ctx *ctx = nullptr;
+  int64_t n = 0;
+  int nchunk = 0;
+  double *V = nullptr, *W = nullptr, *X = nullptr, *AX = nullptr, *T = nullptr, *T2 = nullptr;
+  double *hd = nullptr, *small = nullptr, *theta = nullptr, *part = nullptr;
   ~Dav() {
-    for (double *p : {V, W, X, AX, T, hd, small, theta})
+    for (double *p : {V, W, X, AX, T, T2, hd, small, theta, part})
       if (p) (void)hipFree(p);
   }
 };
 
-#define BL(x)                                                  \
-  do {                                                         \
-    if ((x) != rocblas_status_success) {                       \
-      ctx->err = std::string("rocBLAS: ") + #x;                \
-      return FCG_EDEVICE;                                      \
-    }                                                          \
-  } while (0)
+// dev_out (ma x mb, column-major) = A' B over all n rows; optionally copied to host
+int gram(Dav &d, const double *A, int ma, const double *B, int mb, double *dev_out,
+         std::vector<double> *host_out) {
+  fcg_ctx *ctx = d.ctx;
+  const dim3 grid((unsigned)d.nchunk, (unsigned)((ma + GT - 1) / GT), (unsigned)((mb + GT - 1) / GT));
+  hipLaunchKernelGGL(k_gram_partial, grid, dim3(256), 0, ctx->stream, d.n, A, ma, B, mb, d.part);
+  const int cnt = ma * mb;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((cnt + 255) / 256), dim3(256), 0, ctx->stream, d.nchunk,
+                     cnt, d.part, dev_out);
+  FCG_HIP(hipGetLastError(), ctx);
+  if (host_out) {
+    host_out->resize(cnt);
+    FCG_HIP(hipMemcpyAsync(host_out->data(), dev_out, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                           ctx->stream),
+            ctx);
+    FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  }
+  return FCG_OK;
+}
 
-// Orthogonalise the k columns of T against V[:, :m] (CGS twice) and among
-// themselves; normalise.  Columns below lindep are dropped (compacted).
+// Y = beta Y + alpha A B  (A: n x k resident, B: k x nc on the device)
+int gemm(Dav &d, const double *A, int k, const double *B, int nc, double alpha, double beta,
+         double *Y) {
+  fcg_ctx *ctx = d.ctx;
+  const unsigned blocks = (unsigned)std::min<int64_t>(2048, (d.n + 255) / 256);
+  hipLaunchKernelGGL(k_gemm_small, dim3(blocks), dim3(256), 0, ctx->stream, d.n, k, nc, A, B,
+                     alpha, beta, Y);
+  FCG_HIP(hipGetLastError(), ctx);
+  return FCG_OK;
+}
+
+// Orthonormalise the k columns of T against V[:, :m] and among themselves:
+// two rounds of (block Gram-Schmidt against V, then symmetric orthonormal-
+// isation S^-1/2 of the block).  The first round drops directions whose norm
+// after projection is below lindep; kept columns are compacted to the front.
 int orthonormalize(Dav &d, int m, int k, double lindep, int *kept) {
   fcg_ctx *ctx = d.ctx;
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  const int64_t n = d.n;
-  for (int pass = 0; pass < 2 && m > 0; pass++) {
-    BL(rocblas_dgemm(d.h, rocblas_operation_transpose, rocblas_operation_none, m, k, n, &one,
-                     d.V, n, d.T, n, &zero, d.small, m));
-    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, k, m, &mone, d.V,
-                     n, d.small, m, &one, d.T, n));
-  }
-  int out = 0;
-  for (int j = 0; j < k; j++) {
-    double *tj = d.T + (int64_t)j * n;
-    for (int pass = 0; pass < 2 && out > 0; pass++) {
-      BL(rocblas_dgemv(d.h, rocblas_operation_transpose, n, out, &one, d.T, n, tj, 1, &zero,
-                       d.small, 1));
-      BL(rocblas_dgemv(d.h, rocblas_operation_none, n, out, &mone, d.T, n, d.small, 1, &one, tj,
-                       1));
-    }
-    // re-orthogonalise against V once more for the new column
+  std::vector<double> S, w, U, M;
+  int cur = k;
+  for (int round = 0; round < 2; round++) {
     if (m > 0) {
-      BL(rocblas_dgemv(d.h, rocblas_operation_transpose, n, m, &one, d.V, n, tj, 1, &zero,
-                       d.small, 1));
-      BL(rocblas_dgemv(d.h, rocblas_operation_none, n, m, &mone, d.V, n, d.small, 1, &one, tj,
-                       1));
+      if (int rc = gram(d, d.V, m, d.T, cur, d.small, nullptr)) return rc;
+      if (int rc = gemm(d, d.V, m, d.small, cur, -1.0, 1.0, d.T)) return rc;
     }
-    double nrm = 0.0;
-    BL(rocblas_dnrm2(d.h, n, tj, 1, &nrm));
-    if (!(nrm > lindep)) continue;
-    const double inv = 1.0 / nrm;
-    BL(rocblas_dscal(d.h, n, &inv, tj, 1));
-    if (out != j)
-      FCG_HIP(hipMemcpyAsync(d.T + (int64_t)out * n, tj, sizeof(double) * n,
-                             hipMemcpyDeviceToDevice, ctx->stream),
-              ctx);
-    out++;
+    if (int rc = gram(d, d.T, cur, d.T, cur, d.small, &S)) return rc;
+    jacobi_eigh(cur, S, w, U);
+    const double floor2 = round == 0 ? lindep * lindep : 0.0;
+    std::vector<int> keep;
+    for (int j = cur - 1; j >= 0; j--)  // largest first
+      if (w[j] > floor2 && w[j] > 0.0) keep.push_back(j);
+    const int nk = (int)keep.size();
+    if (nk == 0) {
+      *kept = 0;
+      return FCG_OK;
+    }
+    M.assign((size_t)cur * nk, 0.0);
+    for (int c = 0; c < nk; c++) {
+      const double sc = 1.0 / std::sqrt(w[keep[c]]);
+      for (int i = 0; i < cur; i++) M[i + (size_t)cur * c] = U[i + (size_t)cur * keep[c]] * sc;
+    }
+    FCG_HIP(hipMemcpyAsync(d.small, M.data(), sizeof(double) * M.size(), hipMemcpyHostToDevice,
+                           ctx->stream),
+            ctx);
+    if (int rc = gemm(d, d.T, cur, d.small, nk, 1.0, 0.0, d.T2)) return rc;
+    FCG_HIP(hipMemcpyAsync(d.T, d.T2, sizeof(double) * d.n * nk, hipMemcpyDeviceToDevice,
+                           ctx->stream),
+            ctx);
+    cur = nk;
   }
-  *kept = out;
+  *kept = cur;
   return FCG_OK;
 }
 
@@ -168,21 +247,11 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
                             fcg_davidson_stats *st) {
   if (!ctx || !pl || !v0 || !o || !evals || !evecs || R < 1 || R > FCG_MAX_R) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
-  if (!ctx->blas) {
-    rocblas_handle hb;
-    if (rocblas_create_handle(&hb) != rocblas_status_success) {
-      ctx->err = "rocblas_create_handle failed";
-      return FCG_EDEVICE;
-    }
-    ctx->blas = hb;
-  }
   Dav d;
   d.ctx = ctx;
-  d.h = (rocblas_handle)ctx->blas;
-  BL(rocblas_set_stream(d.h, ctx->stream));
-  BL(rocblas_set_pointer_mode(d.h, rocblas_pointer_mode_host));
   const int64_t n = pl->sp->nvar;
   d.n = n;
+  d.nchunk = (int)((n + GCHUNK - 1) / GCHUNK);
   const int low = o->precision;
   const bool mixed = low != FCG_FP64;
   if (mixed && !(pl->precisions & (1 << low))) {
@@ -190,16 +259,17 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
     return FCG_EINVAL;
   }
   // thick restart keeps the lowest KR Ritz vectors (one sigma launch holds
-  // at most FCG_MAX_R columns)
+  // at most FCG_MAX_R columns); the subspace is capped at 256 vectors
   const int KR = std::min(2 * R, FCG_MAX_R);
-  const int mmax = std::max(KR + R, o->max_ss_vecs * R);
+  const int mmax = std::max(KR + R, std::min(256, o->max_ss_vecs * R));
   const int refresh = std::max(1, o->refresh_k);
   auto alloc = [&](double **p, int64_t cnt) {
     return hipMalloc((void **)p, sizeof(double) * std::max<int64_t>(cnt, 1)) == hipSuccess;
   };
   if (!alloc(&d.V, n * mmax) || !alloc(&d.W, n * mmax) || !alloc(&d.X, n * KR) ||
-      !alloc(&d.AX, n * KR) || !alloc(&d.T, n * KR) || !alloc(&d.hd, n) ||
-      !alloc(&d.small, (int64_t)mmax * mmax) || !alloc(&d.theta, R)) {
+      !alloc(&d.AX, n * KR) || !alloc(&d.T, n * KR) || !alloc(&d.T2, n * KR) ||
+      !alloc(&d.hd, n) || !alloc(&d.small, (int64_t)mmax * mmax) || !alloc(&d.theta, R) ||
+      !alloc(&d.part, (int64_t)d.nchunk * mmax * mmax)) {
     ctx->err = "Davidson workspace allocation failed";
     return FCG_ENOMEM;
   }
@@ -225,18 +295,11 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
   int m = R;
   int ne = R;  // leading subspace columns whose sigma is FP64
   int since_refresh = 0;
-  std::vector<double> G, w, Y, theta(R), res(R);
-  const double one = 1.0, zero = 0.0;
+  std::vector<double> G, w, Y, theta(R), res(R), rr;
   int converged = 0;
   for (int it = 1; it <= o->max_iter; it++) {
     stats.iterations = it;
-    BL(rocblas_dgemm(d.h, rocblas_operation_transpose, rocblas_operation_none, m, m, n, &one, d.V,
-                     n, d.W, n, &zero, d.small, m));
-    G.resize((size_t)m * m);
-    FCG_HIP(hipMemcpyAsync(G.data(), d.small, sizeof(double) * m * m, hipMemcpyDeviceToHost,
-                           ctx->stream),
-            ctx);
-    FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+    if (int rc = gram(d, d.V, m, d.W, m, d.small, &G)) return rc;
     // G[i,j] = v_i' w_j.  Columns below ne carry FP64 sigma: a coupling of
     // an exact column i with a reduced-precision column j is taken as
     // v_j' w_i (= v_i' H v_j exactly), the rest is symmetrised by averaging
@@ -257,15 +320,14 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
                            ctx->stream),
             ctx);
     // lowest nk Ritz vectors X = V Y and their images AX = W Y
-    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, nk, m, &one, d.V, n,
-                     d.small, m, &zero, d.X, n));
-    BL(rocblas_dgemm(d.h, rocblas_operation_none, rocblas_operation_none, n, nk, m, &one, d.W, n,
-                     d.small, m, &zero, d.AX, n));
+    if (int rc = gemm(d, d.V, m, d.small, nk, 1.0, 0.0, d.X)) return rc;
+    if (int rc = gemm(d, d.W, m, d.small, nk, 1.0, 0.0, d.AX)) return rc;
     hipLaunchKernelGGL(k_residual, dim3(1024), dim3(256), 0, ctx->stream, n, R, d.X, d.AX,
                        d.theta, d.T);
+    if (int rc = gram(d, d.T, R, d.T, R, d.small, &rr)) return rc;
     double rmax = 0.0;
     for (int r = 0; r < R; r++) {
-      BL(rocblas_dnrm2(d.h, n, d.T + (int64_t)r * n, 1, &res[r]));
+      res[r] = std::sqrt(std::max(0.0, rr[r + (size_t)R * r]));
       rmax = std::max(rmax, res[r]);
     }
     since_refresh++;
@@ -284,7 +346,7 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
         FCG_HIP(hipMemcpyAsync(d.T, d.X, sizeof(double) * n * nk, hipMemcpyDeviceToDevice,
                                ctx->stream),
                 ctx);
-        if (int rc = orthonormalize(d, 0, nk, 1e-300, &kept)) return rc;
+        if (int rc = orthonormalize(d, 0, nk, 0.0, &kept)) return rc;
         m = kept;
         FCG_HIP(hipMemcpyAsync(d.V, d.T, sizeof(double) * n * m, hipMemcpyDeviceToDevice,
                                ctx->stream),

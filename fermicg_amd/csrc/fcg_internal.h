@@ -36,7 +36,6 @@ struct fcg_ctx {
   std::string err;
   double last_ms = 0.0;
   int last_launches = 0;
-  void *blas = nullptr;  // rocblas_handle
   // scratch for host-pointer sigma calls
   double *d_c = nullptr, *d_s = nullptr;
   int64_t scratch_elems = 0;
@@ -147,5 +146,4 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
                             int precision, hipStream_t s, hipEvent_t before_apply);
-void fcg_blas_destroy(fcg_ctx *ctx);  // fcg_davidson.hip
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s);

@@ -243,16 +243,46 @@ __device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 
-constexpr int APPLY_UNROLL = 8;
+constexpr int APPLY_UNROLL = 16;
 
-// One wavefront per bra box b.  Lane l = i + db*s works on row i of the
-// box's panel and on every sl-th column starting at s (sl = 64/db slices),
-// so a wave reads 64 consecutive panel elements per column step.  The column
-// loop is unrolled APPLY_UNROLL times and the root count R is a template
-// constant, so each lane issues all block, column-index and vector loads of
-// an unrolled step before the first FMA (no per-load waits).  Offsets inside
-// one row panel are 32-bit (a panel holds fewer than 2^31 elements).  Slices
-// are summed in a fixed order through LDS.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+// R consecutive staged values (paired vector LDS reads when R is even)
+template <typename TC, int R>
+__device__ __forceinline__ void stage_read(const TC *p, TC (&v)[R]) {
+  if constexpr (R % 2 == 0) {
+    using V2 = typename Vec2<TC>::type;
+    const V2 *q = reinterpret_cast<const V2 *>(p);
+#pragma unroll
+    for (int h = 0; h < R / 2; h++) {
+      const V2 t = q[h];
+      v[2 * h] = t.x;
+      v[2 * h + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = p[r];
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  // order this wavefront's LDS accesses (all lanes of a wave run in lockstep)
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per bra box b.  The row's panel (db x ncol, column-major) is
+// walked in chunks of 64 columns: lane l loads the column index and the R
+// vector entries of column base+l (coalesced, one load per column rather
+// than per lane) one chunk ahead, and stages them in a per-wave LDS buffer;
+// lane (i, s) with l = i + db*s then multiplies panel row i of columns s,
+// s+sl, s+2sl, ... (sl = 64/db) by the staged entries (one vector LDS read
+// per element, broadcast across the lanes of a column).  Block loads are
+// predicated and unrolled APPLY_UNROLL times.  Slices are summed in a fixed
+// order through LDS, so results are reproducible bit for bit.
 template <typename TK, typename TA, typename TC, int R>
 __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__restrict__ var,
                                                const int64_t *__restrict__ row_koff,
@@ -261,54 +291,64 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
                                                const TK *__restrict__ K, const TC *__restrict__ c,
                                                int64_t nvar, double *__restrict__ sigma) {
   __shared__ TA red[4][64 * R];
+  __shared__ __align__(16) TC stage[4][64 * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * 4 + wave;
   const bool live = b < n_box;
   const int db = live ? (int)(var[b + 1] - var[b]) : 1;
   const int sl = 64 / db;
   const int i = lane % db, s = lane / db;
+  const bool act = s < sl;
   TA acc[R];
 #pragma unroll
   for (int r = 0; r < R; r++) acc[r] = (TA)0;
-  if (live && s < sl) {
+  TC *st = stage[wave];
+  if (live) {  // uniform per wave
     const int64_t c0 = col_ptr[b];
     const int ncol = (int)(col_ptr[b + 1] - c0);
     const TK *Kr = K + row_koff[b] + i;
     const int32_t *ci = colidx + c0;
-    const TC *cr[R];
+    const int nk = (64 + sl - 1) / sl;  // column steps per chunk (uniform)
+    TC nv[R];
+    {
+      const int32_t col = lane < ncol ? ci[lane] : 0;
 #pragma unroll
-    for (int r = 0; r < R; r++) cr[r] = c + (int64_t)r * nvar;
-    int jj = s;
-    const int step = APPLY_UNROLL * sl;
-    for (; jj + (APPLY_UNROLL - 1) * sl < ncol; jj += step) {
-      TK kv[APPLY_UNROLL];
-      int32_t col[APPLY_UNROLL];
-#pragma unroll
-      for (int u = 0; u < APPLY_UNROLL; u++) {
-        kv[u] = Kr[(uint32_t)(db * (jj + u * sl))];
-        col[u] = ci[jj + u * sl];
-      }
-      TC cv[APPLY_UNROLL][R];
-#pragma unroll
-      for (int u = 0; u < APPLY_UNROLL; u++)
-#pragma unroll
-        for (int r = 0; r < R; r++) cv[u][r] = cr[r][col[u]];
-#pragma unroll
-      for (int u = 0; u < APPLY_UNROLL; u++) {
-        const TA k = kval(kv[u]);
-#pragma unroll
-        for (int r = 0; r < R; r++) acc[r] += k * (TA)cv[u][r];
-      }
+      for (int r = 0; r < R; r++) nv[r] = lane < ncol ? c[(int64_t)r * nvar + col] : (TC)0;
     }
-    for (; jj < ncol; jj += sl) {
-      const TA k = kval(Kr[(uint32_t)(db * jj)]);
-      const int32_t cc = ci[jj];
+    for (int base = 0; base < ncol; base += 64) {
+      wave_sync();
 #pragma unroll
-      for (int r = 0; r < R; r++) acc[r] += k * (TA)cr[r][cc];
+      for (int r = 0; r < R; r++) st[lane * R + r] = nv[r];
+      wave_sync();
+      {  // prefetch the next chunk's column entries
+        const int jl = base + 64 + lane;
+        const int32_t col = jl < ncol ? ci[jl] : 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) nv[r] = jl < ncol ? c[(int64_t)r * nvar + col] : (TC)0;
+      }
+      const int lim = act ? min(64, ncol - base) : 0;
+      const TK *Kc = Kr + (uint32_t)(db * base);
+      for (int k0 = 0; k0 < nk; k0 += APPLY_UNROLL) {
+        TK kv[APPLY_UNROLL];
+#pragma unroll
+        for (int u = 0; u < APPLY_UNROLL; u++) {
+          const int j = s + sl * (k0 + u);
+          kv[u] = j < lim ? Kc[(uint32_t)(db * j)] : (TK)0;
+        }
+#pragma unroll
+        for (int u = 0; u < APPLY_UNROLL; u++) {
+          const int j = min(s + sl * (k0 + u), 63);
+          const TA k = kval(kv[u]);
+          TC cvj[R];
+          stage_read<TC, R>(st + j * R, cvj);
+#pragma unroll
+          for (int r = 0; r < R; r++) acc[r] += k * (TA)cvj[r];
+        }
+      }
     }
   }
 #pragma unroll
-  for (int r = 0; r < R; r++) red[wave][lane * R + r] = acc[r];
+  for (int r = 0; r < R; r++) red[wave][lane * R + r] = act ? acc[r] : (TA)0;
   __syncthreads();
   if (live && lane < db) {
 #pragma unroll
