@@ -26,6 +26,7 @@ EXPORTS = [
     "fcg_space_nvar", "fcg_space_destroy", "fcg_plan_create", "fcg_plan_destroy",
     "fcg_plan_stats", "fcg_sigma", "fcg_sigma_device", "fcg_diagonal", "fcg_davidson",
     "fcg_last_sigma_time", "fcg_event_record", "fcg_event_elapsed", "fcg_synchronize",
+    "fcg_plan_create_pair", "fcg_tpsci_pt2",
 ]
 
 
@@ -39,6 +40,11 @@ class DavidsonStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("sigma_fp64", ctypes.c_int),
                 ("sigma_low", ctypes.c_int), ("converged", ctypes.c_int),
                 ("residual", c_f64 * 16)]
+
+
+class PT2Stats(ctypes.Structure):
+    _fields_ = [("n_target_fock", c_i64), ("n_target_var", c_i64), ("n_first_order", c_i64),
+                ("block_pairs", c_i64), ("block_elems", c_i64), ("seconds", c_f64)]
 
 
 class FCGError(RuntimeError):
@@ -81,6 +87,9 @@ def load():
     L.fcg_space_destroy.argtypes = [VP]
     L.fcg_space_destroy.restype = None
     L.fcg_plan_create.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
+    L.fcg_plan_create_pair.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
+    L.fcg_tpsci_pt2.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64), ctypes.c_int,
+                                ctypes.c_int, P(c_f64), P(PT2Stats)]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -231,7 +240,21 @@ class Plan(_Handle):
         ctx.check(ctx.L.fcg_plan_create(ctx.h, ham.h, ops.h, space.h, nbody, mask,
                                         ctypes.byref(h)), "fcg_plan_create")
         pl = cls(ctx, h, keep=(ham, ops, space))
-        pl.nvar = space.nvar()
+        pl.nvar = pl.nvar_ket = space.nvar()
+        return pl
+
+    @classmethod
+    def create_pair(cls, ctx, ham, ops, bra, ket, nbody=4, precisions=("fp64",)):
+        """Plan of H between two spaces: sigma(bra) = H c(ket)."""
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISION[p]
+        h = VP()
+        ctx.check(ctx.L.fcg_plan_create_pair(ctx.h, ham.h, ops.h, bra.h, ket.h, nbody, mask,
+                                             ctypes.byref(h)), "fcg_plan_create_pair")
+        pl = cls(ctx, h, keep=(ham, ops, bra, ket))
+        pl.nvar = bra.nvar()
+        pl.nvar_ket = ket.nvar()
         return pl
 
     def stats(self):
@@ -245,8 +268,8 @@ class Plan(_Handle):
         one = c.ndim == 1
         if one:
             c = c[:, None]
-        if c.shape[0] != self.nvar:
-            raise ValueError("vector length %d != space dimension %d" % (c.shape[0], self.nvar))
+        if c.shape[0] != self.nvar_ket:
+            raise ValueError("vector length %d != space dimension %d" % (c.shape[0], self.nvar_ket))
         R = c.shape[1]
         cc = np.ascontiguousarray(c.T)  # (R, n) == column-major (n, R)
         out = np.zeros((R, self.nvar))
@@ -291,3 +314,19 @@ class Plan(_Handle):
                  "sigma_low": st.sigma_low, "converged": bool(st.converged),
                  "residual": list(st.residual)[:R]}
         return ev, vec.T.copy(), stats
+
+
+def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1):
+    """Second-order energies e2[r] of the TPSCI state (space, c) over its
+    first-order space (fcg_tpsci_pt2); returns (e2, stats dict)."""
+    c = np.asarray(c, dtype=np.float64)
+    if c.ndim == 1:
+        c = c[:, None]
+    R = c.shape[1]
+    cc = np.ascontiguousarray(c.T)
+    e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (R,)))
+    e2 = np.zeros(R)
+    st = PT2Stats()
+    ctx.check(ctx.L.fcg_tpsci_pt2(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64), _p(e0, c_f64),
+                                  nbody, h0_op, _p(e2, c_f64), ctypes.byref(st)), "fcg_tpsci_pt2")
+    return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}

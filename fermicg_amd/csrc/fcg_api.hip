@@ -1,4 +1,5 @@
 // C ABI of libfcg_hip.so: handles, uploads, plan creation, sigma, diagonal.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -54,7 +55,9 @@ int ensure_scratch(fcg_ctx *ctx, int64_t n) {
   return FCG_OK;
 }
 
-int space_to_device(fcg_space *sp) {
+}  // namespace
+
+int fcg_space_to_device(fcg_space *sp) {
   fcg_ctx *ctx = sp->ctx;
   const int64_t nb = sp->n_box * sp->N;
   TRY(dev_upload(ctx, &sp->d_lo, sp->lo.data(), nb));
@@ -67,8 +70,6 @@ int space_to_device(fcg_space *sp) {
   FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
   return FCG_OK;
 }
-
-}  // namespace
 
 extern "C" {
 
@@ -289,7 +290,7 @@ int fcg_bst_space_upload(fcg_ctx_t ctx, int n_clusters, int64_t n_fock, const in
     sp->var_off.push_back(sp->var_off.back() + d);
   }
   sp->nvar = sp->var_off.back();
-  int rc = space_to_device(sp);
+  int rc = fcg_space_to_device(sp);
   if (rc) {
     fcg_space_destroy(sp);
     return rc;
@@ -312,33 +313,45 @@ void fcg_space_destroy(fcg_space_t sp) {
   delete sp;
 }
 
-int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int nbody,
-                    int precisions, fcg_plan_t *out) {
-  if (!ctx || !h || !o || !sp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
+}  // extern "C"
+
+// TPSCI spaces form their boxes at first use, against the ClusterOps table
+int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp) {
+  if (!sp->tpsci || sp->n_box >= 0) return FCG_OK;
+  std::vector<int64_t> fstart = sp->var_off;
+  std::vector<int16_t> cfg = sp->lo;
+  std::vector<int16_t> fock = sp->fock;
+  sp->lo.clear();
+  sp->var_off.clear();
+  int rc = fcg_tpsci_boxes(o, sp->N, sp->n_fock, fock.data(), fstart.data(),
+                           (int64_t)cfg.size() / sp->N, cfg.data(), sp, &ctx->err);
+  if (rc) {
+    sp->lo = cfg;
+    sp->var_off = fstart;
+    sp->box_fock.clear();
+    sp->hi.clear();
+    sp->rank.clear();
+    sp->foff.clear();
+    sp->n_box = -1;
+    return rc;
+  }
+  return fcg_space_to_device(sp);
+}
+
+int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
+                    int nbody, int precisions, fcg_plan **out) {
+  if (!ctx || !h || !o || !sp || !ksp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
   auto t0 = std::chrono::steady_clock::now();
-  if (sp->tpsci && sp->n_box < 0) {
-    // form boxes of the TPSCI space against this ClusterOps table
-    std::vector<int64_t> fstart = sp->var_off;
-    std::vector<int16_t> cfg = sp->lo;
-    std::vector<int16_t> fock = sp->fock;
-    sp->lo.clear();
-    sp->var_off.clear();
-    int rc = fcg_tpsci_boxes(o, sp->N, sp->n_fock, fock.data(), fstart.data(),
-                             (int64_t)cfg.size() / sp->N, cfg.data(), sp, &ctx->err);
-    if (rc) {
-      sp->lo = cfg;
-      sp->var_off = fstart;
-      return rc;
-    }
-    TRY(space_to_device(sp));
-  }
+  TRY(fcg_form_boxes(ctx, o, sp));
+  if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
   HostPlan hp;
-  int rc = fcg_build_host_plan(h, o, sp, nbody, &hp, &ctx->err);
+  int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, &hp, &ctx->err);
   if (rc) return rc;
   fcg_plan *pl = new fcg_plan();
   pl->ctx = ctx;
   pl->sp = sp;
+  pl->ksp = ksp;
   pl->n_pair = (int64_t)hp.pair_B.size();
   pl->n_task = (int64_t)hp.tasks.size();
   pl->n_kel = hp.n_kel;
@@ -386,6 +399,18 @@ int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int
 #undef PTRY
 }
 
+extern "C" {
+
+int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int nbody,
+                    int precisions, fcg_plan_t *out) {
+  return fcg_create_plan(ctx, h, o, sp, sp, nbody, precisions, out);
+}
+
+int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t bra,
+                         fcg_space_t ket, int nbody, int precisions, fcg_plan_t *out) {
+  return fcg_create_plan(ctx, h, o, bra, ket, nbody, precisions, out);
+}
+
 void fcg_plan_destroy(fcg_plan_t pl) {
   if (!pl) return;
   dev_free(pl->d_row_ptr);
@@ -424,7 +449,7 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
     return FCG_EINVAL;
   }
   (void)hipSetDevice(ctx->device);
-  const int64_t n = (int64_t)R * pl->sp->nvar;
+  const int64_t n = (int64_t)R * pl->ksp->nvar;
   if (precision != FCG_FP64 && pl->c32_elems < n) {
     dev_free(pl->d_c32);
     pl->c32_elems = 0;
@@ -441,12 +466,12 @@ int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c, double *sigm
               int precision) {
   if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c || !sigma) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
-  const int64_t n = (int64_t)R * pl->sp->nvar;
-  TRY(ensure_scratch(ctx, n));
-  FCG_HIP(hipMemcpyAsync(ctx->d_c, c, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream),
+  const int64_t nc = (int64_t)R * pl->ksp->nvar, ns = (int64_t)R * pl->sp->nvar;
+  TRY(ensure_scratch(ctx, std::max(nc, ns)));
+  FCG_HIP(hipMemcpyAsync(ctx->d_c, c, sizeof(double) * nc, hipMemcpyHostToDevice, ctx->stream),
           ctx);
   TRY(fcg_sigma_device(ctx, pl, R, ctx->d_c, ctx->d_s, precision));
-  FCG_HIP(hipMemcpyAsync(sigma, ctx->d_s, sizeof(double) * n, hipMemcpyDeviceToHost,
+  FCG_HIP(hipMemcpyAsync(sigma, ctx->d_s, sizeof(double) * ns, hipMemcpyDeviceToHost,
                          ctx->stream),
           ctx);
   FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
@@ -455,6 +480,10 @@ int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c, double *sigm
 
 int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t pl, double *hd) {
   if (!ctx || !pl || !hd) return FCG_EINVAL;
+  if (pl->ksp != pl->sp) {
+    ctx->err = "diagonal needs a plan within one space";
+    return FCG_EINVAL;
+  }
   (void)hipSetDevice(ctx->device);
   const int64_t n = pl->sp->nvar;
   TRY(ensure_scratch(ctx, n));

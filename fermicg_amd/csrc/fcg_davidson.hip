@@ -247,6 +247,10 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
                             fcg_davidson_stats *st) {
   if (!ctx || !pl || !v0 || !o || !evals || !evecs || R < 1 || R > FCG_MAX_R) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
+  if (pl->ksp != pl->sp) {
+    ctx->err = "Davidson needs a plan within one space";
+    return FCG_EINVAL;
+  }
   Dav d;
   d.ctx = ctx;
   const int64_t n = pl->sp->nvar;

@@ -103,7 +103,8 @@ struct fcg_space {
 
 struct fcg_plan {
   fcg_ctx *ctx = nullptr;
-  fcg_space *sp = nullptr;
+  fcg_space *sp = nullptr;   // bra space (rows of H, sigma)
+  fcg_space *ksp = nullptr;  // ket space (columns of H, c); == sp for H within one space
   int64_t n_pair = 0, n_task = 0, n_kel = 0;
   double build_flops = 0.0, build_us = 0.0;
   int precisions = 1;
@@ -134,11 +135,33 @@ struct HostPlan {
   int64_t n_kel = 0;
   double flops = 0.0;
 };
-int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp, int nbody,
-                        HostPlan *out, std::string *err);
+int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
+                        const fcg_space *ket, int nbody, HostPlan *out, std::string *err);
 int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
                     const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
                     fcg_space *sp, std::string *err);
+// every configuration of each listed Fock block, as boxes (first-order spaces)
+int fcg_full_space(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock, fcg_space *sp,
+                   std::string *err);
+// First-order space of a TPSCI state for PT2: every configuration of the
+// Fock blocks F + TransferConfig (F a Fock block of `var`, every cluster
+// sector present in ClusterOps), in first-appearance order, as boxes in
+// `tsp` (host side only); inspace[x] marks the configurations of `var`;
+// hoff/hstride [n_box][N] address the diagonal of op h0_op for each box's
+// first state and its stride.
+struct Pt2Target {
+  std::vector<uint8_t> inspace;
+  std::vector<int64_t> hoff;
+  std::vector<int32_t> hstride;
+};
+int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int h0_op,
+                   fcg_space *tsp, Pt2Target *out, std::string *err);
+
+// shared helpers (fcg_api.hip)
+int fcg_space_to_device(fcg_space *sp);
+int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp);
+int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
+                    int nbody, int precisions, fcg_plan **out);
 
 // kernels (fcg_kernels.hip)
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,

@@ -31,9 +31,9 @@ struct BuildArgs {
   const int64_t *gb_off;
   const int32_t *gb_dims;
   const double *gdata;
-  const int16_t *lo, *hi, *rank;
-  const int64_t *foff;
-  const double *factors;
+  const int16_t *loB, *hiB, *rankB, *loK, *hiK, *rankK;  // bra / ket boxes
+  const int64_t *foffB, *foffK;
+  const double *facB, *facK;
   int N;
   int64_t pair_base;
   double *K;
@@ -63,14 +63,14 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
   const int N = a.N;
   if (tid < N) {
     const int c = tid;
-    sLoB[c] = a.lo[B * N + c] - 1;
-    sLoK[c] = a.lo[K * N + c] - 1;
-    sNB[c] = a.hi[B * N + c] - a.lo[B * N + c] + 1;
-    sNK[c] = a.hi[K * N + c] - a.lo[K * N + c] + 1;
-    sRB[c] = a.rank[B * N + c];
-    sRK[c] = a.rank[K * N + c];
-    sFB[c] = a.foff[B * N + c];
-    sFK[c] = a.foff[K * N + c];
+    sLoB[c] = a.loB[B * N + c] - 1;
+    sLoK[c] = a.loK[K * N + c] - 1;
+    sNB[c] = a.hiB[B * N + c] - a.loB[B * N + c] + 1;
+    sNK[c] = a.hiK[K * N + c] - a.loK[K * N + c] + 1;
+    sRB[c] = a.rankB[B * N + c];
+    sRK[c] = a.rankK[K * N + c];
+    sFB[c] = a.foffB[B * N + c];
+    sFK[c] = a.foffK[K * N + c];
   }
   __syncthreads();
   if (tid == 0) {
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
       const int i = e % rb, j = e / rb;
       double s = 0.0;
       for (int x = 0; x < nrg; x++)
-        s += fac(a.factors, sFB[c], nrg, x, i) * fac(a.factors, sFK[c], nrg, x, j);
+        s += fac(a.facB, sFB[c], nrg, x, i) * fac(a.facK, sFK[c], nrg, x, j);
       sS[sSoff[c] + e] = s;
     }
   }
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
               v = Gp[(int64_t)np * mb * (lk + j)];
             } else {
               for (int J = 0; J < nJ; J++)
-                v += Gp[(int64_t)np * mb * (lk + J)] * a.factors[fK + J + (int64_t)nJ * j];
+                v += Gp[(int64_t)np * mb * (lk + J)] * a.facK[fK + J + (int64_t)nJ * j];
             }
             sT[e] = v;
           }
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
               v = sT[pp + pc * (i + nI * j)];
             } else {
               for (int I = 0; I < nI; I++)
-                v += a.factors[fB + I + (int64_t)nI * i] * sT[pp + pc * (I + nI * j)];
+                v += a.facB[fB + I + (int64_t)nI * i] * sT[pp + pc * (I + nI * j)];
             }
             sG[(p0 + pp) + np * (i + rb * j)] = v;
           }
@@ -289,7 +289,8 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
                                                const int64_t *__restrict__ col_ptr,
                                                const int32_t *__restrict__ colidx,
                                                const TK *__restrict__ K, const TC *__restrict__ c,
-                                               int64_t nvar, double *__restrict__ sigma) {
+                                               int64_t nvar_c, int64_t nvar_s,
+                                               double *__restrict__ sigma) {
   __shared__ TA red[4][64 * R];
   __shared__ __align__(16) TC stage[4][64 * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
     {
       const int32_t col = lane < ncol ? ci[lane] : 0;
 #pragma unroll
-      for (int r = 0; r < R; r++) nv[r] = lane < ncol ? c[(int64_t)r * nvar + col] : (TC)0;
+      for (int r = 0; r < R; r++) nv[r] = lane < ncol ? c[(int64_t)r * nvar_c + col] : (TC)0;
     }
     for (int base = 0; base < ncol; base += 64) {
       wave_sync();
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
         const int jl = base + 64 + lane;
         const int32_t col = jl < ncol ? ci[jl] : 0;
 #pragma unroll
-        for (int r = 0; r < R; r++) nv[r] = jl < ncol ? c[(int64_t)r * nvar + col] : (TC)0;
+        for (int r = 0; r < R; r++) nv[r] = jl < ncol ? c[(int64_t)r * nvar_c + col] : (TC)0;
       }
       const int lim = act ? min(64, ncol - base) : 0;
       const TK *Kc = Kr + (uint32_t)(db * base);
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
     for (int r = 0; r < R; r++) {
       TA v = (TA)0;
       for (int s2 = 0; s2 < sl; s2++) v += red[wave][(lane + db * s2) * R + r];
-      sigma[(int64_t)r * nvar + var[b] + lane] = (double)v;
+      sigma[(int64_t)r * nvar_s + var[b] + lane] = (double)v;
     }
   }
 }
@@ -414,11 +415,17 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.gb_off = o->d_gb_off;
   a.gb_dims = o->d_gb_dims;
   a.gdata = o->d_gdata;
-  a.lo = sp->d_lo;
-  a.hi = sp->d_hi;
-  a.rank = sp->d_rank;
-  a.foff = sp->d_foff;
-  a.factors = sp->d_factors;
+  const fcg_space *kp = pl->ksp;
+  a.loB = sp->d_lo;
+  a.hiB = sp->d_hi;
+  a.rankB = sp->d_rank;
+  a.foffB = sp->d_foff;
+  a.facB = sp->d_factors;
+  a.loK = kp->d_lo;
+  a.hiK = kp->d_hi;
+  a.rankK = kp->d_rank;
+  a.foffK = kp->d_foff;
+  a.facK = kp->d_factors;
   a.N = sp->N;
   a.K = pl->d_K64;
   const int64_t chunk = 1 << 20;
@@ -446,7 +453,8 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
 #define LAUNCH(TK, TA, TC, KP, CP, RR)                                                        \
   case RR:                                                                                    \
     hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>), grid, block, 0, s, sp->n_box, sp->d_var,   \
-                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma); \
+                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, pl->ksp->nvar,     \
+                       sp->nvar, sigma);                                                     \
     break;
 #define BY_R(TK, TA, TC, KP, CP)                                                              \
   switch (R) {                                                                                \
@@ -464,7 +472,7 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
     BY_R(double, double, double, pl->d_K64, c);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * sp->nvar, c,
+  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * pl->ksp->nvar, c,
                      pl->d_c32);
   if (before_apply) (void)hipEventRecord(before_apply, s);
   if (precision == FCG_FP32) {

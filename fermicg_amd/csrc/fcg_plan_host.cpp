@@ -46,10 +46,78 @@ struct LocalTask {
 
 }  // namespace
 
+namespace {
+
+void push_box(fcg_space *sp, int32_t f, const int16_t *lo, const int16_t *hi) {
+  const int N = sp->N;
+  sp->box_fock.push_back(f);
+  int64_t d = 1;
+  for (int c = 0; c < N; c++) {
+    sp->lo.push_back(lo[c]);
+    sp->hi.push_back(hi[c]);
+    sp->rank.push_back((int16_t)(hi[c] - lo[c] + 1));
+    sp->foff.push_back(-1);
+    d *= hi[c] - lo[c] + 1;
+  }
+  sp->var_off.push_back(sp->var_off.back() + d);
+}
+
+// Every configuration of Fock block f (product of M[c] states, first cluster
+// fastest) as boxes of at most FCG_MAX_BOX configurations, each a contiguous
+// run of that order.
+void push_full_block(fcg_space *sp, int32_t f, const std::vector<int> &M) {
+  const int N = sp->N;
+  int16_t lo[FCG_MAXN], hi[FCG_MAXN];
+  int k = 0;
+  int64_t pre = 1;
+  while (k < N && pre * M[k] <= FCG_MAX_BOX) pre *= M[k++];
+  if (k == N) {
+    for (int c = 0; c < N; c++) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
+    push_box(sp, f, lo, hi);
+    return;
+  }
+  const int chunk = (int)(FCG_MAX_BOX / pre);
+  std::vector<int> slow(N, 1);
+  while (true) {
+    for (int a = 1; a <= M[k]; a += chunk) {
+      for (int c = 0; c < N; c++) {
+        if (c < k) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
+        else if (c == k) { lo[c] = (int16_t)a; hi[c] = (int16_t)std::min(M[k], a + chunk - 1); }
+        else { lo[c] = hi[c] = (int16_t)slow[c]; }
+      }
+      push_box(sp, f, lo, hi);
+    }
+    int c = k + 1;
+    for (; c < N; c++) {
+      if (++slow[c] <= M[c]) break;
+      slow[c] = 1;
+    }
+    if (c >= N) break;
+  }
+}
+
+int sector_states(const fcg_ops *o, const int16_t *fk, int N, std::vector<int> *M,
+                  std::string *err) {
+  const int S = o->S;
+  for (int c = 0; c < N; c++) {
+    if (fk[2 * c] < 0 || fk[2 * c + 1] < 0 || fk[2 * c] >= S || fk[2 * c + 1] >= S) {
+      *err = "Fock sector outside the cluster";
+      return FCG_EINVAL;
+    }
+    (*M)[c] = o->nstates[(int64_t)c * S * S + fk[2 * c] * S + fk[2 * c + 1]];
+    if ((*M)[c] <= 0) {
+      *err = "Fock block uses a sector missing from ClusterOps";
+      return FCG_EMISSING_BLOCK;
+    }
+  }
+  return FCG_OK;
+}
+
+}  // namespace
+
 int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
                     const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
                     fcg_space *sp, std::string *err) {
-  const int S = o->S;
   sp->N = N;
   sp->tpsci = true;
   sp->n_fock = n_fock;
@@ -58,36 +126,14 @@ int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock
     *err = "fock_start must cover [0, n_cfg)";
     return FCG_EINVAL;
   }
-  auto push_box = [&](int32_t f, const int16_t *lo, const int16_t *hi) {
-    sp->box_fock.push_back(f);
-    int64_t d = 1;
-    for (int c = 0; c < N; c++) {
-      sp->lo.push_back(lo[c]);
-      sp->hi.push_back(hi[c]);
-      sp->rank.push_back((int16_t)(hi[c] - lo[c] + 1));
-      sp->foff.push_back(-1);
-      d *= hi[c] - lo[c] + 1;
-    }
-    sp->var_off.push_back(sp->var_off.back() + d);
-  };
   sp->var_off.assign(1, 0);
   std::vector<int> M(N);
   for (int64_t f = 0; f < n_fock; f++) {
     const int16_t *fk = fock + f * 2 * N;
+    if (int rc = sector_states(o, fk, N, &M, err)) return rc;
     int64_t prod = 1;
-    for (int c = 0; c < N; c++) {
-      if (fk[2 * c] < 0 || fk[2 * c + 1] < 0 || fk[2 * c] >= S || fk[2 * c + 1] >= S) {
-        *err = "Fock sector outside the cluster";
-        return FCG_EINVAL;
-      }
-      M[c] = o->nstates[(int64_t)c * S * S + fk[2 * c] * S + fk[2 * c + 1]];
-      if (M[c] <= 0) {
-        *err = "Fock block uses a sector missing from ClusterOps";
-        return FCG_EMISSING_BLOCK;
-      }
-      prod *= M[c];
-    }
-    int64_t s0 = fock_start[f], s1 = fock_start[f + 1];
+    for (int c = 0; c < N; c++) prod *= M[c];
+    const int64_t s0 = fock_start[f], s1 = fock_start[f + 1];
     for (int64_t i = s0; i < s1; i++)
       for (int c = 0; c < N; c++)
         if (cfg[i * N + c] < 1 || cfg[i * N + c] > M[c]) {
@@ -106,42 +152,14 @@ int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock
         }
       }
     }
-    int16_t lo[FCG_MAXN], hi[FCG_MAXN];
-    if (!full) {
+    if (full) {
+      push_full_block(sp, (int32_t)f, M);
+    } else {
+      int16_t lo[FCG_MAXN];
       for (int64_t i = s0; i < s1; i++) {
-        for (int c = 0; c < N; c++) lo[c] = hi[c] = cfg[i * N + c];
-        push_box((int32_t)f, lo, hi);
+        for (int c = 0; c < N; c++) lo[c] = cfg[i * N + c];
+        push_box(sp, (int32_t)f, lo, lo);
       }
-      continue;
-    }
-    // split the full block into boxes of at most FCG_MAX_BOX configurations,
-    // each a contiguous run of the caller's order
-    int k = 0;
-    int64_t pre = 1;
-    while (k < N && pre * M[k] <= FCG_MAX_BOX) pre *= M[k++];
-    int chunk = (k < N) ? (int)(FCG_MAX_BOX / pre) : 1;
-    std::vector<int> slow(N, 1);
-    while (true) {
-      if (k < N) {
-        for (int a = 1; a <= M[k]; a += chunk) {
-          for (int c = 0; c < N; c++) {
-            if (c < k) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
-            else if (c == k) { lo[c] = (int16_t)a; hi[c] = (int16_t)std::min(M[k], a + chunk - 1); }
-            else { lo[c] = hi[c] = (int16_t)slow[c]; }
-          }
-          push_box((int32_t)f, lo, hi);
-        }
-      } else {
-        for (int c = 0; c < N; c++) { lo[c] = 1; hi[c] = (int16_t)M[c]; }
-        push_box((int32_t)f, lo, hi);
-        break;
-      }
-      int c = k + 1;
-      for (; c < N; c++) {
-        if (++slow[c] <= M[c]) break;
-        slow[c] = 1;
-      }
-      if (c >= N) break;
     }
     if (sp->var_off.back() != s1) {
       *err = "internal: box partition does not match the configuration order";
@@ -153,23 +171,28 @@ int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock
   return FCG_OK;
 }
 
-int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp, int nbody,
-                        HostPlan *out, std::string *err) {
-  const int N = sp->N, S = o->S;
-  if (h->N != N || o->N != N) {
-    *err = "cluster counts of ham/ops/space differ";
-    return FCG_EINVAL;
+int fcg_full_space(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock, fcg_space *sp,
+                   std::string *err) {
+  sp->N = N;
+  sp->tpsci = true;
+  sp->n_fock = n_fock;
+  sp->fock.assign(fock, fock + n_fock * 2 * N);
+  sp->var_off.assign(1, 0);
+  std::vector<int> M(N);
+  for (int64_t f = 0; f < n_fock; f++) {
+    if (int rc = sector_states(o, fock + f * 2 * N, N, &M, err)) return rc;
+    push_full_block(sp, (int32_t)f, M);
   }
-  std::unordered_map<std::string, int64_t> fmap;
-  fmap.reserve(sp->n_fock * 2);
-  for (int64_t f = 0; f < sp->n_fock; f++) fmap.emplace(fock_key(&sp->fock[f * 2 * N], N), f);
-  // boxes of each Fock block
-  std::vector<std::vector<int32_t>> fboxes(sp->n_fock);
-  std::vector<int64_t> dim(sp->n_box);
+  sp->n_box = (int64_t)sp->box_fock.size();
+  sp->nvar = sp->var_off.back();
+  return FCG_OK;
+}
+
+// Validate one space's boxes against the ClusterOps state counts.
+static int check_space(const fcg_ops *o, const fcg_space *sp, std::string *err) {
+  const int N = sp->N, S = o->S;
   for (int64_t b = 0; b < sp->n_box; b++) {
-    fboxes[sp->box_fock[b]].push_back((int32_t)b);
-    dim[b] = sp->var_off[b + 1] - sp->var_off[b];
-    if (dim[b] > FCG_MAX_BOX) {
+    if (sp->var_off[b + 1] - sp->var_off[b] > FCG_MAX_BOX) {
       *err = "block dimension exceeds " + std::to_string(FCG_MAX_BOX);
       return FCG_EOVERFLOW;
     }
@@ -187,6 +210,32 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
       }
     }
   }
+  if (sp->nvar > INT32_MAX) {
+    *err = "more than 2^31 variables";
+    return FCG_EOVERFLOW;
+  }
+  return FCG_OK;
+}
+
+int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
+                        const fcg_space *ket, int nbody, HostPlan *out, std::string *err) {
+  const int N = bra->N, S = o->S;
+  if (h->N != N || o->N != N || ket->N != N) {
+    *err = "cluster counts of ham/ops/spaces differ";
+    return FCG_EINVAL;
+  }
+  if (int rc = check_space(o, bra, err)) return rc;
+  if (ket != bra)
+    if (int rc = check_space(o, ket, err)) return rc;
+  // ket Fock configurations and their boxes
+  std::unordered_map<std::string, int64_t> fmap;
+  fmap.reserve(ket->n_fock * 2);
+  for (int64_t f = 0; f < ket->n_fock; f++) fmap.emplace(fock_key(&ket->fock[f * 2 * N], N), f);
+  std::vector<std::vector<int32_t>> kboxes(ket->n_fock), bboxes(bra->n_fock);
+  for (int64_t k = 0; k < ket->n_box; k++) kboxes[ket->box_fock[k]].push_back((int32_t)k);
+  for (int64_t b = 0; b < bra->n_box; b++) bboxes[bra->box_fock[b]].push_back((int32_t)b);
+  auto dim_b = [&](int64_t b) { return bra->var_off[b + 1] - bra->var_off[b]; };
+  auto dim_k = [&](int64_t k) { return ket->var_off[k + 1] - ket->var_off[k]; };
   // terms of each TransferConfig grouped by active-cluster mask
   struct Group { uint32_t mask; std::vector<int> terms; };
   std::vector<std::vector<Group>> groups(h->n_tc);
@@ -201,14 +250,13 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
       else it->terms.push_back(t);
     }
   }
-  const int nthr = std::max(1, omp_get_max_threads());
-  std::vector<std::vector<LocalTask>> per_fock(sp->n_fock);
+  std::vector<std::vector<LocalTask>> per_fock(bra->n_fock);
   int errcode = FCG_OK;
   std::string errmsg;
-#pragma omp parallel for schedule(dynamic, 4) num_threads(nthr)
-  for (int64_t fb = 0; fb < sp->n_fock; fb++) {
-    if (errcode) continue;
-    const int16_t *F = &sp->fock[fb * 2 * N];
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t fb = 0; fb < bra->n_fock; fb++) {
+    if (errcode || bboxes[fb].empty()) continue;
+    const int16_t *F = &bra->fock[fb * 2 * N];
     std::vector<LocalTask> loc;
     int16_t fk[2 * FCG_MAXN];
     for (int g = 0; g < h->n_tc && !errcode; g++) {
@@ -221,7 +269,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
       if (!ok) continue;
       auto it = fmap.find(fock_key(fk, N));
       if (it == fmap.end()) continue;
-      int64_t fki = it->second;
+      const int64_t fki = it->second;
       for (const Group &G : groups[g]) {
         // Γ blocks and sign per term (depend only on the Fock pair)
         std::vector<TaskDev> tt;
@@ -231,9 +279,9 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
           d.meta = term_sign(h, t, fk) < 0 ? 1 : 0;
           for (int a = 0; a < 4; a++) d.blk[a] = -1;
           for (int a = 0; a < h->nbody[t]; a++) {
-            int c = h->clusters[4 * t + a];
-            int blk = o->block(c, h->ops[4 * t + a], F[2 * c] * S + F[2 * c + 1],
-                               fk[2 * c] * S + fk[2 * c + 1]);
+            const int c = h->clusters[4 * t + a];
+            const int blk = o->block(c, h->ops[4 * t + a], F[2 * c] * S + F[2 * c + 1],
+                                     fk[2 * c] * S + fk[2 * c + 1]);
             if (blk < 0) {
 #pragma omp critical
               {
@@ -259,8 +307,8 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
           }
           tt.push_back(d);
         }
-        for (int32_t B : fboxes[fb])
-          for (int32_t K : fboxes[fki]) {
+        for (int32_t B : bboxes[fb])
+          for (int32_t K : kboxes[fki]) {
             // inactive clusters: equal TuckerConfig range (reference
             // check_term); with identity factors on both sides (TPSCI boxes)
             // overlapping ranges suffice -- the states must then be equal,
@@ -269,10 +317,10 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
             for (int c = 0; c < N && eq; c++) {
               if (G.mask >> c & 1u) continue;
               const int64_t ib = (int64_t)B * N + c, ik = (int64_t)K * N + c;
-              if (sp->foff[ib] < 0 && sp->foff[ik] < 0)
-                eq = std::max(sp->lo[ib], sp->lo[ik]) <= std::min(sp->hi[ib], sp->hi[ik]);
+              if (bra->foff[ib] < 0 && ket->foff[ik] < 0)
+                eq = std::max(bra->lo[ib], ket->lo[ik]) <= std::min(bra->hi[ib], ket->hi[ik]);
               else
-                eq = sp->lo[ib] == sp->lo[ik] && sp->hi[ib] == sp->hi[ik];
+                eq = bra->lo[ib] == ket->lo[ik] && bra->hi[ib] == ket->hi[ik];
             }
             if (!eq) continue;
             for (const TaskDev &d : tt) loc.push_back({B, K, d});
@@ -289,23 +337,20 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
     return errcode;
   }
   // assemble pairs ordered by bra box then ket box
-  std::vector<int64_t> order(sp->n_fock);
-  std::iota(order.begin(), order.end(), 0);
-  out->pair_B.clear(); out->pair_K.clear(); out->pair_t0.clear(); out->tasks.clear();
-  std::vector<int64_t> cnt(sp->n_box + 1, 0);
-  size_t ntask = 0;
-  for (auto &v : per_fock) ntask += v.size();
-  out->tasks.reserve(ntask);
   struct PairRec { int32_t B, K; int64_t t0, t1; };
   std::vector<PairRec> pairs;
-  for (int64_t fb = 0; fb < sp->n_fock; fb++) {
+  std::vector<TaskDev> tasks;
+  size_t ntask = 0;
+  for (auto &v : per_fock) ntask += v.size();
+  tasks.reserve(ntask);
+  for (int64_t fb = 0; fb < bra->n_fock; fb++) {
     auto &v = per_fock[fb];
     size_t i = 0;
     while (i < v.size()) {
       size_t j = i;
-      int64_t t0 = (int64_t)out->tasks.size();
-      while (j < v.size() && v[j].B == v[i].B && v[j].K == v[i].K) out->tasks.push_back(v[j++].t);
-      pairs.push_back({v[i].B, v[i].K, t0, (int64_t)out->tasks.size()});
+      const int64_t t0 = (int64_t)tasks.size();
+      while (j < v.size() && v[j].B == v[i].B && v[j].K == v[i].K) tasks.push_back(v[j++].t);
+      pairs.push_back({v[i].B, v[i].K, t0, (int64_t)tasks.size()});
       i = j;
     }
     std::vector<LocalTask>().swap(v);
@@ -313,58 +358,56 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
   std::stable_sort(pairs.begin(), pairs.end(), [](const PairRec &a, const PairRec &b) {
     return a.B != b.B ? a.B < b.B : a.K < b.K;
   });
-  // tasks must be contiguous in pair order: rebuild the task array
-  std::vector<TaskDev> tasks2;
-  tasks2.reserve(out->tasks.size());
+  // tasks contiguous in pair order; per-task chain order; limits; flop count
   const int64_t np = (int64_t)pairs.size();
+  out->tasks.clear();
+  out->tasks.reserve(tasks.size());
   out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.resize(np + 1);
   out->pair_koff.resize(np);
-  out->row_ptr.assign(sp->n_box + 1, 0);
-  out->diag_koff.assign(sp->n_box, -1);
+  out->row_ptr.assign(bra->n_box + 1, 0);
+  out->diag_koff.assign(bra->n_box, -1);
   int64_t koff = 0;
   double flops = 0.0;
   for (int64_t p = 0; p < np; p++) {
     const PairRec &r = pairs[p];
     out->pair_B[p] = r.B;
     out->pair_K[p] = r.K;
-    out->pair_t0[p] = (int64_t)tasks2.size();
-    for (int64_t t = r.t0; t < r.t1; t++) tasks2.push_back(out->tasks[t]);
+    out->pair_t0[p] = (int64_t)out->tasks.size();
+    for (int64_t t = r.t0; t < r.t1; t++) out->tasks.push_back(tasks[t]);
     out->pair_koff[p] = koff;
-    if (r.B == r.K) out->diag_koff[r.B] = koff;
+    if (ket == bra && r.B == r.K) out->diag_koff[r.B] = koff;
     out->row_ptr[r.B + 1]++;
-    const int64_t db = dim[r.B], dk = dim[r.K];
+    const int64_t db = dim_b(r.B), dk = dim_k(r.K);
     koff += db * dk;
-    // limits of the block-build kernel and its flop count
+    const int64_t B = r.B, K = r.K;
     int64_t ssum = 0;
-    for (int c = 0; c < N; c++) {
-      int64_t B = r.B, K = r.K;
-      if (sp->lo[B * N + c] == sp->lo[K * N + c] && sp->hi[B * N + c] == sp->hi[K * N + c] &&
-          !(sp->foff[B * N + c] < 0 && sp->foff[K * N + c] < 0))
-        ssum += (int64_t)sp->rank[B * N + c] * sp->rank[K * N + c];
-    }
+    for (int c = 0; c < N; c++)
+      if (bra->lo[B * N + c] == ket->lo[K * N + c] && bra->hi[B * N + c] == ket->hi[K * N + c] &&
+          !(bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0))
+        ssum += (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
     if (ssum > FCG_LDS_S) {
       *err = "inactive-cluster overlaps exceed LDS budget";
       return FCG_EOVERFLOW;
     }
     for (int64_t t = r.t0; t < r.t1; t++) {
-      TaskDev &d = tasks2[out->pair_t0[p] + (t - r.t0)];
+      TaskDev &d = out->tasks[out->pair_t0[p] + (t - r.t0)];
       const int tm = d.term, nb = h->nbody[tm];
       int64_t np_[4], rr_[4];
       int64_t rest0 = 1;
       for (int a = 0; a < nb; a++) {
         const int c = h->clusters[4 * tm + a];
         np_[a] = (nb == 1) ? 1 : h->dims[4 * tm + a];
-        rr_[a] = (int64_t)sp->rank[(int64_t)r.B * N + c] * sp->rank[(int64_t)r.K * N + c];
+        rr_[a] = (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
         rest0 *= np_[a];
-        const int64_t nI = sp->hi[(int64_t)r.B * N + c] - sp->lo[(int64_t)r.B * N + c] + 1;
-        const int64_t nJ = sp->hi[(int64_t)r.K * N + c] - sp->lo[(int64_t)r.K * N + c] + 1;
-        const int64_t rk = sp->rank[(int64_t)r.K * N + c];
+        const int64_t nI = bra->hi[B * N + c] - bra->lo[B * N + c] + 1;
+        const int64_t nJ = ket->hi[K * N + c] - ket->lo[K * N + c] + 1;
+        const int64_t rk = ket->rank[K * N + c];
         if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) {
           *err = "Γ slice exceeds LDS budget";
           return FCG_EOVERFLOW;
         }
-        const bool ident = sp->foff[(int64_t)r.B * N + c] < 0 && sp->foff[(int64_t)r.K * N + c] < 0;
-        if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * sp->rank[(int64_t)r.B * N + c] * rk);
+        const bool ident = bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0;
+        if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * bra->rank[B * N + c] * rk);
       }
       if (rest0 > FCG_LDS_X) {
         *err = "term integrals exceed LDS budget";
@@ -397,33 +440,114 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *sp,
       flops += best_fl + 2.0 * db * dk;
     }
   }
-  out->pair_t0[np] = (int64_t)tasks2.size();
-  out->tasks.swap(tasks2);
-  for (int64_t b = 0; b < sp->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
+  out->pair_t0[np] = (int64_t)out->tasks.size();
+  for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
   // row panels: the blocks of one bra row are contiguous, so the row is a
-  // column-major db x (sum dk) panel; colidx maps its columns to variables
-  out->row_koff.assign(sp->n_box, 0);
-  out->col_ptr.assign(sp->n_box + 1, 0);
+  // column-major db x (sum dk) panel; colidx maps its columns to ket variables
+  out->row_koff.assign(bra->n_box, 0);
+  out->col_ptr.assign(bra->n_box + 1, 0);
   int64_t ncol = 0;
-  for (int64_t b = 0; b < sp->n_box; b++) {
+  for (int64_t b = 0; b < bra->n_box; b++) {
     out->col_ptr[b] = ncol;
     if (out->row_ptr[b + 1] > out->row_ptr[b]) out->row_koff[b] = out->pair_koff[out->row_ptr[b]];
-    for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) ncol += dim[out->pair_K[q]];
+    for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) ncol += dim_k(out->pair_K[q]);
   }
-  out->col_ptr[sp->n_box] = ncol;
-  if (sp->nvar > INT32_MAX) {
-    *err = "more than 2^31 variables";
-    return FCG_EOVERFLOW;
-  }
+  out->col_ptr[bra->n_box] = ncol;
   out->colidx.resize(ncol);
-  for (int64_t b = 0; b < sp->n_box; b++) {
+  for (int64_t b = 0; b < bra->n_box; b++) {
     int64_t at = out->col_ptr[b];
     for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) {
       const int32_t K = out->pair_K[q];
-      for (int64_t j = 0; j < dim[K]; j++) out->colidx[at++] = (int32_t)(sp->var_off[K] + j);
+      for (int64_t j = 0; j < dim_k(K); j++) out->colidx[at++] = (int32_t)(ket->var_off[K] + j);
     }
   }
   out->n_kel = koff;
   out->flops = flops;
+  return FCG_OK;
+}
+
+// Target Fock blocks follow compute_pt2_energy's job list
+// (src/tpsci_pt2_energy.jl:49-79): F_ket + TransferConfig with every
+// cluster sector in range and present in ClusterOps.
+int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int h0_op,
+                   fcg_space *tsp, Pt2Target *out, std::string *err) {
+  const int N = var->N, S = o->S;
+  auto key_of = [N](const int16_t *f) {
+    std::string k(2 * N, '\0');
+    for (int i = 0; i < 2 * N; i++) k[i] = (char)(f[i] + 1);
+    return k;
+  };
+  std::unordered_map<std::string, int64_t> tmap;
+  std::vector<int16_t> tfock, fx(2 * N);
+  for (int64_t f = 0; f < var->n_fock; f++) {
+    const int16_t *F = &var->fock[f * 2 * N];
+    for (int g = 0; g < h->n_tc; g++) {
+      const int16_t *tc = &h->tc[(int64_t)g * 2 * N];
+      bool ok = true;
+      for (int c = 0; c < N && ok; c++) {
+        const int a = F[2 * c] + tc[2 * c], b = F[2 * c + 1] + tc[2 * c + 1];
+        ok = a >= 0 && b >= 0 && a < S && b < S && o->nstates[(int64_t)c * S * S + a * S + b] > 0;
+        fx[2 * c] = (int16_t)a;
+        fx[2 * c + 1] = (int16_t)b;
+      }
+      if (!ok) continue;
+      if (tmap.emplace(key_of(fx.data()), (int64_t)tfock.size() / (2 * N)).second)
+        tfock.insert(tfock.end(), fx.begin(), fx.end());
+    }
+  }
+  const int64_t n_tf = (int64_t)tfock.size() / (2 * N);
+  if (int rc = fcg_full_space(o, N, n_tf, tfock.data(), tsp, err)) return rc;
+  // first variable of each target Fock block
+  std::vector<int64_t> fvar0(n_tf, -1);
+  for (int64_t b = 0; b < tsp->n_box; b++)
+    if (fvar0[tsp->box_fock[b]] < 0) fvar0[tsp->box_fock[b]] = tsp->var_off[b];
+  // variational configurations (project_out!, src/tpsci_outer.jl:869-889);
+  // full blocks are enumerated first cluster fastest
+  out->inspace.assign(tsp->nvar, 0);
+  std::vector<int> M(N);
+  for (int64_t b = 0; b < var->n_box; b++) {
+    const int16_t *F = &var->fock[(int64_t)var->box_fock[b] * 2 * N];
+    auto it = tmap.find(key_of(F));
+    if (it == tmap.end()) continue;  // no TransferConfig maps this block onto itself
+    for (int c = 0; c < N; c++) M[c] = o->nstates[(int64_t)c * S * S + F[2 * c] * S + F[2 * c + 1]];
+    const int64_t d = var->var_off[b + 1] - var->var_off[b];
+    for (int64_t e = 0; e < d; e++) {
+      int64_t r = e, lin = 0, stride = 1;
+      for (int c = 0; c < N; c++) {
+        const int rk = var->rank[b * N + c];
+        const int ic = (int)(r % rk) + var->lo[b * N + c] - 1;
+        r /= rk;
+        lin += ic * stride;
+        stride *= M[c];
+      }
+      const int64_t x = fvar0[it->second] + lin;
+      if (x < 0 || x >= tsp->nvar) {
+        *err = "internal: variational configuration outside its target block";
+        return FCG_EINVAL;
+      }
+      out->inspace[x] = 1;
+    }
+  }
+  // diagonal of H0 = sum_c op h0_op (compute_diagonal!, src/tpsci_outer.jl:735-757)
+  out->hoff.assign(tsp->n_box * N, 0);
+  out->hstride.assign(tsp->n_box * N, 0);
+  for (int64_t b = 0; b < tsp->n_box; b++) {
+    const int16_t *F = &tsp->fock[(int64_t)tsp->box_fock[b] * 2 * N];
+    for (int c = 0; c < N; c++) {
+      const int code = F[2 * c] * S + F[2 * c + 1];
+      const int blk = o->block(c, h0_op, code, code);
+      if (blk < 0) {
+        *err = "H0 block missing on cluster " + std::to_string(c);
+        return FCG_EMISSING_BLOCK;
+      }
+      const int m = o->gb_dims[3 * blk + 1];
+      if (o->gb_dims[3 * blk] != 1 || o->gb_dims[3 * blk + 2] != m || tsp->hi[b * N + c] > m) {
+        *err = "H0 block of cluster " + std::to_string(c) + " is not square over the sector";
+        return FCG_EINVAL;
+      }
+      out->hoff[b * N + c] = o->gb_off[blk] + (int64_t)(tsp->lo[b * N + c] - 1) * (m + 1);
+      out->hstride[b * N + c] = m + 1;
+    }
+  }
   return FCG_OK;
 }

@@ -3,6 +3,7 @@
     tps_ci_matvec(ci_vector, cluster_ops, clustered_ham)      tpsci_outer.jl:462
     compute_diagonal(ci_vector, cluster_ops, clustered_ham)   tpsci_outer.jl:694
     tps_ci_davidson(ci_vector, cluster_ops, clustered_ham; …) tpsci_outer.jl:364
+    compute_pt2_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:10
 
 Same arguments, same meaning, same output layout (rows in the TPSCIstate's
 flattening order, one column per root).  All arithmetic runs in
@@ -36,6 +37,7 @@ class Engine:
         self._ops = {}
         self._ham = {}
         self._plans = {}
+        self._local = {}
 
     def ops(self, cluster_ops):
         key = id(cluster_ops)
@@ -66,7 +68,19 @@ class Engine:
             self._plans[key] = pl
         return pl
 
+    def local_operator(self, clustered_ham, op_string):
+        """The 1-body part of `clustered_ham` relabelled to `op_string`
+        (extract_1body_operator), cached so its device plans are reused."""
+        from .operator import extract_1body_operator
+        key = (id(clustered_ham), op_string)
+        hit = self._local.get(key)
+        if hit is None or hit[0] is not clustered_ham:
+            hit = (clustered_ham, extract_1body_operator(clustered_ham, op_string=op_string))
+            self._local[key] = hit
+        return hit[1]
+
     def clear(self):
+        self._local.clear()
         self._plans.clear()
         self._ham.clear()
         self._ops.clear()
@@ -110,6 +124,62 @@ def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lin
     out.set_vector(v)
     out.davidson_stats = stats
     return e, out
+
+
+def orthonormalize(v):
+    """Modified Gram-Schmidt over the root columns, in place order
+    (orthonormalize!, src/type_TPSCIstate.jl:569-582)."""
+    v = np.array(v, dtype=np.float64, copy=True)
+    v[:, 0] /= np.linalg.norm(v[:, 0])
+    for r in range(1, v.shape[1]):
+        for r0 in range(r):
+            v[:, r] -= v[:, r0] * (v[:, r0] @ v[:, r])
+        v[:, r] /= np.linalg.norm(v[:, r])
+    return v
+
+
+def compute_expectation_value(ci_vector, cluster_ops, clustered_ham, engine=None):
+    """<r|H|r> per root (compute_expectation_value_parallel,
+    src/tpsci_outer.jl:601-690): sigma on the GPU, one dot product per root."""
+    eng = engine or default_engine()
+    c = ci_vector.coeffs
+    s = eng.tpsci_plan(ci_vector, cluster_ops, clustered_ham).sigma(c)
+    return np.einsum("ir,ir->r", c, s)
+
+
+def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
+                       thresh_foi=1e-9, prescreen=True, verbose=1, engine=None):
+    """Second-order energy correction e2[r] of a TPSCI state over its
+    first-order space (compute_pt2_energy, src/tpsci_pt2_energy.jl:10-175).
+
+    As in the reference: the state is orthonormalised (Gram-Schmidt over
+    roots), E0 = <r|H0|r> with H0 the clusters' local `H0` operators unless
+    given, and e2[r] = sum_x sigma[x,r]^2 / (E0[r] - H0_diag[x]) over every
+    configuration of the Fock blocks F_ket + TransferConfig that is not in
+    the variational space.  sigma over those blocks runs on the GPU without
+    screening, so the result is the reference's thresh_foi -> 0 limit;
+    `thresh_foi` and `prescreen` are accepted for call compatibility.
+    Returns e2 (length R)."""
+    from .cluster_ops import OP_NAMES
+    eng = engine or default_engine()
+    ci = _copy_state(ci_vector)
+    ci.set_vector(orthonormalize(ci.coeffs))
+    if E0 is None:
+        E0 = compute_expectation_value(ci, cluster_ops, eng.local_operator(clustered_ham, H0),
+                                       engine=eng)
+    E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
+    space = _lib.DeviceSpace.tpsci(eng.ctx, ci)
+    e2, stats = _lib.tpsci_pt2(eng.ctx, eng.ham(clustered_ham, nbody), eng.ops(cluster_ops), space,
+                               ci.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0))
+    if verbose:
+        evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
+        print(" first-order space: %d Fock blocks, %d configurations; %.3f s"
+              % (stats["n_target_fock"], stats["n_first_order"], stats["seconds"]))
+        print(" %5s %12s %12s" % ("Root", "E(0)", "E(2)"))
+        for r in range(len(e2)):
+            print(" %5d %12.8f %12.8f" % (r + 1, evar[r], evar[r] + e2[r]))
+    compute_pt2_energy.last_stats = stats
+    return e2
 
 
 def _copy_state(s):

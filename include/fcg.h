@@ -105,6 +105,13 @@ void fcg_space_destroy(fcg_space_t sp);
  * (1<<FCG_FP64)|(1<<FCG_FP32)|(1<<FCG_BF16X2) copies kept resident. */
 int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp, int nbody,
                     int precisions, fcg_plan_t *out);
+/* Hamiltonian between two spaces (rows: bra, columns: ket): sigma_bra =
+ * H c_ket, i.e. the open matvec of the reference (open_matvec_thread,
+ * src/tpsci_matvec_thread.jl:11-129) restricted to the configurations of
+ * `bra`, without screening.  fcg_diagonal and fcg_davidson need a plan from
+ * fcg_plan_create. */
+int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t bra,
+                         fcg_space_t ket, int nbody, int precisions, fcg_plan_t *out);
 void fcg_plan_destroy(fcg_plan_t plan);
 /* Plan statistics: [0] block pairs, [1] (term, block pair) tasks,
  * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us),
@@ -152,6 +159,30 @@ typedef struct {
 int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *v0,
                  const fcg_davidson_opts *opts, double *evals, double *evecs,
                  fcg_davidson_stats *stats);
+
+/* Second-order energy of a TPSCI state over its first-order space (replaces
+ * compute_pt2_energy, reference src/tpsci_pt2_energy.jl:10-175, with its
+ * per-Fock-block job _pt2_job :178-266 and sum _sum_pt2 :269-282).
+ * var: the state's TPSCI space; c: its coefficients (n_var x R, host,
+ * orthonormal columns as the reference's orthonormalize! leaves them);
+ * e0[r] = <r|H0|r> with H0 the sum of the clusters' local operator h0_op
+ * (the reference's H0 string; 1 = "Hcmf", its default).
+ * e2[r] = sum_x sigma[x,r]^2 / (e0[r] - H0_diag[x]) over every
+ * configuration x of the Fock blocks F_ket + TransferConfig outside the
+ * variational space, with sigma = H c: the reference's thresh_foi -> 0
+ * limit (no screening). */
+typedef struct {
+  int64_t n_target_fock;  /* Fock blocks of the first-order space (jobs) */
+  int64_t n_target_var;   /* configurations in those blocks */
+  int64_t n_first_order;  /* ... outside the variational space */
+  int64_t block_pairs;    /* Hamiltonian blocks built */
+  int64_t block_elems;    /* their elements */
+  double seconds;         /* wall time of the call */
+} fcg_pt2_stats;
+
+int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
+                  const double *c, const double *e0, int nbody, int h0_op, double *e2,
+                  fcg_pt2_stats *stats);
 
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */

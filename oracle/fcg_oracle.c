@@ -9,9 +9,16 @@
  *   build_full_H               src/tpsci_outer.jl:10-50
  *   compute_diagonal           src/tpsci_outer.jl:694-715
  *   FockConfig - TransferConfig  src/Indexing.jl:21-23
+ *   compute_pt2_energy         src/tpsci_pt2_energy.jl:10-175, _pt2_job :178-266,
+ *                              _sum_pt2 :269-282
+ *   contract_matvec_thread     src/tpsci_matvec_thread.jl:279-820 (prescreen=false)
+ *   _collect_significant_thread!  src/tpsci_matvec_thread.jl:830-895
+ *   project_out!               src/tpsci_outer.jl:869-889
+ *   compute_diagonal! (op)     src/tpsci_outer.jl:735-757
  */
 #include "fcg_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -292,4 +299,241 @@ int orc_tpsci_diagonal(const orc_ops *o, const orc_ham *h, const orc_space *sp, 
     }
   }
   return 0;
+}
+
+/* ---- second-order energy over the first-order space ---- */
+
+/* insert-or-find on an open-addressing table of Fock tuples stored in `keys` */
+static int64_t fock_intern(int64_t *slot, int64_t cap, int16_t *keys, int64_t *n_keys,
+                           const int16_t *f, int N) {
+  uint64_t hsh = hash_tuple(f, 2 * N) & (cap - 1);
+  while (slot[hsh] >= 0) {
+    if (memcmp(keys + slot[hsh] * 2 * N, f, sizeof(int16_t) * 2 * N) == 0) return slot[hsh];
+    hsh = (hsh + 1) & (cap - 1);
+  }
+  memcpy(keys + *n_keys * 2 * N, f, sizeof(int16_t) * 2 * N);
+  slot[hsh] = *n_keys;
+  return (*n_keys)++;
+}
+
+/* contract_matvec_thread for one term and one ket configuration
+ * (src/tpsci_matvec_thread.jl:279-820 with prescreen=false): the
+ * integrals contracted with the ket column of each active cluster's Γ,
+ * giving T[I1..Ik] over the active clusters' bra states; elements above
+ * thresh / max|c_ket| (_collect_significant_thread!, :830-895) are added,
+ * times the state sign and c_ket, to the dense sigma of the target Fock
+ * block.  Returns -3 when a 1-body Γ block is missing (the reference reads
+ * it unconditionally); multi-body terms with a missing block are skipped
+ * (their `haskey(...) || return`). */
+static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx,
+                    const int16_t *fk, const int16_t *ket, const double *cket /* R */, int R,
+                    double thresh, const int64_t *stride, int64_t nx, double *sig, double **w1,
+                    double **w2, int64_t *wcap) {
+  const int N = o->n_clusters;
+  const int k = h->nbody[t];
+  const int32_t *cl = h->clusters + 4 * t;
+  const int32_t *op = h->ops + 4 * t;
+  const double *col[4];
+  int np[4], mb[4];
+  for (int a = 0; a < k; a++) {
+    const int32_t *d;
+    const double *blk = gamma_block(o, cl[a], op[a], fx, fk, &d);
+    if (!blk) return k == 1 ? -3 : 0;
+    np[a] = d[0];
+    mb[a] = d[1];
+    col[a] = blk + (int64_t)d[0] * d[1] * (ket[cl[a]] - 1);
+  }
+  /* workspace: the largest intermediate of the contraction below */
+  int64_t need = k == 1 ? mb[0] : 1;
+  if (k > 1) {
+    for (int m = k; m >= 0; m--) {
+      int64_t sz = 1;
+      for (int a = 0; a < k; a++) sz *= a < m ? np[a] : mb[a];
+      need = sz > need ? sz : need;
+    }
+  }
+  if (need > *wcap) {
+    free(*w1);
+    free(*w2);
+    *w1 = (double *)malloc(sizeof(double) * need);
+    *w2 = (double *)malloc(sizeof(double) * need);
+    *wcap = need;
+  }
+  /* Y[p1..pm, I(m+1)..Ik], contracted from the last active cluster inwards */
+  double *Y = *w1, *Z = *w2;
+  int64_t ny;
+  if (k == 1) {
+    ny = mb[0];
+    for (int64_t i = 0; i < ny; i++) Y[i] = col[0][i];
+  } else {
+    const double *V = h->ints + h->ioff[t];
+    ny = 1;
+    for (int a = 0; a < k; a++) ny *= np[a];
+    for (int64_t i = 0; i < ny; i++) Y[i] = V[i];
+    int64_t B = 1;
+    for (int m = k - 1; m >= 0; m--) {
+      int64_t A = 1;
+      for (int a = 0; a < m; a++) A *= np[a];
+      const int P = np[m], M = mb[m];
+      for (int64_t b = 0; b < B; b++)
+        for (int i = 0; i < M; i++)
+          for (int64_t a = 0; a < A; a++) {
+            double acc = 0.0;
+            for (int p = 0; p < P; p++) acc += Y[a + A * (p + (int64_t)P * b)] * col[m][p + (int64_t)P * i];
+            Z[a + A * (i + (int64_t)M * b)] = acc;
+          }
+      B *= M;
+      double *tmp = Y;
+      Y = Z;
+      Z = tmp;
+    }
+    ny = B;
+  }
+  const double sgn = state_sign(h, t, fk);
+  double cmax = 0.0;
+  for (int r = 0; r < R; r++) cmax = fabs(cket[r]) > cmax ? fabs(cket[r]) : cmax;
+  const double tcur = thresh / cmax;
+  /* base linear index: the ket configuration with the active clusters zeroed */
+  int64_t base = 0;
+  for (int c = 0; c < N; c++) {
+    int act = 0;
+    for (int a = 0; a < k; a++) act |= cl[a] == c;
+    if (!act) base += (int64_t)(ket[c] - 1) * stride[c];
+  }
+  for (int64_t e = 0; e < ny; e++) {
+    const double v = Y[e];
+    if (!(fabs(v) > tcur)) continue;
+    int64_t lin = base, r0 = e;
+    for (int a = 0; a < k; a++) {
+      lin += (r0 % mb[a]) * stride[cl[a]];
+      r0 /= mb[a];
+    }
+    for (int r = 0; r < R; r++) sig[lin + nx * r] += sgn * v * cket[r];
+  }
+  return 0;
+}
+
+int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
+                  const double *c, const double *e0, int nbody, int h0_op, double thresh,
+                  double *e2, int64_t *n_jobs, int nthreads) {
+  const int N = o->n_clusters, S = o->S;
+  if (N > MAXN || R < 1 || R > 64) return -1;
+  /* jobs: target Fock blocks F_ket + TransferConfig (tpsci_pt2_energy.jl:49-79) */
+  const int64_t n_cand = sp->n_fock * (int64_t)h->n_tc;
+  int64_t cap = 16;
+  while (cap < 2 * n_cand) cap <<= 1;
+  int64_t *slot = (int64_t *)malloc(sizeof(int64_t) * cap);
+  for (int64_t i = 0; i < cap; i++) slot[i] = -1;
+  int16_t *jfock = (int16_t *)malloc(sizeof(int16_t) * 2 * N * (n_cand > 0 ? n_cand : 1));
+  int64_t *cand_job = (int64_t *)malloc(sizeof(int64_t) * (n_cand > 0 ? n_cand : 1));
+  int64_t nj = 0;
+  for (int64_t f = 0; f < sp->n_fock; f++) {
+    const int16_t *fk = sp->fock + f * 2 * N;
+    for (int g = 0; g < h->n_tc; g++) {
+      int16_t fx[2 * MAXN];
+      int ok = 1;
+      for (int i = 0; i < 2 * N && ok; i++) {
+        fx[i] = fk[i] + h->tc[(int64_t)g * 2 * N + i];
+        ok = fx[i] >= 0 && fx[i] < S;
+      }
+      for (int ci = 0; ci < N && ok; ci++) {
+        const int32_t *d;
+        ok = gamma_block(o, ci, 0, fx, fx, &d) != NULL;
+      }
+      cand_job[f * h->n_tc + g] = ok ? fock_intern(slot, cap, jfock, &nj, fx, N) : -1;
+    }
+  }
+  fock_map vm;
+  fock_map_build(&vm, sp->fock, sp->n_fock, N);
+  double *je2 = (double *)calloc((size_t)(nj > 0 ? nj : 1) * R, sizeof(double));
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t j = 0; j < nj; j++) {
+    const int16_t *fx = jfock + j * 2 * N;
+    int64_t stride[MAXN], nx = 1;
+    int M[MAXN];
+    for (int ci = 0; ci < N; ci++) {
+      const int32_t *d = NULL;
+      if (!gamma_block(o, ci, 0, fx, fx, &d)) continue; /* excluded when the job was formed */
+      M[ci] = d[1];
+      stride[ci] = nx;
+      nx *= M[ci];
+    }
+    double *sig = (double *)calloc((size_t)nx * R, sizeof(double));
+    double *w1 = NULL, *w2 = NULL;
+    int64_t wcap = 0;
+    double cket[64];
+    int rc = 0;
+    /* _pt2_job: every (TransferConfig group, ket Fock block) feeding F_x */
+    for (int64_t f = 0; f < sp->n_fock && !rc; f++) {
+      for (int g = 0; g < h->n_tc && !rc; g++) {
+        if (cand_job[f * h->n_tc + g] != j) continue;
+        const int16_t *fk = sp->fock + f * 2 * N;
+        for (int t = h->tc_start[g]; t < h->tc_start[g + 1] && !rc; t++) {
+          if (h->nbody[t] > nbody) continue;
+          for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
+            for (int r = 0; r < R; r++) cket[r] = c[(int64_t)r * sp->n_cfg + kj];
+            rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, sig,
+                          &w1, &w2, &wcap);
+          }
+        }
+      }
+    }
+    /* project_out! (src/tpsci_outer.jl:869-889): drop variational configurations */
+    unsigned char *out = (unsigned char *)calloc((size_t)nx, 1);
+    const int64_t vf = fock_map_find(&vm, fx);
+    if (vf >= 0)
+      for (int64_t i = sp->fock_start[vf]; i < sp->fock_start[vf + 1]; i++) {
+        int64_t lin = 0;
+        for (int ci = 0; ci < N; ci++) lin += (int64_t)(sp->cfg[i * N + ci] - 1) * stride[ci];
+        out[lin] = 1;
+      }
+    /* compute_diagonal! with the H0 operator (src/tpsci_outer.jl:735-757) */
+    const double *hb[MAXN];
+    for (int ci = 0; ci < N && !rc; ci++) {
+      const int32_t *d;
+      hb[ci] = gamma_block(o, ci, h0_op, fx, fx, &d);
+      if (!hb[ci]) rc = -3;
+    }
+    /* _sum_pt2 (src/tpsci_pt2_energy.jl:269-282) */
+    if (!rc)
+      for (int64_t x = 0; x < nx; x++) {
+        if (out[x]) continue;
+        double hd = 0.0;
+        int64_t rem = x;
+        for (int ci = 0; ci < N; ci++) {
+          const int64_t i = rem % M[ci];
+          rem /= M[ci];
+          hd += hb[ci][i * (M[ci] + 1)];
+        }
+        for (int r = 0; r < R; r++) {
+          const double s = sig[x + nx * r];
+          je2[j * R + r] += s * s / (e0[r] - hd);
+        }
+      }
+    if (rc) err = rc;
+    free(out);
+    free(w1);
+    free(w2);
+    free(sig);
+  }
+  for (int r = 0; r < R; r++) {
+    double s = 0.0, comp = 0.0;
+    for (int64_t j = 0; j < nj; j++) {
+      const double v = je2[j * R + r], t = s + v;
+      comp += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    e2[r] = s + comp;
+  }
+  if (n_jobs) *n_jobs = nj;
+  free(je2);
+  free(vm.slot);
+  free(cand_job);
+  free(jfock);
+  free(slot);
+  return err;
 }

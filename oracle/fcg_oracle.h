@@ -63,6 +63,19 @@ int orc_tpsci_sigma_rows(const orc_ops *ops, const orc_ham *h, const orc_space *
                          int R, const double *c, const int64_t *rows, int64_t n_rows,
                          double *sigma_rows, int nthreads);
 
+/* Reference compute_pt2_energy (src/tpsci_pt2_energy.jl:10-175) from the
+ * point where the state is orthonormal and E0 = <r|H0|r> is known: jobs
+ * F_x = F_ket + TransferConfig (kept when every cluster sector has an "H"
+ * block), per job the open matvec of every term (nbody or fewer clusters)
+ * on every ket configuration with the element threshold `thresh` of
+ * _collect_significant_thread! (prescreen=false), project_out! of the
+ * variational configurations, Hd from the diagonal of op h0_op (1 = Hcmf)
+ * and e2[r] = sum_x sigma[x,r]^2 / (e0[r] - Hd[x]).  Jobs summed in order
+ * (compensated).  Returns 0, -3 on a missing 1-body or H0 block. */
+int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
+                  const double *c, const double *e0, int nbody, int h0_op, double thresh,
+                  double *e2, int64_t *n_jobs, int nthreads);
+
 #endif
 
 #ifndef FCG_ORACLE_BST

@@ -47,6 +47,8 @@ def lib():
         L.orc_tpsci_diagonal.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64)]
         L.orc_tpsci_sigma_rows.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64),
                                            P(i64), i64, P(f64), ctypes.c_int]
+        L.orc_tpsci_pt2.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64), P(f64),
+                                    ctypes.c_int, ctypes.c_int, f64, P(f64), P(i64), ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -121,6 +123,23 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle build_full_H failed: %d" % rc)
         return H
+
+    def pt2(self, c, e0, nbody=4, h0_op=1, thresh=0.0, nthreads=0):
+        """compute_pt2_energy restated (prescreen=false); returns (e2, n_jobs)."""
+        c = np.asarray(c, dtype=np.float64)
+        if c.ndim == 1:
+            c = c[:, None]
+        R = c.shape[1]
+        cc = np.ascontiguousarray(c.T)
+        e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (R,)))
+        e2 = np.zeros(R)
+        nj = i64()
+        rc = lib().orc_tpsci_pt2(ctypes.byref(self.ops), ctypes.byref(self.ham),
+                                 ctypes.byref(self.space), R, _p(cc, f64), _p(e0, f64), nbody,
+                                 h0_op, thresh, _p(e2, f64), ctypes.byref(nj), nthreads)
+        if rc:
+            raise RuntimeError("oracle pt2 failed: %d" % rc)
+        return e2, nj.value
 
     def diagonal(self):
         hd = np.zeros(self.n_cfg)
