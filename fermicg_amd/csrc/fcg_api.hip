@@ -371,14 +371,25 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_pair_K, hp.pair_K.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_pair_t0, hp.pair_t0.data(), pl->n_pair + 1));
   PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
+  PTRY(dev_upload(ctx, &pl->d_pair_col, hp.pair_col.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
   PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
+  PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
   PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
   PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
   if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
   if (pl->precisions & (1 << FCG_BF16X2)) PTRY(dev_alloc(ctx, &pl->d_Kbf, pl->n_kel));
+  if (pl->n_kel > 0) {
+    // zero columns pad each row panel to whole quads
+    const hipError_t e = hipMemsetAsync(pl->d_K64, 0, sizeof(double) * pl->n_kel, ctx->stream);
+    if (e != hipSuccess) {
+      ctx->err = std::string("block clear: ") + hipGetErrorString(e);
+      fcg_plan_destroy(pl);
+      return FCG_EDEVICE;
+    }
+  }
   if (pl->n_pair > 0) {
     hipError_t e = fcg_launch_build(pl, h, o, ctx->stream);
     if (e == hipSuccess) e = fcg_launch_convert(pl, ctx->stream);
@@ -392,6 +403,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   // device copies of the task bookkeeping are only needed for the build
   dev_free(pl->d_tasks);
   dev_free(pl->d_pair_t0);
+  dev_free(pl->d_pair_col);
   pl->build_us =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   *out = pl;
@@ -418,8 +430,10 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   dev_free(pl->d_pair_K);
   dev_free(pl->d_pair_t0);
   dev_free(pl->d_pair_koff);
+  dev_free(pl->d_pair_col);
   dev_free(pl->d_tasks);
   dev_free(pl->d_diag_koff);
+  dev_free(pl->d_diag_col);
   dev_free(pl->d_row_koff);
   dev_free(pl->d_col_ptr);
   dev_free(pl->d_colidx);

@@ -18,6 +18,16 @@ constexpr int FCG_LDS_X = 4096;       // chain intermediates (doubles)
 constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
+constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
+
+// Row panel storage.  The Hamiltonian blocks of one bra box (db rows) sit
+// side by side as a db x ncol panel, ncol a multiple of FCG_QUAD; the panel
+// is stored as consecutive column quads, each quad row-major (db rows of
+// FCG_QUAD consecutive columns), so one 16-byte FP32 load reads four columns
+// of one row.
+__host__ __device__ inline int64_t panel_index(int db, int row, int64_t col) {
+  return (int64_t)FCG_QUAD * db * (col / FCG_QUAD) + (int64_t)FCG_QUAD * row + col % FCG_QUAD;
+}
 
 #define FCG_HIP(call, ctx)                                                        \
   do {                                                                            \
@@ -112,9 +122,11 @@ struct fcg_plan {
   int64_t *d_row_ptr = nullptr;   // [n_box+1] pairs of each bra box
   int32_t *d_pair_B = nullptr, *d_pair_K = nullptr;
   int64_t *d_pair_t0 = nullptr;   // [n_pair+1]
-  int64_t *d_pair_koff = nullptr; // [n_pair]
+  int64_t *d_pair_koff = nullptr; // [n_pair] panel of the pair's bra box
+  int32_t *d_pair_col = nullptr;  // [n_pair] first panel column of the pair
   TaskDev *d_tasks = nullptr;
-  int64_t *d_diag_koff = nullptr; // [n_box] offset of (B,B) block or -1
+  int64_t *d_diag_koff = nullptr; // [n_box] panel holding the (B,B) block or -1
+  int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
   int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
   int64_t *d_col_ptr = nullptr;   // [n_box+1] columns of each bra row
   int32_t *d_colidx = nullptr;    // [n_cols] column -> variable index
@@ -130,7 +142,7 @@ struct fcg_plan {
 struct HostPlan {
   std::vector<int32_t> pair_B, pair_K;
   std::vector<int64_t> pair_t0, pair_koff, row_ptr, diag_koff, row_koff, col_ptr;
-  std::vector<int32_t> colidx;
+  std::vector<int32_t> pair_col, diag_col, colidx;
   std::vector<TaskDev> tasks;
   int64_t n_kel = 0;
   double flops = 0.0;

@@ -14,9 +14,9 @@
 //                 reference's contract_matrix_element (src/tpsci_inner.jl:
 //                 10-216).
 // k_apply         sigma = H c: one wavefront per bra box streams that box's
-//                 row of blocks, stored back to back as one column-major
-//                 panel (db x sum dk), with a column -> variable gather of c.
-//                 HBM-bound; owner-computes with a fixed-order butterfly
+//                 row of blocks, stored side by side as one panel (db x sum
+//                 dk, in column quads, panel_index), with a column -> variable
+//                 gather of c.  HBM-bound; owner-computes with a fixed-order
 //                 reduction, so results are bit-reproducible.
 #include "fcg_internal.h"
 
@@ -25,6 +25,7 @@ namespace {
 struct BuildArgs {
   const int32_t *pair_B, *pair_K;
   const int64_t *pair_t0, *pair_koff;
+  const int32_t *pair_col;
   const TaskDev *tasks;
   const TermDev *terms;
   const double *ints;
@@ -228,10 +229,11 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
     __syncthreads();
   }
   double *Kout = a.K + a.pair_koff[p];
+  const int64_t col0 = a.pair_col[p];
 #pragma unroll
   for (int k = 0; k < KPT; k++) {
     const int e = tid + BT * k;
-    if (e < db * dk) Kout[e] = acc[k];
+    if (e < db * dk) Kout[panel_index(db, e % db, col0 + e / db)] = acc[k];
   }
 }
 
@@ -243,7 +245,7 @@ __device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 
-constexpr int APPLY_UNROLL = 16;
+constexpr int APPLY_UNROLL = 8;
 
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
@@ -274,15 +276,50 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront per bra box b.  The row's panel (db x ncol, column-major) is
-// walked in chunks of 64 columns: lane l loads the column index and the R
-// vector entries of column base+l (coalesced, one load per column rather
-// than per lane) one chunk ahead, and stages them in a per-wave LDS buffer;
-// lane (i, s) with l = i + db*s then multiplies panel row i of columns s,
-// s+sl, s+2sl, ... (sl = 64/db) by the staged entries (one vector LDS read
-// per element, broadcast across the lanes of a column).  Block loads are
-// predicated and unrolled APPLY_UNROLL times.  Slices are summed in a fixed
-// order through LDS, so results are reproducible bit for bit.
+// One column quad of one panel row as loaded from HBM (16 or 32 bytes).
+template <typename TK> struct RawQuad;
+template <> struct RawQuad<float> {
+  float4 v;
+  __device__ void load(const float *p) { v = *reinterpret_cast<const float4 *>(p); }
+  __device__ void zero() { v = make_float4(0.f, 0.f, 0.f, 0.f); }
+  __device__ void get(float (&k)[4]) const { k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w; }
+};
+template <> struct RawQuad<uint32_t> {
+  uint4 v;
+  __device__ void load(const uint32_t *p) { v = *reinterpret_cast<const uint4 *>(p); }
+  __device__ void zero() { v = make_uint4(0u, 0u, 0u, 0u); }
+  __device__ void get(float (&k)[4]) const {
+    k[0] = kval(v.x); k[1] = kval(v.y); k[2] = kval(v.z); k[3] = kval(v.w);
+  }
+};
+template <> struct RawQuad<double> {
+  double2 a, b;
+  __device__ void load(const double *p) {
+    a = reinterpret_cast<const double2 *>(p)[0];
+    b = reinterpret_cast<const double2 *>(p)[1];
+  }
+  __device__ void zero() { a = b = make_double2(0.0, 0.0); }
+  __device__ void get(double (&k)[4]) const { k[0] = a.x; k[1] = a.y; k[2] = b.x; k[3] = b.y; }
+};
+
+// Column quads staged per chunk: the per-wave stage holds 4*CG*R entries
+// within 8 KB, and at least the 64*R partial sums of the final reduction.
+template <typename TC, int R>
+constexpr int stage_quads() {
+  constexpr int q = 8192 / (FCG_QUAD * R * (int)sizeof(TC));
+  return q > 64 ? 64 : q;
+}
+
+// sigma[var[b]+i, r] = sum_j panel_b[i, j] c[colidx_b[j], r] for every bra
+// box b.  One wavefront per bra box; lane l = i + db*s (s < sl = 64/db) owns
+// panel row i and every sl-th column quad, loading each quad with one
+// 16-byte (FP32, BF16X2) or two 16-byte (FP64) loads; consecutive lanes read
+// consecutive quads, so a wave streams contiguous panel memory.  The column
+// indices and vector entries of the next CG quads are gathered cooperatively
+// (one quad per lane) one chunk ahead and staged in a per-wave LDS buffer,
+// read back with vector LDS loads broadcast across the lanes of a quad.
+// Slices are summed in a fixed order through LDS, so results are
+// reproducible bit for bit.
 template <typename TK, typename TA, typename TC, int R>
 __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__restrict__ var,
                                                const int64_t *__restrict__ row_koff,
@@ -291,8 +328,10 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
                                                const TK *__restrict__ K, const TC *__restrict__ c,
                                                int64_t nvar_c, int64_t nvar_s,
                                                double *__restrict__ sigma) {
-  __shared__ TA red[4][64 * R];
-  __shared__ __align__(16) TC stage[4][64 * R];
+  static_assert(sizeof(TA) == sizeof(TC), "stage buffer doubles as reduction buffer");
+  constexpr int CG = stage_quads<TC, R>();
+  static_assert(CG * FCG_QUAD >= 64, "reduction needs 64*R staged entries");
+  __shared__ __align__(16) TC stage[4][CG * FCG_QUAD * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t b = (int64_t)blockIdx.x * 4 + wave;
   const bool live = b < n_box;
@@ -306,56 +345,69 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
   TC *st = stage[wave];
   if (live) {  // uniform per wave
     const int64_t c0 = col_ptr[b];
-    const int ncol = (int)(col_ptr[b + 1] - c0);
-    const TK *Kr = K + row_koff[b] + i;
+    const int ngrp = (int)((col_ptr[b + 1] - c0) / FCG_QUAD);
+    const TK *Kr = K + row_koff[b] + FCG_QUAD * i;
     const int32_t *ci = colidx + c0;
-    const int nk = (64 + sl - 1) / sl;  // column steps per chunk (uniform)
-    TC nv[R];
-    {
-      const int32_t col = lane < ncol ? ci[lane] : 0;
+    const int nk = (CG + sl - 1) / sl;  // quad steps per chunk (uniform)
+    TC nv[FCG_QUAD][R];
+    auto gather = [&](int g) {
+      const bool ok = lane < CG && g < ngrp;
+      const int4 idx = ok ? *reinterpret_cast<const int4 *>(ci + (int64_t)FCG_QUAD * g)
+                          : make_int4(0, 0, 0, 0);
+      const int col[FCG_QUAD] = {idx.x, idx.y, idx.z, idx.w};
 #pragma unroll
-      for (int r = 0; r < R; r++) nv[r] = lane < ncol ? c[(int64_t)r * nvar_c + col] : (TC)0;
-    }
-    for (int base = 0; base < ncol; base += 64) {
+      for (int q = 0; q < FCG_QUAD; q++)
+#pragma unroll
+        for (int r = 0; r < R; r++) nv[q][r] = ok ? c[(int64_t)r * nvar_c + col[q]] : (TC)0;
+    };
+    gather(lane);
+    for (int base = 0; base < ngrp; base += CG) {
       wave_sync();
+      if (lane < CG) {
 #pragma unroll
-      for (int r = 0; r < R; r++) st[lane * R + r] = nv[r];
-      wave_sync();
-      {  // prefetch the next chunk's column entries
-        const int jl = base + 64 + lane;
-        const int32_t col = jl < ncol ? ci[jl] : 0;
+        for (int q = 0; q < FCG_QUAD; q++)
 #pragma unroll
-        for (int r = 0; r < R; r++) nv[r] = jl < ncol ? c[(int64_t)r * nvar_c + col] : (TC)0;
+          for (int r = 0; r < R; r++) st[(FCG_QUAD * lane + q) * R + r] = nv[q][r];
       }
-      const int lim = act ? min(64, ncol - base) : 0;
-      const TK *Kc = Kr + (uint32_t)(db * base);
+      wave_sync();
+      gather(base + CG + lane);  // next chunk, in flight during this one
+      const int lim = act ? min(CG, ngrp - base) : 0;
+      const TK *Kc = Kr + (int64_t)FCG_QUAD * db * base;
       for (int k0 = 0; k0 < nk; k0 += APPLY_UNROLL) {
-        TK kv[APPLY_UNROLL];
+        RawQuad<TK> kv[APPLY_UNROLL];
 #pragma unroll
         for (int u = 0; u < APPLY_UNROLL; u++) {
           const int j = s + sl * (k0 + u);
-          kv[u] = j < lim ? Kc[(uint32_t)(db * j)] : (TK)0;
+          if (j < lim) kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * j));
+          else kv[u].zero();
         }
 #pragma unroll
         for (int u = 0; u < APPLY_UNROLL; u++) {
-          const int j = min(s + sl * (k0 + u), 63);
-          const TA k = kval(kv[u]);
-          TC cvj[R];
-          stage_read<TC, R>(st + j * R, cvj);
+          const int j = min(s + sl * (k0 + u), CG - 1);
+          TA kq[FCG_QUAD];
+          kv[u].get(kq);
 #pragma unroll
-          for (int r = 0; r < R; r++) acc[r] += k * (TA)cvj[r];
+          for (int q = 0; q < FCG_QUAD; q++) {
+            TC cvj[R];
+            stage_read<TC, R>(st + (FCG_QUAD * j + q) * R, cvj);
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r] += kq[q] * (TA)cvj[r];
+          }
         }
       }
     }
   }
+  // fixed-order slice reduction in this wave's stage buffer
+  wave_sync();
+  TA *red = reinterpret_cast<TA *>(st);
 #pragma unroll
-  for (int r = 0; r < R; r++) red[wave][lane * R + r] = act ? acc[r] : (TA)0;
-  __syncthreads();
+  for (int r = 0; r < R; r++) red[lane * R + r] = act ? acc[r] : (TA)0;
+  wave_sync();
   if (live && lane < db) {
 #pragma unroll
     for (int r = 0; r < R; r++) {
       TA v = (TA)0;
-      for (int s2 = 0; s2 < sl; s2++) v += red[wave][(lane + db * s2) * R + r];
+      for (int s2 = 0; s2 < sl; s2++) v += red[(lane + db * s2) * R + r];
       sigma[(int64_t)r * nvar_s + var[b] + lane] = (double)v;
     }
   }
@@ -389,14 +441,14 @@ __global__ void k_to_f32(int64_t n, const double *__restrict__ x, float *__restr
 }
 
 __global__ void k_diag(int64_t n_box, const int64_t *__restrict__ var,
-                       const int64_t *__restrict__ dkoff, const double *__restrict__ K,
-                       double *__restrict__ hd) {
+                       const int64_t *__restrict__ dkoff, const int32_t *__restrict__ dcol,
+                       const double *__restrict__ K, double *__restrict__ hd) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n_box) return;
   const int64_t v0 = var[b];
   const int db = (int)(var[b + 1] - v0);
   for (int i = 0; i < db; i++)
-    hd[v0 + i] = dkoff[b] >= 0 ? K[dkoff[b] + i + (int64_t)db * i] : 0.0;
+    hd[v0 + i] = dkoff[b] >= 0 ? K[dkoff[b] + panel_index(db, i, (int64_t)dcol[b] + i)] : 0.0;
 }
 
 }  // namespace
@@ -409,6 +461,7 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.pair_K = pl->d_pair_K;
   a.pair_t0 = pl->d_pair_t0;
   a.pair_koff = pl->d_pair_koff;
+  a.pair_col = pl->d_pair_col;
   a.tasks = pl->d_tasks;
   a.terms = h->d_terms;
   a.ints = h->d_ints;
@@ -488,6 +541,6 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s) {
   const fcg_space *sp = pl->sp;
   hipLaunchKernelGGL(k_diag, dim3((unsigned)((sp->n_box + 255) / 256)), dim3(256), 0, s,
-                     sp->n_box, sp->d_var, pl->d_diag_koff, pl->d_K64, hd);
+                     sp->n_box, sp->d_var, pl->d_diag_koff, pl->d_diag_col, pl->d_K64, hd);
   return hipGetLastError();
 }

@@ -364,9 +364,11 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   out->tasks.reserve(tasks.size());
   out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.resize(np + 1);
   out->pair_koff.resize(np);
+  out->pair_col.resize(np);
   out->row_ptr.assign(bra->n_box + 1, 0);
   out->diag_koff.assign(bra->n_box, -1);
-  int64_t koff = 0;
+  out->diag_col.assign(bra->n_box, 0);
+  std::vector<int64_t> row_cols(bra->n_box, 0);
   double flops = 0.0;
   for (int64_t p = 0; p < np; p++) {
     const PairRec &r = pairs[p];
@@ -374,11 +376,15 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     out->pair_K[p] = r.K;
     out->pair_t0[p] = (int64_t)out->tasks.size();
     for (int64_t t = r.t0; t < r.t1; t++) out->tasks.push_back(tasks[t]);
-    out->pair_koff[p] = koff;
-    if (ket == bra && r.B == r.K) out->diag_koff[r.B] = koff;
-    out->row_ptr[r.B + 1]++;
     const int64_t db = dim_b(r.B), dk = dim_k(r.K);
-    koff += db * dk;
+    if (row_cols[r.B] + dk > INT32_MAX / FCG_MAX_BOX) {
+      *err = "row panel too wide";
+      return FCG_EOVERFLOW;
+    }
+    out->pair_col[p] = (int32_t)row_cols[r.B];
+    if (ket == bra && r.B == r.K) out->diag_col[r.B] = (int32_t)row_cols[r.B];
+    row_cols[r.B] += dk;
+    out->row_ptr[r.B + 1]++;
     const int64_t B = r.B, K = r.K;
     int64_t ssum = 0;
     for (int c = 0; c < N; c++)
@@ -442,18 +448,25 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   }
   out->pair_t0[np] = (int64_t)out->tasks.size();
   for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
-  // row panels: the blocks of one bra row are contiguous, so the row is a
-  // column-major db x (sum dk) panel; colidx maps its columns to ket variables
+  // row panels: the blocks of one bra row side by side form a db x ncol
+  // panel (ncol padded to a multiple of FCG_QUAD with zero columns), stored
+  // in column quads (panel_index); colidx maps its columns to ket variables
   out->row_koff.assign(bra->n_box, 0);
   out->col_ptr.assign(bra->n_box + 1, 0);
-  int64_t ncol = 0;
+  int64_t ncol = 0, koff = 0;
   for (int64_t b = 0; b < bra->n_box; b++) {
+    const int64_t w = (row_cols[b] + FCG_QUAD - 1) / FCG_QUAD * FCG_QUAD;
     out->col_ptr[b] = ncol;
-    if (out->row_ptr[b + 1] > out->row_ptr[b]) out->row_koff[b] = out->pair_koff[out->row_ptr[b]];
-    for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) ncol += dim_k(out->pair_K[q]);
+    out->row_koff[b] = koff;
+    if (ket == bra)
+      for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++)
+        if (out->pair_K[q] == b) out->diag_koff[b] = koff;
+    ncol += w;
+    koff += dim_b(b) * w;
   }
   out->col_ptr[bra->n_box] = ncol;
-  out->colidx.resize(ncol);
+  for (int64_t p = 0; p < np; p++) out->pair_koff[p] = out->row_koff[out->pair_B[p]];
+  out->colidx.assign(ncol, 0);
   for (int64_t b = 0; b < bra->n_box; b++) {
     int64_t at = out->col_ptr[b];
     for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) {
