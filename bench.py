@@ -186,6 +186,16 @@ def main():
     bytes_alg = ps["block_elems"] * esz + 4.0 * ps["panel_cols"] + n * R * (csz + 8)
     k_ms = kernel_ms[prec]["apply_kernel_ms"]
     achieved = bytes_alg / (k_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured by the rocprofv3 --pmc passes of this
+    # workload (committed under profiles/, copy-calibrated); null without one
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "%s_apply_traffic.json" % meta["workload"])
+    if os.path.exists(tpath):
+        with open(tpath) as fh_:
+            tj = json.load(fh_)
+        if prec in tj.get("bytes_per_launch", {}):
+            traffic = tj["bytes_per_launch"][prec]
+            traffic_src = os.path.relpath(tpath, ROOT)
     out = {
         "metric": "sigma builds/s (mixed FP32 vs FP64 GPU at |dE|<=1 uEh)",
         "value": rates[prec],
@@ -211,7 +221,8 @@ def main():
         "abs_dE_mixed_vs_fp64": dE,
         "parity_rel_vs_cpu_oracle": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "bytes_algorithmic": bytes_alg,
                      "kernel": "k_apply_f32 (block-sparse H c)" if prec != "fp64" else "k_apply_f64"},
         "cpu_baseline": cpu,
     }

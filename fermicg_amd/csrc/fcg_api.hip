@@ -377,6 +377,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
+  PTRY(dev_upload(ctx, &pl->d_order, hp.order.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
   PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
   if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
@@ -436,11 +437,12 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   dev_free(pl->d_diag_col);
   dev_free(pl->d_row_koff);
   dev_free(pl->d_col_ptr);
+  dev_free(pl->d_order);
   dev_free(pl->d_colidx);
   dev_free(pl->d_K64);
   dev_free(pl->d_K32);
   dev_free(pl->d_Kbf);
-  dev_free(pl->d_c32);
+  if (pl->d_ci) (void)hipFree(pl->d_ci);
   delete pl;
 }
 
@@ -464,15 +466,18 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
   }
   (void)hipSetDevice(ctx->device);
   const int64_t n = (int64_t)R * pl->ksp->nvar;
-  if (precision != FCG_FP64 && pl->c32_elems < n) {
-    dev_free(pl->d_c32);
-    pl->c32_elems = 0;
-    TRY(dev_alloc(ctx, &pl->d_c32, n));
-    pl->c32_elems = n;
+  if (pl->ci_bytes < n * 8) {  // interleaved copy of c, sized for the FP64 form
+    if (pl->d_ci) (void)hipFree(pl->d_ci);
+    pl->d_ci = nullptr;
+    pl->ci_bytes = 0;
+    double *p = nullptr;
+    TRY(dev_alloc(ctx, &p, n));
+    pl->d_ci = p;
+    pl->ci_bytes = n * 8;
   }
   FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0), ctx);
   FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
-  ctx->last_launches = precision == FCG_FP64 ? 1 : 2;
+  ctx->last_launches = 2;
   return FCG_OK;
 }
 

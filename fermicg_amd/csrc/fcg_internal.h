@@ -129,20 +129,21 @@ struct fcg_plan {
   int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
   int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
   int64_t *d_col_ptr = nullptr;   // [n_box+1] columns of each bra row
+  int32_t *d_order = nullptr;     // [n_box] bra boxes, largest panel first
   int32_t *d_colidx = nullptr;    // [n_cols] column -> variable index
   int64_t n_cols = 0;
   double *d_K64 = nullptr;
   float *d_K32 = nullptr;
   uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
-  float *d_c32 = nullptr;         // FP32 copy of c for reduced precision
-  int64_t c32_elems = 0;
+  void *d_ci = nullptr;           // c interleaved ([n_var][R], FP64 or FP32) for the apply
+  int64_t ci_bytes = 0;
 };
 
 // host plan construction (fcg_plan_host.cpp)
 struct HostPlan {
   std::vector<int32_t> pair_B, pair_K;
   std::vector<int64_t> pair_t0, pair_koff, row_ptr, diag_koff, row_koff, col_ptr;
-  std::vector<int32_t> pair_col, diag_col, colidx;
+  std::vector<int32_t> pair_col, diag_col, colidx, order;
   std::vector<TaskDev> tasks;
   int64_t n_kel = 0;
   double flops = 0.0;

@@ -245,11 +245,62 @@ __device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 
-constexpr int APPLY_UNROLL = 8;
+// Quad loads in flight per lane (32-bit and 64-bit blocks), and wavefronts
+// (bra boxes) per workgroup.
+#ifndef FCG_APPLY_UNROLL32
+#define FCG_APPLY_UNROLL32 16
+#endif
+#ifndef FCG_APPLY_UNROLL64
+#define FCG_APPLY_UNROLL64 16
+#endif
+#ifndef FCG_APPLY_WPB32
+#define FCG_APPLY_WPB32 4
+#endif
+#ifndef FCG_APPLY_WPB64
+#define FCG_APPLY_WPB64 1
+#endif
+#ifndef FCG_APPLY_CLAMP32
+#define FCG_APPLY_CLAMP32 1
+#endif
+#ifndef FCG_APPLY_CLAMP64
+#define FCG_APPLY_CLAMP64 0
+#endif
+template <typename TK>
+constexpr int apply_unroll() { return sizeof(TK) == 8 ? FCG_APPLY_UNROLL64 : FCG_APPLY_UNROLL32; }
+template <typename TK>
+constexpr int apply_wpb() { return sizeof(TK) == 8 ? FCG_APPLY_WPB64 : FCG_APPLY_WPB32; }
+// clamped unconditional quad loads (measured faster for 32-bit blocks,
+// slower for 64-bit ones) or predicated loads
+template <typename TK>
+constexpr bool apply_clamp() { return sizeof(TK) == 8 ? FCG_APPLY_CLAMP64 : FCG_APPLY_CLAMP32; }
 
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
+
+// R consecutive values from global memory (16- or 8-byte vector loads when
+// R allows; the interleaved vector keeps a variable's R roots contiguous)
+template <typename TC, int R>
+__device__ __forceinline__ void load_row(const TC *__restrict__ p, TC (&v)[R]) {
+  if constexpr (sizeof(TC) == 4 && R % 4 == 0) {
+#pragma unroll
+    for (int h = 0; h < R / 4; h++) {
+      const float4 t = reinterpret_cast<const float4 *>(p)[h];
+      v[4 * h] = t.x; v[4 * h + 1] = t.y; v[4 * h + 2] = t.z; v[4 * h + 3] = t.w;
+    }
+  } else if constexpr (R % 2 == 0) {
+    using V2 = typename Vec2<TC>::type;
+#pragma unroll
+    for (int h = 0; h < R / 2; h++) {
+      const V2 t = reinterpret_cast<const V2 *>(p)[h];
+      v[2 * h] = t.x;
+      v[2 * h + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = p[r];
+  }
+}
 
 // R consecutive staged values (paired vector LDS reads when R is even)
 template <typename TC, int R>
@@ -311,7 +362,9 @@ constexpr int stage_quads() {
 }
 
 // sigma[var[b]+i, r] = sum_j panel_b[i, j] c[colidx_b[j], r] for every bra
-// box b.  One wavefront per bra box; lane l = i + db*s (s < sl = 64/db) owns
+// box b.  One wavefront per bra box, boxes taken in `order` (largest panel
+// first, so the long rows start early and similar rows share a workgroup);
+// lane l = i + db*s (s < sl = 64/db) owns
 // panel row i and every sl-th column quad, loading each quad with one
 // 16-byte (FP32, BF16X2) or two 16-byte (FP64) loads; consecutive lanes read
 // consecutive quads, so a wave streams contiguous panel memory.  The column
@@ -321,20 +374,24 @@ constexpr int stage_quads() {
 // Slices are summed in a fixed order through LDS, so results are
 // reproducible bit for bit.
 template <typename TK, typename TA, typename TC, int R>
-__global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__restrict__ var,
+__global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
+                                                          const int32_t *__restrict__ order,
+                                                          const int64_t *__restrict__ var,
                                                const int64_t *__restrict__ row_koff,
                                                const int64_t *__restrict__ col_ptr,
                                                const int32_t *__restrict__ colidx,
                                                const TK *__restrict__ K, const TC *__restrict__ c,
-                                               int64_t nvar_c, int64_t nvar_s,
-                                               double *__restrict__ sigma) {
+                                               int64_t nvar_s, double *__restrict__ sigma) {
   static_assert(sizeof(TA) == sizeof(TC), "stage buffer doubles as reduction buffer");
+  constexpr int APPLY_UNROLL = apply_unroll<TK>();
+  constexpr int APPLY_WPB = apply_wpb<TK>();
   constexpr int CG = stage_quads<TC, R>();
   static_assert(CG * FCG_QUAD >= 64, "reduction needs 64*R staged entries");
-  __shared__ __align__(16) TC stage[4][CG * FCG_QUAD * R];
+  __shared__ __align__(16) TC stage[APPLY_WPB][CG * FCG_QUAD * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t b = (int64_t)blockIdx.x * 4 + wave;
-  const bool live = b < n_box;
+  const int64_t w = (int64_t)blockIdx.x * APPLY_WPB + wave;
+  const bool live = w < n_box;
+  const int64_t b = live ? order[w] : 0;
   const int db = live ? (int)(var[b + 1] - var[b]) : 1;
   const int sl = 64 / db;
   const int i = lane % db, s = lane / db;
@@ -356,9 +413,14 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
                           : make_int4(0, 0, 0, 0);
       const int col[FCG_QUAD] = {idx.x, idx.y, idx.z, idx.w};
 #pragma unroll
-      for (int q = 0; q < FCG_QUAD; q++)
+      for (int q = 0; q < FCG_QUAD; q++) {
+        if (ok) {
+          load_row<TC, R>(c + (int64_t)col[q] * R, nv[q]);
+        } else {
 #pragma unroll
-        for (int r = 0; r < R; r++) nv[q][r] = ok ? c[(int64_t)r * nvar_c + col[q]] : (TC)0;
+          for (int r = 0; r < R; r++) nv[q][r] = (TC)0;
+        }
+      }
     };
     gather(lane);
     for (int base = 0; base < ngrp; base += CG) {
@@ -375,11 +437,23 @@ __global__ __launch_bounds__(256) void k_apply(int64_t n_box, const int64_t *__r
       const TK *Kc = Kr + (int64_t)FCG_QUAD * db * base;
       for (int k0 = 0; k0 < nk; k0 += APPLY_UNROLL) {
         RawQuad<TK> kv[APPLY_UNROLL];
+        if constexpr (apply_clamp<TK>()) {
+          // unconditional loads from a clamped (valid) quad, then a select:
+          // no branch per load
+          const int jmax = max(lim - 1, 0);
 #pragma unroll
-        for (int u = 0; u < APPLY_UNROLL; u++) {
-          const int j = s + sl * (k0 + u);
-          if (j < lim) kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * j));
-          else kv[u].zero();
+          for (int u = 0; u < APPLY_UNROLL; u++)
+            kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * min(s + sl * (k0 + u), jmax)));
+#pragma unroll
+          for (int u = 0; u < APPLY_UNROLL; u++)
+            if (s + sl * (k0 + u) >= lim) kv[u].zero();
+        } else {
+#pragma unroll
+          for (int u = 0; u < APPLY_UNROLL; u++) {
+            const int j = s + sl * (k0 + u);
+            if (j < lim) kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * j));
+            else kv[u].zero();
+          }
         }
 #pragma unroll
         for (int u = 0; u < APPLY_UNROLL; u++) {
@@ -434,10 +508,13 @@ __global__ void k_convert(int64_t n, const double *__restrict__ K64, float *__re
   }
 }
 
-__global__ void k_to_f32(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+// y[i*R + r] = x[r*n + i]: the trial vectors in the apply's interleaved
+// layout (a variable's R roots contiguous), converted to TC
+template <typename TC>
+__global__ void k_interleave(int64_t n, int R, const double *__restrict__ x, TC *__restrict__ y) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * R;
        e += (int64_t)gridDim.x * blockDim.x)
-    y[e] = (float)x[e];
+    y[e] = (TC)x[(e % R) * n + e / R];
 }
 
 __global__ void k_diag(int64_t n_box, const int64_t *__restrict__ var,
@@ -502,12 +579,17 @@ hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
                             int precision, hipStream_t s, hipEvent_t before_apply) {
   const fcg_space *sp = pl->sp;
-  const dim3 grid((unsigned)((sp->n_box + 3) / 4)), block(256);
+  if (sp->n_box == 0) {
+    if (before_apply) (void)hipEventRecord(before_apply, s);
+    return hipSuccess;
+  }
 #define LAUNCH(TK, TA, TC, KP, CP, RR)                                                        \
   case RR:                                                                                    \
-    hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>), grid, block, 0, s, sp->n_box, sp->d_var,   \
-                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, pl->ksp->nvar,     \
-                       sp->nvar, sigma);                                                     \
+    hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>),                                             \
+                       dim3((unsigned)((sp->n_box + apply_wpb<TK>() - 1) / apply_wpb<TK>())), \
+                       dim3(64 * apply_wpb<TK>()), 0, s, sp->n_box, pl->d_order,             \
+                       sp->d_var,                                                             \
+                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma); \
     break;
 #define BY_R(TK, TA, TC, KP, CP)                                                              \
   switch (R) {                                                                                \
@@ -520,18 +602,21 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
     LAUNCH(TK, TA, TC, KP, CP, 16)                                                            \
     default: return hipErrorInvalidValue;                                                     \
   }
+  const int64_t nk = pl->ksp->nvar;
   if (precision == FCG_FP64) {
+    double *ci = static_cast<double *>(pl->d_ci);
+    hipLaunchKernelGGL(k_interleave<double>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
     if (before_apply) (void)hipEventRecord(before_apply, s);
-    BY_R(double, double, double, pl->d_K64, c);
+    BY_R(double, double, double, pl->d_K64, ci);
     return hipGetLastError();
   }
-  hipLaunchKernelGGL(k_to_f32, dim3(1024), dim3(256), 0, s, (int64_t)R * pl->ksp->nvar, c,
-                     pl->d_c32);
+  float *ci = static_cast<float *>(pl->d_ci);
+  hipLaunchKernelGGL(k_interleave<float>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
   if (before_apply) (void)hipEventRecord(before_apply, s);
   if (precision == FCG_FP32) {
-    BY_R(float, float, float, pl->d_K32, pl->d_c32);
+    BY_R(float, float, float, pl->d_K32, ci);
   } else {
-    BY_R(uint32_t, float, float, pl->d_Kbf, pl->d_c32);
+    BY_R(uint32_t, float, float, pl->d_Kbf, ci);
   }
 #undef BY_R
 #undef LAUNCH

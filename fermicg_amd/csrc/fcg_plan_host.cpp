@@ -474,6 +474,13 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       for (int64_t j = 0; j < dim_k(K); j++) out->colidx[at++] = (int32_t)(ket->var_off[K] + j);
     }
   }
+  // apply order: largest panels first (ties by box index)
+  out->order.resize(bra->n_box);
+  std::iota(out->order.begin(), out->order.end(), 0);
+  std::stable_sort(out->order.begin(), out->order.end(), [&](int32_t a, int32_t b) {
+    return dim_b(a) * (out->col_ptr[a + 1] - out->col_ptr[a]) >
+           dim_b(b) * (out->col_ptr[b + 1] - out->col_ptr[b]);
+  });
   out->n_kel = koff;
   out->flops = flops;
   return FCG_OK;
