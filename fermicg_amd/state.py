@@ -33,39 +33,51 @@ def fock_configs_within(bases, ref_fock, max_transfer):
     """Fock configurations with the reference's total (Nα, Nβ), every sector
     present in the cluster bases, and at most `max_transfer` electrons moved
     between clusters.  Order: Iterators.product over each cluster's sectors
-    (first cluster fastest), as expand_to_full_space! (tpsci_outer.jl:830-857)."""
+    (first cluster fastest), as expand_to_full_space! (tpsci_outer.jl:830-857).
+
+    The transferred count splits into an alpha and a beta part, so the alpha
+    and beta occupation tuples are enumerated separately (pruned by their
+    totals and their own transfer) and then paired."""
     na = sum(f[0] for f in ref_fock)
     nb = sum(f[1] for f in ref_fock)
-    secs = [sorted(cb.sectors()) for cb in bases]
     N = len(bases)
-    # bounds on what clusters 0..c-1 can still contribute, for pruning
-    amin = [0] * (N + 1); amax = [0] * (N + 1); bmin = [0] * (N + 1); bmax = [0] * (N + 1)
-    for c in range(N):
-        amin[c + 1] = amin[c] + min(s[0] for s in secs[c])
-        amax[c + 1] = amax[c] + max(s[0] for s in secs[c])
-        bmin[c + 1] = bmin[c] + min(s[1] for s in secs[c])
-        bmax[c + 1] = bmax[c] + max(s[1] for s in secs[c])
     order = [cb.sectors() for cb in bases]
+    secset = [set(o) for o in order]
+
+    def spin_tuples(spin, total):
+        vals = [sorted({s[spin] for s in order[c]}) for c in range(N)]
+        lo = [0] * (N + 1); hi = [0] * (N + 1)
+        for c in range(N):
+            lo[c + 1] = lo[c] + vals[c][0]
+            hi[c + 1] = hi[c] + vals[c][-1]
+        out, cur = [], [0] * N
+
+        def rec(c, rem, tr):
+            if c < 0:
+                if rem == 0:
+                    out.append((tuple(cur), tr))
+                return
+            for v in vals[c]:
+                t = tr + max(0, v - ref_fock[c][spin])
+                r2 = rem - v
+                if t > max_transfer or r2 < lo[c] or r2 > hi[c]:
+                    continue
+                cur[c] = v
+                rec(c - 1, r2, t)
+
+        rec(N - 1, total, 0)
+        return out
+
+    A = spin_tuples(0, na)
+    B = spin_tuples(1, nb)
     out = []
-    chosen = [None] * N
-
-    def rec(c, ra, rb, tr):
-        # choose cluster c (slowest first); clusters < c remain
-        if c < 0:
-            if ra == 0 and rb == 0:
-                out.append(tuple((int(a), int(b)) for a, b in chosen))
-            return
-        for s in order[c]:
-            t = tr + max(0, s[0] - ref_fock[c][0]) + max(0, s[1] - ref_fock[c][1])
-            a2, b2 = ra - s[0], rb - s[1]
-            if t > max_transfer or a2 < amin[c] or a2 > amax[c] or b2 < bmin[c] or b2 > bmax[c]:
+    for ta, tra in A:
+        for tb, trb in B:
+            if tra + trb > max_transfer:
                 continue
-            chosen[c] = s
-            rec(c - 1, a2, b2, t)
-
-    rec(N - 1, na, nb, 0)
-    # product order with the first cluster fastest
-    key = {s: i for i, s in enumerate(order[0])}
+            f = tuple((ta[c], tb[c]) for c in range(N))
+            if all(f[c] in secset[c] for c in range(N)):
+                out.append(f)
     idx = [{s: i for i, s in enumerate(order[c])} for c in range(N)]
     out.sort(key=lambda f: tuple(idx[c][f[c]] for c in reversed(range(N))))
     return out

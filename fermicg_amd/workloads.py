@@ -71,3 +71,69 @@ def bst_medium(seed=1, R=2, n_clusters=6, max_roots=16, max_rank=8):
     p.meta = {"workload": "bst_%do_%dc_M%d_r%d" % (4 * n_clusters, n_clusters, max_roots, max_rank),
               "n_fock": len(p.focks), "n_blocks": len(st.blocks), "n_var": len(st)}
     return p, st
+
+
+def splitmix64(x):
+    """splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def rank1_space(p, R=4, frac=0.159, seed=2):
+    """Rank <= 1 ClusterConfigs over the problem's Fock configurations: in
+    every Fock block the all-ground configuration (1,...,1) and each
+    configuration with exactly one cluster c in state s >= 2, the latter kept
+    when splitmix64(seed ^ key) < frac * 2^64 with key = (Fock index << 16) |
+    (c << 8) | s.  Fock blocks in the problem's order; inside a block the
+    ground configuration first, then by cluster, then by state."""
+    N = len(p.bases)
+    focks = p.focks
+    M = np.array([[p.bases[c].n_states(f[c]) for c in range(N)] for f in focks], np.int64)
+    nf = len(focks)
+    # candidate (fock, cluster, state) triples, s = 2..M
+    fi, ci, si = [], [], []
+    for c in range(N):
+        for s in range(2, int(M[:, c].max()) + 1):
+            f = np.nonzero(M[:, c] >= s)[0]
+            fi.append(f); ci.append(np.full(len(f), c)); si.append(np.full(len(f), s))
+    fi = np.concatenate(fi); ci = np.concatenate(ci); si = np.concatenate(si)
+    key = (fi.astype(np.uint64) << np.uint64(16)) | (ci.astype(np.uint64) << np.uint64(8)) | si.astype(np.uint64)
+    h = splitmix64(key ^ np.uint64(seed))
+    keep = h < np.uint64(int(frac * 2.0 ** 64))
+    fi, ci, si = fi[keep], ci[keep], si[keep]
+    order = np.lexsort((si, ci, fi))
+    fi, ci, si = fi[order], ci[order], si[order]
+    counts = np.bincount(fi, minlength=nf) + 1          # + the ground configuration
+    start = np.zeros(nf + 1, np.int64)
+    start[1:] = np.cumsum(counts)
+    cfg = np.ones((start[-1], N), np.int16)
+    # row of each excited configuration: after its block's ground row
+    rows = start[fi] + 1 + (np.arange(len(fi)) - np.searchsorted(fi, fi))
+    cfg[rows, ci] = si
+    from .state import TPSCIstate
+    st = TPSCIstate(N, R)
+    st.fock = np.array(focks, np.int16).reshape(nf, N, 2)
+    st.fock_start = start
+    st.cfg = cfg
+    st.coeffs = np.zeros((len(cfg), R))
+    return st
+
+
+def tpsci_sparse(seed=2, R=4, n_clusters=8, max_roots=16, max_transfer=3, frac=0.159):
+    """32 orbitals / 8 clusters / M=16 / Fock configurations within 3
+    transferred electrons, rank <= 1 configurations selected by hash (about
+    2e6); R random normalised start vectors."""
+    p = build_problem(4 * n_clusters, n_clusters, max_roots, 8 * n_clusters, seed,
+                      max_transfer=max_transfer)
+    st = rank1_space(p, R=R, frac=frac, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 3000))
+    v = rng.normal(size=(len(st), R))
+    v /= np.linalg.norm(v, axis=0)
+    st.set_vector(v)
+    p.meta = {"workload": "tpsci_%do_%dc_M%d_rank1" % (4 * n_clusters, n_clusters, max_roots),
+              "n_fock": len(p.focks), "n_cfg": len(st)}
+    return p, st
