@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci"])
+    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci", "sparse"])
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
@@ -62,6 +62,9 @@ def main():
         from fermicg_amd.bst import cache_hamiltonian, ci_solve
         from fermicg_amd.workloads import bst_medium
         p, st = bst_medium(n_clusters=args.clusters)
+    elif args.workload == "sparse":
+        from fermicg_amd.workloads import tpsci_sparse
+        p, st = tpsci_sparse()
     else:
         from fermicg_amd.workloads import tpsci_small
         p, st = tpsci_small(R=1)
@@ -151,6 +154,17 @@ def main():
                              "uses the contraction share (cached ops, as in ci_solve iterations); "
                              "with per-term op builds: %.4f sigma/s"
                              % (len(rows), nb, threads, wall, frac / wall)}
+        elif args.workload == "sparse":
+            orc = Oracle(fo, fh, st)
+            idx = np.sort(rng.choice(n, min(args.cpu_rows, n), replace=False))
+            ta = time.perf_counter()
+            ref = orc.sigma_rows(st.coeffs, idx, nthreads=threads)
+            wall = time.perf_counter() - ta
+            cpu = {"value": len(idx) / n / wall, "unit": "sigma builds/s", "cores": threads,
+                   "kind": "port",
+                   "sample": "C restatement of tps_ci_matvec (src/tpsci_outer.jl:462) on %d of %d "
+                             "bra configurations (seeded), %d OpenMP threads, %.2f s wall"
+                             % (len(idx), n, threads, wall)}
         else:
             orc = Oracle(fo, fh, st)
             idx = np.arange(n)
@@ -208,7 +222,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == "fp32" else ("f64" if prec == "fp64" else "bf16x2/f32"),
-        "data": "synthetic (seeded integrals, random Tucker cores and factors)",
+        "data": {"bst": "synthetic (seeded integrals, random Tucker cores and factors)",
+                 "sparse": "synthetic (seeded integrals, hash-selected rank-1 configurations, "
+                           "random start vectors)",
+                 "tpsci": "synthetic (seeded integrals, random start vector)"}[args.workload],
         "config": {"workload": meta["workload"], "n_var": int(n), "roots": int(R),
                    "n_fock": meta.get("n_fock"), "n_blocks": meta.get("n_blocks"),
                    "block_pairs": ps["pairs"], "terms_x_pairs": ps["tasks"],

@@ -345,6 +345,10 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   auto t0 = std::chrono::steady_clock::now();
   TRY(fcg_form_boxes(ctx, o, sp));
   if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
+  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box) {
+    // every box one configuration: join and build on the device
+    return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out);
+  }
   HostPlan hp;
   int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, &hp, &ctx->err);
   if (rc) return rc;
@@ -377,7 +381,9 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
-  PTRY(dev_upload(ctx, &pl->d_order, hp.order.data(), sp->n_box));
+  PTRY(dev_upload(ctx, &pl->d_order, hp.order.data(), (int64_t)hp.order.size()));
+  pl->n_multi = hp.n_multi;
+  pl->n_single_slots = hp.n_single_slots;
   PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
   PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
   if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));

@@ -19,6 +19,8 @@ constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
+constexpr int FCG_CSR_WPB = 4;        // one-row boxes: wavefronts (rows) per workgroup
+constexpr int FCG_XCDS = 8;           // MI355X accelerator complex dies (workgroups round-robin)
 
 // Row panel storage.  The Hamiltonian blocks of one bra box (db rows) sit
 // side by side as a db x ncol panel, ncol a multiple of FCG_QUAD; the panel
@@ -129,7 +131,11 @@ struct fcg_plan {
   int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
   int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
   int64_t *d_col_ptr = nullptr;   // [n_box+1] columns of each bra row
-  int32_t *d_order = nullptr;     // [n_box] bra boxes, largest panel first
+  // apply schedule: [0, n_multi) boxes of several rows, largest panel first;
+  // then n_single_slots slots of one-row boxes (-1 = empty slot) laid out so
+  // that each XCD's workgroups take a contiguous run of rows
+  int32_t *d_order = nullptr;
+  int64_t n_multi = 0, n_single_slots = 0;
   int32_t *d_colidx = nullptr;    // [n_cols] column -> variable index
   int64_t n_cols = 0;
   double *d_K64 = nullptr;
@@ -144,10 +150,14 @@ struct HostPlan {
   std::vector<int32_t> pair_B, pair_K;
   std::vector<int64_t> pair_t0, pair_koff, row_ptr, diag_koff, row_koff, col_ptr;
   std::vector<int32_t> pair_col, diag_col, colidx, order;
+  int64_t n_multi = 0, n_single_slots = 0;
   std::vector<TaskDev> tasks;
   int64_t n_kel = 0;
   double flops = 0.0;
 };
+// apply schedule of a plan (see fcg_plan::d_order)
+void fcg_apply_order(const std::vector<int64_t> &var_off, const std::vector<int64_t> &col_ptr,
+                     std::vector<int32_t> *order, int64_t *n_multi, int64_t *n_single_slots);
 int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
                         const fcg_space *ket, int nbody, HostPlan *out, std::string *err);
 int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
@@ -169,6 +179,16 @@ struct Pt2Target {
 };
 int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int h0_op,
                    fcg_space *tsp, Pt2Target *out, std::string *err);
+
+// Fock pairs of single-configuration spaces (CSR over bra Fock blocks)
+int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket, int nbody,
+                   int S, std::vector<int64_t> *ptr, std::vector<int32_t> *fk,
+                   std::vector<int32_t> *g);
+// plan of a space whose boxes are all single configurations, joined and
+// built on the device (fcg_sparse.hip)
+int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
+                           int precisions, fcg_plan **out);
+constexpr int FCG_SPARSE_MAX_BLOCK = 256;  // configurations per Fock block (device join)
 
 // shared helpers (fcg_api.hip)
 int fcg_space_to_device(fcg_space *sp);

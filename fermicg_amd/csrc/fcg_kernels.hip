@@ -390,7 +390,7 @@ __global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
   __shared__ __align__(16) TC stage[APPLY_WPB][CG * FCG_QUAD * R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * APPLY_WPB + wave;
-  const bool live = w < n_box;
+  const bool live = w < n_box && order[w] >= 0;
   const int64_t b = live ? order[w] : 0;
   const int db = live ? (int)(var[b + 1] - var[b]) : 1;
   const int sl = 64 / db;
@@ -484,6 +484,71 @@ __global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
       for (int s2 = 0; s2 < sl; s2++) v += red[(lane + db * s2) * R + r];
       sigma[(int64_t)r * nvar_s + var[b] + lane] = (double)v;
     }
+  }
+}
+
+// One-row boxes (db = 1, the rows of sparse TPSCI spaces): no row shares a
+// column with another, so there is nothing to stage; the wave streams the
+// row's column quads (lane l takes quads l, l+64, ...), each lane loading its
+// quads' blocks and column indices and gathering the vector entries itself,
+// APPLY_UNROLL quads in flight.  Rows come from `order` (XCD-grouped slots,
+// -1 = empty); the 64 lane sums are combined by a fixed butterfly.
+template <typename TK, typename TA, typename TC, int R>
+__global__ __launch_bounds__(64 * FCG_CSR_WPB) void k_apply_rows(
+    int64_t n_slots, const int32_t *__restrict__ order, const int64_t *__restrict__ var,
+    const int64_t *__restrict__ row_koff, const int64_t *__restrict__ col_ptr,
+    const int32_t *__restrict__ colidx, const TK *__restrict__ K, const TC *__restrict__ c,
+    int64_t nvar_s, double *__restrict__ sigma) {
+  constexpr int U = apply_unroll<TK>() / 2;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t w = (int64_t)blockIdx.x * FCG_CSR_WPB + wave;
+  const int32_t b = w < n_slots ? order[w] : -1;
+  if (b < 0) return;  // uniform per wave
+  const int64_t c0 = col_ptr[b];
+  const int ngrp = (int)((col_ptr[b + 1] - c0) / FCG_QUAD);
+  const TK *Kr = K + row_koff[b];
+  const int32_t *ci = colidx + c0;
+  TA acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = (TA)0;
+  for (int base = lane; base < ngrp; base += 64 * U) {
+    RawQuad<TK> kv[U];
+    int4 idx[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = base + 64 * u;
+      if (q < ngrp) {
+        kv[u].load(Kr + (int64_t)FCG_QUAD * q);
+        idx[u] = *reinterpret_cast<const int4 *>(ci + (int64_t)FCG_QUAD * q);
+      } else {
+        kv[u].zero();
+        idx[u] = make_int4(0, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      TA kq[FCG_QUAD];
+      kv[u].get(kq);
+      const int col[FCG_QUAD] = {idx[u].x, idx[u].y, idx[u].z, idx[u].w};
+#pragma unroll
+      for (int q = 0; q < FCG_QUAD; q++) {
+        TC cv[R];
+        load_row<TC, R>(c + (int64_t)col[q] * R, cv);
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] += kq[q] * (TA)cv[r];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    TA v = acc[r];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    acc[r] = v;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) sigma[(int64_t)r * nvar_s + var[b]] = (double)acc[r];
   }
 }
 
@@ -585,11 +650,19 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
   }
 #define LAUNCH(TK, TA, TC, KP, CP, RR)                                                        \
   case RR:                                                                                    \
-    hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>),                                             \
-                       dim3((unsigned)((sp->n_box + apply_wpb<TK>() - 1) / apply_wpb<TK>())), \
-                       dim3(64 * apply_wpb<TK>()), 0, s, sp->n_box, pl->d_order,             \
-                       sp->d_var,                                                             \
-                       pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma); \
+    if (pl->n_multi > 0)                                                                      \
+      hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>),                                           \
+                         dim3((unsigned)((pl->n_multi + apply_wpb<TK>() - 1) /                \
+                                         apply_wpb<TK>())),                                   \
+                         dim3(64 * apply_wpb<TK>()), 0, s, pl->n_multi, pl->d_order,          \
+                         sp->d_var, pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP,      \
+                         sp->nvar, sigma);                                                    \
+    if (pl->n_single_slots > 0)                                                               \
+      hipLaunchKernelGGL((k_apply_rows<TK, TA, TC, RR>),                                      \
+                         dim3((unsigned)(pl->n_single_slots / FCG_CSR_WPB)),                  \
+                         dim3(64 * FCG_CSR_WPB), 0, s, pl->n_single_slots,                    \
+                         pl->d_order + pl->n_multi, sp->d_var, pl->d_row_koff,                \
+                         pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma);               \
     break;
 #define BY_R(TK, TA, TC, KP, CP)                                                              \
   switch (R) {                                                                                \

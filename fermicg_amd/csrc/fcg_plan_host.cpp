@@ -474,13 +474,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       for (int64_t j = 0; j < dim_k(K); j++) out->colidx[at++] = (int32_t)(ket->var_off[K] + j);
     }
   }
-  // apply order: largest panels first (ties by box index)
-  out->order.resize(bra->n_box);
-  std::iota(out->order.begin(), out->order.end(), 0);
-  std::stable_sort(out->order.begin(), out->order.end(), [&](int32_t a, int32_t b) {
-    return dim_b(a) * (out->col_ptr[a + 1] - out->col_ptr[a]) >
-           dim_b(b) * (out->col_ptr[b + 1] - out->col_ptr[b]);
-  });
+  fcg_apply_order(bra->var_off, out->col_ptr, &out->order, &out->n_multi, &out->n_single_slots);
   out->n_kel = koff;
   out->flops = flops;
   return FCG_OK;
@@ -570,4 +564,86 @@ int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int
     }
   }
   return FCG_OK;
+}
+
+// Connected Fock pairs for a space of single-configuration rows: for each bra
+// Fock block, the ket Fock blocks F_bra - TransferConfig present in `ket`,
+// with the TransferConfig group, in group order (the lookup of
+// tps_ci_matvec, src/tpsci_outer.jl:479-481).  Groups without a term of at
+// most nbody clusters are skipped.
+int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket, int nbody,
+                   int S, std::vector<int64_t> *ptr, std::vector<int32_t> *fk_out,
+                   std::vector<int32_t> *g_out) {
+  const int N = bra->N;
+  std::unordered_map<std::string, int32_t> fmap;
+  fmap.reserve(ket->n_fock * 2);
+  for (int64_t f = 0; f < ket->n_fock; f++) fmap.emplace(fock_key(&ket->fock[f * 2 * N], N), (int32_t)f);
+  std::vector<char> live(h->n_tc, 0);
+  for (int g = 0; g < h->n_tc; g++)
+    for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++)
+      if (h->nbody[t] <= nbody) live[g] = 1;
+  std::vector<std::vector<int32_t>> per(bra->n_fock);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t fb = 0; fb < bra->n_fock; fb++) {
+    const int16_t *F = &bra->fock[fb * 2 * N];
+    int16_t fk[2 * FCG_MAXN];
+    std::vector<int32_t> &v = per[fb];
+    for (int g = 0; g < h->n_tc; g++) {
+      if (!live[g]) continue;
+      const int16_t *tc = &h->tc[(int64_t)g * 2 * N];
+      bool ok = true;
+      for (int i = 0; i < 2 * N && ok; i++) {
+        fk[i] = F[i] - tc[i];
+        ok = fk[i] >= 0 && fk[i] < S;
+      }
+      if (!ok) continue;
+      auto it = fmap.find(fock_key(fk, N));
+      if (it == fmap.end()) continue;
+      v.push_back(it->second);
+      v.push_back(g);
+    }
+  }
+  ptr->assign(bra->n_fock + 1, 0);
+  for (int64_t fb = 0; fb < bra->n_fock; fb++) (*ptr)[fb + 1] = (*ptr)[fb] + (int64_t)per[fb].size() / 2;
+  fk_out->resize((*ptr)[bra->n_fock]);
+  g_out->resize((*ptr)[bra->n_fock]);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t fb = 0; fb < bra->n_fock; fb++) {
+    int64_t at = (*ptr)[fb];
+    for (size_t i = 0; i < per[fb].size(); i += 2, at++) {
+      (*fk_out)[at] = per[fb][i];
+      (*g_out)[at] = per[fb][i + 1];
+    }
+  }
+  return FCG_OK;
+}
+
+// Boxes of several rows: largest panel first (ties by box index), so the long
+// rows start early.  One-row boxes: in space order, which keeps rows of one
+// Fock block (sharing their ket Fock blocks) together; workgroup g of
+// FCG_CSR_WPB rows runs on XCD g mod 8, so slot g is given the row group
+// (g mod 8) * G + g / 8 and each XCD's L2 sees one contiguous run of rows.
+void fcg_apply_order(const std::vector<int64_t> &var_off, const std::vector<int64_t> &col_ptr,
+                     std::vector<int32_t> *order, int64_t *n_multi, int64_t *n_single_slots) {
+  const int64_t n_box = (int64_t)var_off.size() - 1;
+  std::vector<int32_t> multi, single;
+  for (int64_t b = 0; b < n_box; b++)
+    (var_off[b + 1] - var_off[b] > 1 ? multi : single).push_back((int32_t)b);
+  std::stable_sort(multi.begin(), multi.end(), [&](int32_t a, int32_t b) {
+    return (var_off[a + 1] - var_off[a]) * (col_ptr[a + 1] - col_ptr[a]) >
+           (var_off[b + 1] - var_off[b]) * (col_ptr[b + 1] - col_ptr[b]);
+  });
+  const int64_t ns = (int64_t)single.size();
+  const int64_t nw = (ns + FCG_CSR_WPB - 1) / FCG_CSR_WPB;
+  const int64_t g8 = (nw + FCG_XCDS - 1) / FCG_XCDS;
+  const int64_t slots = ns ? g8 * FCG_XCDS * FCG_CSR_WPB : 0;
+  *order = multi;
+  order->resize(multi.size() + slots, -1);
+  for (int64_t w = 0; w < slots; w++) {
+    const int64_t g = w / FCG_CSR_WPB, k = w % FCG_CSR_WPB;
+    const int64_t idx = ((g % FCG_XCDS) * g8 + g / FCG_XCDS) * FCG_CSR_WPB + k;
+    (*order)[multi.size() + w] = idx < ns ? single[idx] : -1;
+  }
+  *n_multi = (int64_t)multi.size();
+  *n_single_slots = slots;
 }

@@ -1,0 +1,455 @@
+// Device-side join and Hamiltonian build for spaces whose boxes are all single
+// configurations (sparse TPSCI spaces: the rank-1 configuration sets of large
+// cluster counts).  The host join of fcg_plan_host.cpp enumerates (box, box,
+// term) tasks explicitly, which at ~1e10 connected triples does not fit in
+// host memory; here the host only lists connected Fock pairs and the GPU does
+// the rest.
+//
+// Per bra Fock block (one workgroup), per connected ket Fock block
+// F_ket = F_bra - TransferConfig (src/tpsci_outer.jl:479-481), every
+// (bra configuration I, ket configuration J) pair is tested against every
+// term of the TransferConfig group with the reference's check_term rule
+// (src/type_ClusteredTerm.jl:92-176: states equal on the clusters the term
+// does not touch).  Two passes:
+//   count  the structurally nonzero (I, J) per row I;
+//   fill   the element H[I, J] = sum over passing terms of the reference's
+//          contract_matrix_element (src/tpsci_inner.jl:10-216, _contract
+//          :270-328) with the sign of compute_terms_state_sign
+//          (src/type_ClusteredTerm.jl:181-197), written at its row position.
+// Rows are the panels of the sigma apply with db = 1 (columns in ket Fock
+// pair order, then ket configuration order), so k_apply, the diagonal and
+// Davidson run unchanged.  Positions come from per-row bitmaps and popcounts,
+// so the layout and every value are deterministic.
+#include <algorithm>
+#include <chrono>
+#include <numeric>
+
+#include "fcg_internal.h"
+
+namespace {
+
+struct SparseTerm {
+  int32_t nbody;
+  int32_t cl[4];
+  int32_t op[4];
+  int32_t dims[4];
+  int32_t odd;   // bit a: operator string a has odd fermion parity
+  int32_t pad;
+  int64_t ioff;
+  uint64_t inact[2];  // 0xFF per cluster byte the term does not touch
+};
+
+struct SparseArgs {
+  int N, S, n_ops, nbody;
+  const int64_t *fstart;  // [n_fock+1] configurations (rows) of each Fock block
+  const int16_t *fock;    // [n_fock][N][2]
+  const int16_t *cfg;     // [n_cfg][N] 1-based states
+  const int64_t *fp_ptr;  // [n_fock+1] connected ket Fock blocks of each bra block
+  const int32_t *fp_fk, *fp_g;
+  const int32_t *tc_start;
+  const SparseTerm *terms;
+  const double *ints;
+  const int32_t *gblock;  // [N][n_ops][S*S][S*S]
+  const int64_t *gb_off;
+  const int32_t *gb_dims;
+  const double *gdata;
+  int64_t fb_base;
+  int32_t *row_nnz;                  // count pass
+  unsigned long long *stats;         // [0] triples, [1] flops, [2] missing blocks
+  const int64_t *row_start;          // fill pass
+  double *K;
+  int32_t *colidx;
+  int64_t *diag_koff;
+  int32_t *diag_col;
+};
+
+constexpr int ST = 256;  // threads per workgroup
+constexpr int MB = FCG_SPARSE_MAX_BLOCK;
+
+__device__ __forceinline__ void pack_states(const int16_t *c, int N, uint64_t (&p)[2]) {
+  uint64_t w0 = 0, w1 = 0;
+  for (int k = 0; k < N; k++) {
+    const uint64_t v = (uint64_t)(uint16_t)c[k] & 0xFFu;
+    if (k < 8) w0 |= v << (8 * k);
+    else w1 |= v << (8 * (k - 8));
+  }
+  p[0] = w0;
+  p[1] = w1;
+}
+
+__device__ __forceinline__ int state_of(const uint64_t *p, int c) {
+  return (int)((p[c >> 3] >> (8 * (c & 7))) & 0xFFu);
+}
+
+// contract_matrix_element for one term and one (I, J); *miss set when a
+// needed Γ block is absent
+__device__ double contract(const SparseArgs &a, const SparseTerm &T, const int16_t *fb,
+                           const int16_t *fk, const uint64_t *pI, const uint64_t *pJ,
+                           bool *miss) {
+  const int SS = a.S * a.S;
+  const double *g[4];
+  for (int q = 0; q < T.nbody; q++) {
+    const int c = T.cl[q];
+    const int cb = fb[2 * c] * a.S + fb[2 * c + 1], ck = fk[2 * c] * a.S + fk[2 * c + 1];
+    const int blk = a.gblock[(((int64_t)c * a.n_ops + T.op[q]) * SS + cb) * SS + ck];
+    if (blk < 0) {
+      *miss = true;
+      return 0.0;
+    }
+    const int np = a.gb_dims[3 * blk], mb = a.gb_dims[3 * blk + 1];
+    g[q] = a.gdata + a.gb_off[blk] +
+           (int64_t)np * ((state_of(pI, c) - 1) + (int64_t)mb * (state_of(pJ, c) - 1));
+  }
+  // sign from the ket Fock configuration
+  int par = 0;
+  for (int q = 0; q < T.nbody; q++) {
+    if (!((T.odd >> q) & 1)) continue;
+    int ne = 0;
+    for (int c = 0; c < T.cl[q]; c++) ne += fk[2 * c] + fk[2 * c + 1];
+    par ^= ne & 1;
+  }
+  const double sgn = par ? -1.0 : 1.0;
+  if (T.nbody == 1) return sgn * g[0][0];
+  const double *V = a.ints + T.ioff;
+  const int d0 = T.dims[0], d1 = T.dims[1];
+  double acc = 0.0;
+  if (T.nbody == 2) {
+    for (int p1 = 0; p1 < d1; p1++) {
+      double s = 0.0;
+      for (int p0 = 0; p0 < d0; p0++) s += V[p0 + d0 * p1] * g[0][p0];
+      acc += s * g[1][p1];
+    }
+  } else if (T.nbody == 3) {
+    const int d2 = T.dims[2];
+    for (int p2 = 0; p2 < d2; p2++)
+      for (int p1 = 0; p1 < d1; p1++) {
+        const double *Vp = V + d0 * (p1 + d1 * p2);
+        double s = 0.0;
+        for (int p0 = 0; p0 < d0; p0++) s += Vp[p0] * g[0][p0];
+        acc += s * g[1][p1] * g[2][p2];
+      }
+  } else {
+    const int d2 = T.dims[2], d3 = T.dims[3];
+    for (int p3 = 0; p3 < d3; p3++)
+      for (int p2 = 0; p2 < d2; p2++) {
+        const double t23 = g[2][p2] * g[3][p3];
+        for (int p1 = 0; p1 < d1; p1++) {
+          const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
+          double s = 0.0;
+          for (int p0 = 0; p0 < d0; p0++) s += Vp[p0] * g[0][p0];
+          acc += s * g[1][p1] * t23;
+        }
+      }
+  }
+  return sgn * acc;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
+  __shared__ uint64_t sPI[MB][2], sPJ[MB][2];
+  __shared__ uint64_t sBits[MB][MB / 64];
+  __shared__ int32_t sCur[MB];
+  __shared__ int16_t sFb[2 * FCG_MAXN], sFk[2 * FCG_MAXN];
+  __shared__ unsigned long long sTrip, sFlop, sMiss;
+  const int tid = threadIdx.x;
+  const int N = a.N;
+  const int64_t fb = a.fb_base + blockIdx.x;
+  const int64_t b0 = a.fstart[fb];
+  const int nb = (int)(a.fstart[fb + 1] - b0);
+  for (int I = tid; I < nb; I += ST) {
+    pack_states(a.cfg + (b0 + I) * N, N, sPI[I]);
+    sCur[I] = 0;
+  }
+  if (tid < 2 * N) sFb[tid] = a.fock[fb * 2 * N + tid];
+  if (tid == 0) sTrip = sFlop = sMiss = 0;
+  unsigned long long trip = 0, flop = 0, miss = 0;
+  __syncthreads();
+  for (int64_t q = a.fp_ptr[fb]; q < a.fp_ptr[fb + 1]; q++) {
+    const int64_t fk = a.fp_fk[q];
+    const int g = a.fp_g[q];
+    const int64_t k0 = a.fstart[fk];
+    const int nk = (int)(a.fstart[fk + 1] - k0);
+    for (int J = tid; J < nk; J += ST) pack_states(a.cfg + (k0 + J) * N, N, sPJ[J]);
+    if (tid < 2 * N) sFk[tid] = a.fock[fk * 2 * N + tid];
+    for (int e = tid; e < nb * (MB / 64); e += ST) sBits[e / (MB / 64)][e % (MB / 64)] = 0;
+    __syncthreads();
+    const int t0 = a.tc_start[g], t1 = a.tc_start[g + 1];
+    // structural nonzeros: some term passes check_term
+    for (int e = tid; e < nb * nk; e += ST) {
+      const int I = e / nk, J = e % nk;
+      bool nz = false;
+      for (int t = t0; t < t1; t++) {
+        const SparseTerm &T = a.terms[t];
+        if (T.nbody > a.nbody) continue;
+        if (((sPI[I][0] ^ sPJ[J][0]) & T.inact[0]) | ((sPI[I][1] ^ sPJ[J][1]) & T.inact[1]))
+          continue;
+        nz = true;
+        if (!FILL) {
+          trip++;
+          uint64_t v = 1;
+          if (T.nbody > 1)
+            for (int k = 0; k < T.nbody; k++) v *= (uint64_t)T.dims[k];
+          flop += 2 * v;
+        }
+      }
+      if (nz) atomicOr(reinterpret_cast<unsigned long long *>(&sBits[I][J >> 6]),
+                       1ull << (J & 63));
+    }
+    __syncthreads();
+    if (FILL) {
+      for (int e = tid; e < nb * nk; e += ST) {
+        const int I = e / nk, J = e % nk;
+        const uint64_t w = sBits[I][J >> 6];
+        if (!((w >> (J & 63)) & 1ull)) continue;
+        int below = __popcll(w & ((1ull << (J & 63)) - 1ull));
+        for (int k = 0; k < (J >> 6); k++) below += __popcll(sBits[I][k]);
+        const int64_t pos = a.row_start[b0 + I] + sCur[I] + below;
+        double v = 0.0;
+        bool ms = false;
+        for (int t = t0; t < t1; t++) {
+          const SparseTerm &T = a.terms[t];
+          if (T.nbody > a.nbody) continue;
+          if (((sPI[I][0] ^ sPJ[J][0]) & T.inact[0]) | ((sPI[I][1] ^ sPJ[J][1]) & T.inact[1]))
+            continue;
+          v += contract(a, T, sFb, sFk, sPI[I], sPJ[J], &ms);
+        }
+        if (ms) miss++;
+        a.K[pos] = v;
+        a.colidx[pos] = (int32_t)(k0 + J);
+        if (fk == fb && J == I) {
+          a.diag_koff[b0 + I] = a.row_start[b0 + I];
+          a.diag_col[b0 + I] = sCur[I] + below;
+        }
+      }
+      __syncthreads();
+    }
+    for (int I = tid; I < nb; I += ST) {
+      int n = 0;
+      for (int k = 0; k < MB / 64; k++) n += __popcll(sBits[I][k]);
+      sCur[I] += n;
+    }
+    __syncthreads();
+  }
+  if (!FILL)
+    for (int I = tid; I < nb; I += ST) a.row_nnz[b0 + I] = sCur[I];
+  if (trip) atomicAdd(&sTrip, trip);
+  if (flop) atomicAdd(&sFlop, flop);
+  if (miss) atomicAdd(&sMiss, miss);
+  __syncthreads();
+  if (tid == 0) {
+    if (sTrip) atomicAdd(&a.stats[0], sTrip);
+    if (sFlop) atomicAdd(&a.stats[1], sFlop);
+    if (sMiss) atomicAdd(&a.stats[2], sMiss);
+  }
+}
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <typename T>
+int upload(fcg_ctx *ctx, DBuf<T> &b, const T *src, int64_t n) {
+  if (hipMalloc((void **)&b.p, sizeof(T) * std::max<int64_t>(n, 1)) != hipSuccess) {
+    ctx->err = "hipMalloc failed in the sparse plan";
+    return FCG_ENOMEM;
+  }
+  if (n > 0)
+    FCG_HIP(hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream), ctx);
+  return FCG_OK;
+}
+
+constexpr int64_t FB_CHUNK = 65536;  // bra Fock blocks per launch
+
+}  // namespace
+
+int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
+                           int precisions, fcg_plan **out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int N = sp->N, S = o->S;
+  const int64_t n_row = sp->n_box;
+  // configurations of each Fock block are consecutive rows
+  std::vector<int64_t> fstart(sp->n_fock + 1, 0);
+  for (int64_t b = 0; b < n_row; b++) {
+    const int32_t f = sp->box_fock[b];
+    if (b > 0 && f < sp->box_fock[b - 1]) {
+      ctx->err = "sparse plan: Fock blocks must be contiguous";
+      return FCG_EINVAL;
+    }
+    fstart[f + 1]++;
+    for (int c = 0; c < N; c++)
+      if (sp->lo[b * N + c] > 255) {
+        ctx->err = "sparse plan: state index above 255";
+        return FCG_EOVERFLOW;
+      }
+  }
+  for (int64_t f = 0; f < sp->n_fock; f++) {
+    if (fstart[f + 1] > FCG_SPARSE_MAX_BLOCK) {
+      ctx->err = "sparse plan: more than 256 configurations in a Fock block";
+      return FCG_EOVERFLOW;
+    }
+    fstart[f + 1] += fstart[f];
+  }
+  std::vector<int64_t> fp_ptr;
+  std::vector<int32_t> fp_fk, fp_g;
+  fcg_fock_pairs(h, sp, sp, nbody, S, &fp_ptr, &fp_fk, &fp_g);
+  const int64_t n_fp = fp_ptr.back();
+  // term table for the device join
+  std::vector<SparseTerm> terms(h->n_terms);
+  for (int t = 0; t < h->n_terms; t++) {
+    SparseTerm &T = terms[t];
+    T = SparseTerm{};
+    T.nbody = h->nbody[t];
+    T.ioff = h->ioff[t];
+    T.inact[0] = T.inact[1] = 0;
+    for (int c = 0; c < N; c++) {
+      bool act = false;
+      for (int q = 0; q < T.nbody; q++) act |= h->clusters[4 * t + q] == c;
+      if (!act) T.inact[c >> 3] |= 0xFFull << (8 * (c & 7));
+    }
+    for (int q = 0; q < 4; q++) {
+      T.cl[q] = h->clusters[4 * t + q];
+      T.op[q] = h->ops[4 * t + q];
+      T.dims[q] = h->dims[4 * t + q];
+      if (q < T.nbody && h->parity[4 * t + q] == 1) T.odd |= 1 << q;
+    }
+  }
+  DBuf<int64_t> d_fstart, d_fp_ptr;
+  DBuf<int32_t> d_fp_fk, d_fp_g, d_tcs, d_gblock, d_nnz;
+  DBuf<int16_t> d_fock;
+  DBuf<SparseTerm> d_terms;
+  DBuf<unsigned long long> d_stats;
+  int rc = upload(ctx, d_fstart, fstart.data(), (int64_t)fstart.size());
+  if (!rc) rc = upload(ctx, d_fp_ptr, fp_ptr.data(), (int64_t)fp_ptr.size());
+  if (!rc) rc = upload(ctx, d_fp_fk, fp_fk.data(), n_fp);
+  if (!rc) rc = upload(ctx, d_fp_g, fp_g.data(), n_fp);
+  if (!rc) rc = upload(ctx, d_tcs, h->tc_start.data(), (int64_t)h->tc_start.size());
+  if (!rc) rc = upload(ctx, d_gblock, o->gblock.data(), (int64_t)o->gblock.size());
+  if (!rc) rc = upload(ctx, d_fock, sp->fock.data(), (int64_t)sp->fock.size());
+  if (!rc) rc = upload(ctx, d_terms, terms.data(), (int64_t)terms.size());
+  if (rc) return rc;
+  if (hipMalloc((void **)&d_nnz.p, sizeof(int32_t) * std::max<int64_t>(n_row, 1)) != hipSuccess ||
+      hipMalloc((void **)&d_stats.p, 3 * sizeof(unsigned long long)) != hipSuccess) {
+    ctx->err = "hipMalloc failed in the sparse plan";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemsetAsync(d_stats.p, 0, 3 * sizeof(unsigned long long), ctx->stream), ctx);
+  SparseArgs a{};
+  a.N = N;
+  a.S = S;
+  a.n_ops = o->n_ops;
+  a.nbody = nbody;
+  a.fstart = d_fstart.p;
+  a.fock = d_fock.p;
+  a.cfg = sp->d_lo;
+  a.fp_ptr = d_fp_ptr.p;
+  a.fp_fk = d_fp_fk.p;
+  a.fp_g = d_fp_g.p;
+  a.tc_start = d_tcs.p;
+  a.terms = d_terms.p;
+  a.ints = h->d_ints;
+  a.gblock = d_gblock.p;
+  a.gb_off = o->d_gb_off;
+  a.gb_dims = o->d_gb_dims;
+  a.gdata = o->d_gdata;
+  a.row_nnz = d_nnz.p;
+  a.stats = d_stats.p;
+  for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    const unsigned nblk = (unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0);
+    hipLaunchKernelGGL(k_sparse_rows<false>, dim3(nblk), dim3(ST), 0, ctx->stream, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<int32_t> nnz(n_row);
+  unsigned long long st[3];
+  FCG_HIP(hipMemcpyAsync(nnz.data(), d_nnz.p, sizeof(int32_t) * n_row, hipMemcpyDeviceToHost,
+                         ctx->stream), ctx);
+  FCG_HIP(hipMemcpyAsync(st, d_stats.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream), ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  // row panels (db = 1): widths padded to whole quads
+  std::vector<int64_t> col_ptr(n_row + 1, 0);
+  for (int64_t b = 0; b < n_row; b++)
+    col_ptr[b + 1] = col_ptr[b] + (nnz[b] + FCG_QUAD - 1) / FCG_QUAD * FCG_QUAD;
+  const int64_t n_cols = col_ptr[n_row];
+  if (n_cols > INT64_MAX / 8) return FCG_EOVERFLOW;
+  fcg_plan *pl = new fcg_plan();
+  pl->ctx = ctx;
+  pl->sp = pl->ksp = sp;
+  pl->n_pair = n_fp;
+  pl->n_task = (int64_t)st[0];
+  pl->n_kel = n_cols;
+  pl->n_cols = n_cols;
+  pl->build_flops = (double)st[1];
+  pl->precisions = precisions | (1 << FCG_FP64);
+  std::vector<int32_t> order;
+  std::vector<int64_t> var_off(n_row + 1);
+  std::iota(var_off.begin(), var_off.end(), 0);
+  fcg_apply_order(var_off, col_ptr, &order, &pl->n_multi, &pl->n_single_slots);
+  auto fail = [&](int code, const char *what) {
+    if (code == FCG_EDEVICE || code == FCG_ENOMEM) ctx->err = std::string("sparse plan: ") + what;
+    fcg_plan_destroy(pl);
+    return code;
+  };
+  auto alloc = [&](auto **p, int64_t n) {
+    using T = std::remove_reference_t<decltype(**p)>;
+    return hipMalloc((void **)p, sizeof(T) * std::max<int64_t>(n, 1)) == hipSuccess;
+  };
+  if (!alloc(&pl->d_row_koff, n_row) || !alloc(&pl->d_col_ptr, n_row + 1) ||
+      !alloc(&pl->d_order, (int64_t)order.size()) || !alloc(&pl->d_colidx, n_cols) ||
+      !alloc(&pl->d_K64, n_cols) || !alloc(&pl->d_diag_koff, n_row) ||
+      !alloc(&pl->d_diag_col, n_row))
+    return fail(FCG_ENOMEM, "device allocation");
+  if ((pl->precisions & (1 << FCG_FP32)) && !alloc(&pl->d_K32, n_cols))
+    return fail(FCG_ENOMEM, "device allocation");
+  if ((pl->precisions & (1 << FCG_BF16X2)) && !alloc(&pl->d_Kbf, n_cols))
+    return fail(FCG_ENOMEM, "device allocation");
+  hipStream_t s = ctx->stream;
+  if (hipMemcpyAsync(pl->d_row_koff, col_ptr.data(), sizeof(int64_t) * n_row,
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(pl->d_col_ptr, col_ptr.data(), sizeof(int64_t) * (n_row + 1),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(pl->d_order, order.data(), sizeof(int32_t) * order.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(pl->d_colidx, 0, sizeof(int32_t) * std::max<int64_t>(n_cols, 1), s) !=
+          hipSuccess ||
+      hipMemsetAsync(pl->d_K64, 0, sizeof(double) * std::max<int64_t>(n_cols, 1), s) !=
+          hipSuccess ||
+      hipMemsetAsync(pl->d_diag_koff, 0xFF, sizeof(int64_t) * std::max<int64_t>(n_row, 1), s) !=
+          hipSuccess ||
+      hipMemsetAsync(pl->d_diag_col, 0, sizeof(int32_t) * std::max<int64_t>(n_row, 1), s) !=
+          hipSuccess)
+    return fail(FCG_EDEVICE, "upload");
+  a.row_start = pl->d_col_ptr;
+  a.K = pl->d_K64;
+  a.colidx = pl->d_colidx;
+  a.diag_koff = pl->d_diag_koff;
+  a.diag_col = pl->d_diag_col;
+  if (hipMemsetAsync(d_stats.p, 0, 3 * sizeof(unsigned long long), s) != hipSuccess)
+    return fail(FCG_EDEVICE, "upload");
+  for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    const unsigned nblk = (unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0);
+    hipLaunchKernelGGL(k_sparse_rows<true>, dim3(nblk), dim3(ST), 0, s, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = fcg_launch_convert(pl, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(st, d_stats.p, sizeof(st), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    ctx->err = std::string("sparse plan build: ") + hipGetErrorString(e);
+    fcg_plan_destroy(pl);
+    return FCG_EDEVICE;
+  }
+  if (st[2]) {
+    ctx->err = "sparse plan: a term needs a Γ block that ClusterOps lacks";
+    fcg_plan_destroy(pl);
+    return FCG_EMISSING_BLOCK;
+  }
+  pl->build_us =
+      std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  *out = pl;
+  return FCG_OK;
+}

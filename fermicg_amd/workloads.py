@@ -103,7 +103,9 @@ def rank1_space(p, R=4, frac=0.159, seed=2):
     fi = np.concatenate(fi); ci = np.concatenate(ci); si = np.concatenate(si)
     key = (fi.astype(np.uint64) << np.uint64(16)) | (ci.astype(np.uint64) << np.uint64(8)) | si.astype(np.uint64)
     h = splitmix64(key ^ np.uint64(seed))
-    keep = h < np.uint64(int(frac * 2.0 ** 64))
+    keep = h < np.uint64(min(int(frac * 2.0 ** 64), 2 ** 64 - 1))
+    if frac >= 1.0:
+        keep[:] = True
     fi, ci, si = fi[keep], ci[keep], si[keep]
     order = np.lexsort((si, ci, fi))
     fi, ci, si = fi[order], ci[order], si[order]
