@@ -37,7 +37,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci", "sparse"])
+    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci", "sparse", "serve"])
+    ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve workload: arrivals per s")
+    ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
@@ -52,6 +55,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serve":
+        return serve(args)
     import torch
     from fermicg_amd import _lib
     from fermicg_amd.marshal import flatten_ham, flatten_ops
@@ -242,6 +247,49 @@ def main():
                      "traffic_source": traffic_src, "bytes_algorithmic": bytes_alg,
                      "kernel": "k_apply_f32 (block-sparse H c)" if prec != "fp64" else "k_apply_f64"},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def serve(args):
+    """Served mix: a Poisson stream of independent Davidson solves of
+    2-8 clusters, M in {4, 8, 16}, 1-4 roots (fermicg_amd.serve), FP64 then
+    mixed precision over the same stream."""
+    from fermicg_amd.serve import SolveServer, request_stream, summarize
+    reqs = request_stream(args.requests, rate=args.rate, seed=0)
+    # producer processes start before anything in this process touches the GPU
+    srv = SolveServer(slots=args.slots, producers=min(16, os.cpu_count() or 1), hbm_budget=200e9,
+                      conv_thresh=1e-7).start()
+    runs, res_all = {}, {}
+    for prec in ("fp64", "fp32"):
+        srv.set_precision(prec)
+        res = srv.run(reqs, log=log)
+        res_all[prec] = res
+        runs[prec] = dict(summarize(res), peak_hbm_bytes=srv.peak_bytes)
+        log(prec, runs[prec])
+    srv.close()
+    dE = max((max(abs(a - b) for a, b in zip(r64["energies"], r32["energies"]))
+              for r64, r32 in zip(res_all["fp64"], res_all["fp32"])
+              if r64["energies"] and r32["energies"]), default=None)
+    kinds = {}
+    for r in res_all["fp64"]:
+        kinds[r["kind"]] = kinds.get(r["kind"], 0) + 1
+    out = {
+        "metric": "solves/s of the served mix (mixed FP32 vs FP64 GPU at |dE|<=1 uEh); p50/p99 latency",
+        "value": runs["fp32"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.requests, "warmup": 0,
+        "ms_per_step": 1e3 / runs["fp32"]["solves_per_s"] if runs["fp32"]["solves_per_s"] else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded integrals per request, Poisson arrivals)",
+        "config": {"workload": "serve_mix_poisson%d" % int(args.rate), "requests": args.requests,
+                   "rate_per_s": args.rate, "slots": args.slots, "spaces": kinds,
+                   "parallelism": "single GPU"},
+        "serve": runs, "abs_dE_mixed_vs_fp64": dE,
+        "latency_s": {p: {"p50": runs[p]["p50_latency_s"], "p99": runs[p]["p99_latency_s"]}
+                      for p in runs},
+        "requests": [{k: r[k] for k in ("rid", "n_clusters", "M", "R", "kind", "n_var",
+                                        "latency", "plan_s", "solve_s", "footprint", "iterations")}
+                     for r in res_all["fp32"]],
     }
     print(json.dumps(out))
 

@@ -26,7 +26,8 @@ EXPORTS = [
     "fcg_space_nvar", "fcg_space_destroy", "fcg_plan_create", "fcg_plan_destroy",
     "fcg_plan_stats", "fcg_sigma", "fcg_sigma_device", "fcg_diagonal", "fcg_davidson",
     "fcg_last_sigma_time", "fcg_event_record", "fcg_event_elapsed", "fcg_synchronize",
-    "fcg_plan_create_pair", "fcg_tpsci_pt2",
+    "fcg_plan_create_pair", "fcg_tpsci_pt2", "fcg_plan_create_limited", "fcg_plan_device_bytes",
+    "fcg_set_plan_budget", "fcg_plan_budget",
 ]
 
 
@@ -90,6 +91,11 @@ def load():
     L.fcg_plan_create_pair.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
     L.fcg_tpsci_pt2.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64), ctypes.c_int,
                                 ctypes.c_int, P(c_f64), P(PT2Stats)]
+    L.fcg_plan_create_limited.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, c_i64,
+                                          P(c_i64), P(VP)]
+    L.fcg_plan_device_bytes.argtypes = [VP, P(c_i64)]
+    L.fcg_set_plan_budget.argtypes = [c_i64]
+    L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -244,6 +250,30 @@ class Plan(_Handle):
         return pl
 
     @classmethod
+    def create_limited(cls, ctx, ham, ops, space, max_bytes, nbody=4, precisions=("fp64",)):
+        """Plan if its device footprint fits in max_bytes, else None (nothing
+        allocated); returns (plan or None, footprint bytes)."""
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISION[p]
+        h = VP()
+        need = c_i64()
+        rc = ctx.L.fcg_plan_create_limited(ctx.h, ham.h, ops.h, space.h, nbody, mask,
+                                           int(max_bytes), ctypes.byref(need), ctypes.byref(h))
+        if rc == -2 and not h.value:
+            return None, need.value
+        ctx.check(rc, "fcg_plan_create_limited")
+        pl = cls(ctx, h, keep=(ham, ops, space))
+        pl.nvar = pl.nvar_ket = space.nvar()
+        return pl, need.value
+
+    def device_bytes(self):
+        b = c_i64()
+        self.ctx.check(self.ctx.L.fcg_plan_device_bytes(self.h, ctypes.byref(b)),
+                       "fcg_plan_device_bytes")
+        return b.value
+
+    @classmethod
     def create_pair(cls, ctx, ham, ops, bra, ket, nbody=4, precisions=("fp64",)):
         """Plan of H between two spaces: sigma(bra) = H c(ket)."""
         mask = 0
@@ -330,3 +360,17 @@ def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1):
     ctx.check(ctx.L.fcg_tpsci_pt2(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64), _p(e0, c_f64),
                                   nbody, h0_op, _p(e2, c_f64), ctypes.byref(st)), "fcg_tpsci_pt2")
     return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}
+
+
+def set_plan_budget(nbytes):
+    """Process-wide device-byte budget for live plans (fcg_set_plan_budget)."""
+    rc = load().fcg_set_plan_budget(int(nbytes))
+    if rc:
+        raise FCGError("fcg_set_plan_budget: %s" % ERRORS.get(rc, rc))
+
+
+def plan_budget():
+    """(budget, bytes held by live plans)."""
+    b, h = c_i64(), c_i64()
+    load().fcg_plan_budget(ctypes.byref(b), ctypes.byref(h))
+    return b.value, h.value

@@ -1,5 +1,6 @@
 // C ABI of libfcg_hip.so: handles, uploads, plan creation, sigma, diagonal.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 
@@ -338,8 +339,41 @@ int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp) {
   return fcg_space_to_device(sp);
 }
 
+namespace {
+std::atomic<int64_t> g_plan_reserved{0};
+std::atomic<int64_t> g_plan_budget{INT64_MAX};
+}  // namespace
+
+int fcg_plan_reserve(fcg_ctx *ctx, int64_t bytes, int64_t max_bytes, int64_t *need) {
+  if (need) *need = bytes;
+  if (bytes > max_bytes) {
+    ctx->err = "plan needs " + std::to_string(bytes) + " device bytes, limit " +
+               std::to_string(max_bytes);
+    return FCG_ENOMEM;
+  }
+  int64_t cur = g_plan_reserved.load();
+  do {
+    if (cur + bytes > g_plan_budget.load()) {
+      ctx->err = "plan needs " + std::to_string(bytes) + " device bytes; " +
+                 std::to_string(cur) + " of the plan budget " +
+                 std::to_string(g_plan_budget.load()) + " are held";
+      return FCG_ENOMEM;
+    }
+  } while (!g_plan_reserved.compare_exchange_weak(cur, cur + bytes));
+  return FCG_OK;
+}
+
+void fcg_plan_release(int64_t bytes) { g_plan_reserved.fetch_sub(bytes); }
+
+int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisions) {
+  int64_t per = 8;
+  if (precisions & (1 << FCG_FP32)) per += 4;
+  if (precisions & (1 << FCG_BF16X2)) per += 4;
+  return n_kel * per + n_cols * 4 + n_box * 48;
+}
+
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
-                    int nbody, int precisions, fcg_plan **out) {
+                    int nbody, int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need) {
   if (!ctx || !h || !o || !sp || !ksp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
   auto t0 = std::chrono::steady_clock::now();
@@ -347,12 +381,22 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
   if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box) {
     // every box one configuration: join and build on the device
-    return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out);
+    return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
   }
   HostPlan hp;
   int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, &hp, &ctx->err);
   if (rc) return rc;
+  int64_t reserved = 0;
+  {
+    // the pair and task tables live on the device during the build
+    const int64_t bytes = fcg_plan_bytes(hp.n_kel, (int64_t)hp.colidx.size(), sp->n_box,
+                                         precisions | (1 << FCG_FP64)) +
+                          (int64_t)hp.pair_B.size() * 36 + (int64_t)hp.tasks.size() * sizeof(TaskDev);
+    if (int rc2 = fcg_plan_reserve(ctx, bytes, max_bytes, need)) return rc2;
+    reserved = bytes;
+  }
   fcg_plan *pl = new fcg_plan();
+  pl->reserved_bytes = reserved;
   pl->ctx = ctx;
   pl->sp = sp;
   pl->ksp = ksp;
@@ -430,8 +474,34 @@ int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t br
   return fcg_create_plan(ctx, h, o, bra, ket, nbody, precisions, out);
 }
 
+int fcg_plan_create_limited(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int nbody,
+                            int precisions, int64_t max_bytes, int64_t *need_bytes,
+                            fcg_plan_t *out) {
+  if (need_bytes) *need_bytes = 0;
+  return fcg_create_plan(ctx, h, o, sp, sp, nbody, precisions, out, max_bytes, need_bytes);
+}
+
+int fcg_set_plan_budget(int64_t bytes) {
+  if (bytes <= 0) return FCG_EINVAL;
+  g_plan_budget.store(bytes);
+  return FCG_OK;
+}
+
+int fcg_plan_budget(int64_t *budget, int64_t *held) {
+  if (budget) *budget = g_plan_budget.load();
+  if (held) *held = g_plan_reserved.load();
+  return FCG_OK;
+}
+
+int fcg_plan_device_bytes(fcg_plan_t pl, int64_t *bytes) {
+  if (!pl || !bytes) return FCG_EINVAL;
+  *bytes = fcg_plan_bytes(pl->n_kel, pl->n_cols, pl->sp->n_box, pl->precisions);
+  return FCG_OK;
+}
+
 void fcg_plan_destroy(fcg_plan_t pl) {
   if (!pl) return;
+  fcg_plan_release(pl->reserved_bytes);
   dev_free(pl->d_row_ptr);
   dev_free(pl->d_pair_B);
   dev_free(pl->d_pair_K);

@@ -143,6 +143,7 @@ struct fcg_plan {
   uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
   void *d_ci = nullptr;           // c interleaved ([n_var][R], FP64 or FP32) for the apply
   int64_t ci_bytes = 0;
+  int64_t reserved_bytes = 0;     // held against the process-wide plan budget
 };
 
 // host plan construction (fcg_plan_host.cpp)
@@ -187,14 +188,24 @@ int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket,
 // plan of a space whose boxes are all single configurations, joined and
 // built on the device (fcg_sparse.hip)
 int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
-                           int precisions, fcg_plan **out);
+                           int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need);
+// device bytes a plan holds while it is alive (blocks in each precision,
+// column indices, row tables), and the extra bytes its build needs
+int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisions);
+// reserve a plan's bytes against max_bytes and the process-wide plan budget
+// (FCG_ENOMEM, nothing held, when either would be exceeded); release on destroy
+int fcg_plan_reserve(fcg_ctx *ctx, int64_t bytes, int64_t max_bytes, int64_t *need);
+void fcg_plan_release(int64_t bytes);
 constexpr int FCG_SPARSE_MAX_BLOCK = 256;  // configurations per Fock block (device join)
 
 // shared helpers (fcg_api.hip)
 int fcg_space_to_device(fcg_space *sp);
 int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp);
+// max_bytes: refuse (FCG_ENOMEM, nothing allocated, *need set) when the
+// plan's device footprint would exceed it
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
-                    int nbody, int precisions, fcg_plan **out);
+                    int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
+                    int64_t *need = nullptr);
 
 // kernels (fcg_kernels.hip)
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,

@@ -267,7 +267,7 @@ constexpr int64_t FB_CHUNK = 65536;  // bra Fock blocks per launch
 }  // namespace
 
 int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
-                           int precisions, fcg_plan **out) {
+                           int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need) {
   const auto t0 = std::chrono::steady_clock::now();
   const int N = sp->N, S = o->S;
   const int64_t n_row = sp->n_box;
@@ -374,8 +374,15 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   for (int64_t b = 0; b < n_row; b++)
     col_ptr[b + 1] = col_ptr[b] + (nnz[b] + FCG_QUAD - 1) / FCG_QUAD * FCG_QUAD;
   const int64_t n_cols = col_ptr[n_row];
-  if (n_cols > INT64_MAX / 8) return FCG_EOVERFLOW;
+  if (n_cols > INT64_MAX / 32) return FCG_EOVERFLOW;
+  int64_t reserved = 0;
+  {
+    const int64_t bytes = fcg_plan_bytes(n_cols, n_cols, n_row, precisions | (1 << FCG_FP64));
+    if (int rc2 = fcg_plan_reserve(ctx, bytes, max_bytes, need)) return rc2;
+    reserved = bytes;
+  }
   fcg_plan *pl = new fcg_plan();
+  pl->reserved_bytes = reserved;
   pl->ctx = ctx;
   pl->sp = pl->ksp = sp;
   pl->n_pair = n_fp;

@@ -112,6 +112,24 @@ int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp,
  * fcg_plan_create. */
 int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t bra,
                          fcg_space_t ket, int nbody, int precisions, fcg_plan_t *out);
+/* fcg_plan_create with admission by device footprint: when the plan would
+ * hold more than max_bytes of device memory it returns FCG_ENOMEM having
+ * allocated nothing; *need_bytes receives the footprint either way.  No
+ * counterpart in the reference; for the serving loop, which admits solves
+ * as HBM frees up. */
+int fcg_plan_create_limited(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp,
+                            int nbody, int precisions, int64_t max_bytes, int64_t *need_bytes,
+                            fcg_plan_t *out);
+/* Device bytes held by a plan (blocks, column indices, row tables). */
+int fcg_plan_device_bytes(fcg_plan_t plan, int64_t *bytes);
+/* Process-wide budget for the device bytes of all live plans (default
+ * unlimited): every plan creation reserves its footprint against it before
+ * allocating and returns FCG_ENOMEM, having allocated nothing, when it would
+ * not fit; destroying a plan releases its bytes.  fcg_plan_budget reports the
+ * budget and the bytes held.  No counterpart in the reference; for the
+ * serving loop, which admits concurrent solves by HBM footprint. */
+int fcg_set_plan_budget(int64_t bytes);
+int fcg_plan_budget(int64_t *budget, int64_t *held);
 void fcg_plan_destroy(fcg_plan_t plan);
 /* Plan statistics: [0] block pairs, [1] (term, block pair) tasks,
  * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us),
