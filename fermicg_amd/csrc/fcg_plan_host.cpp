@@ -570,26 +570,87 @@ int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int
 // Fock block, the ket Fock blocks F_bra - TransferConfig present in `ket`,
 // with the TransferConfig group, in group order (the lookup of
 // tps_ci_matvec, src/tpsci_outer.jl:479-481).  Groups without a term of at
-// most nbody clusters are skipped.
+// most nbody clusters are skipped.  Fock configurations are packed into two
+// 64-bit words (4 bits per electron count) and looked up in an
+// open-addressing table.
+namespace {
+struct FockKey {
+  uint64_t w[2];
+  bool operator==(const FockKey &o) const { return w[0] == o.w[0] && w[1] == o.w[1]; }
+};
+inline uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  return x ^ (x >> 33);
+}
+}  // namespace
+
 int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket, int nbody,
                    int S, std::vector<int64_t> *ptr, std::vector<int32_t> *fk_out,
                    std::vector<int32_t> *g_out) {
   const int N = bra->N;
-  std::unordered_map<std::string, int32_t> fmap;
-  fmap.reserve(ket->n_fock * 2);
-  for (int64_t f = 0; f < ket->n_fock; f++) fmap.emplace(fock_key(&ket->fock[f * 2 * N], N), (int32_t)f);
-  std::vector<char> live(h->n_tc, 0);
+  if (S > 16 || N > 16) {  // 4-bit counts, 32 counts in two words
+    // general case: string keys
+    std::unordered_map<std::string, int32_t> fmap;
+    for (int64_t f = 0; f < ket->n_fock; f++) fmap.emplace(fock_key(&ket->fock[f * 2 * N], N), (int32_t)f);
+    ptr->assign(bra->n_fock + 1, 0);
+    fk_out->clear();
+    g_out->clear();
+    int16_t fk[2 * FCG_MAXN];
+    for (int64_t fb = 0; fb < bra->n_fock; fb++) {
+      const int16_t *F = &bra->fock[fb * 2 * N];
+      for (int g = 0; g < h->n_tc; g++) {
+        bool ok = false;
+        for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++) ok |= h->nbody[t] <= nbody;
+        for (int i = 0; i < 2 * N && ok; i++) {
+          fk[i] = F[i] - h->tc[(int64_t)g * 2 * N + i];
+          ok = fk[i] >= 0 && fk[i] < S;
+        }
+        if (!ok) continue;
+        auto it = fmap.find(fock_key(fk, N));
+        if (it == fmap.end()) continue;
+        fk_out->push_back(it->second);
+        g_out->push_back(g);
+      }
+      (*ptr)[fb + 1] = (int64_t)fk_out->size();
+    }
+    return FCG_OK;
+  }
+  auto pack = [N](const int16_t *f) {
+    FockKey k{{0, 0}};
+    for (int i = 0; i < 2 * N; i++) k.w[i >> 4] |= (uint64_t)(f[i] & 0xF) << (4 * (i & 15));
+    return k;
+  };
+  // ket table: key and Fock index side by side (one cache line per probe)
+  struct Slot {
+    FockKey k;
+    int64_t f;
+  };
+  int64_t cap = 16;
+  while (cap < 2 * ket->n_fock) cap <<= 1;
+  std::vector<Slot> table(cap, Slot{{{0, 0}}, -1});
+  for (int64_t f = 0; f < ket->n_fock; f++) {
+    const FockKey k = pack(&ket->fock[f * 2 * N]);
+    uint64_t hsh = mix64(k.w[0] ^ mix64(k.w[1])) & (cap - 1);
+    while (table[hsh].f >= 0) hsh = (hsh + 1) & (cap - 1);
+    table[hsh] = Slot{k, f};
+  }
+  std::vector<int> live;
   for (int g = 0; g < h->n_tc; g++)
     for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++)
-      if (h->nbody[t] <= nbody) live[g] = 1;
+      if (h->nbody[t] <= nbody) {
+        live.push_back(g);
+        break;
+      }
   std::vector<std::vector<int32_t>> per(bra->n_fock);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t fb = 0; fb < bra->n_fock; fb++) {
     const int16_t *F = &bra->fock[fb * 2 * N];
     int16_t fk[2 * FCG_MAXN];
     std::vector<int32_t> &v = per[fb];
-    for (int g = 0; g < h->n_tc; g++) {
-      if (!live[g]) continue;
+    for (int g : live) {
       const int16_t *tc = &h->tc[(int64_t)g * 2 * N];
       bool ok = true;
       for (int i = 0; i < 2 * N && ok; i++) {
@@ -597,10 +658,16 @@ int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket,
         ok = fk[i] >= 0 && fk[i] < S;
       }
       if (!ok) continue;
-      auto it = fmap.find(fock_key(fk, N));
-      if (it == fmap.end()) continue;
-      v.push_back(it->second);
-      v.push_back(g);
+      const FockKey k = pack(fk);
+      uint64_t hsh = mix64(k.w[0] ^ mix64(k.w[1])) & (cap - 1);
+      while (table[hsh].f >= 0) {
+        if (table[hsh].k == k) {
+          v.push_back((int32_t)table[hsh].f);
+          v.push_back(g);
+          break;
+        }
+        hsh = (hsh + 1) & (cap - 1);
+      }
     }
   }
   ptr->assign(bra->n_fock + 1, 0);

@@ -81,26 +81,84 @@ __device__ __forceinline__ int state_of(const uint64_t *p, int c) {
   return (int)((p[c >> 3] >> (8 * (c & 7))) & 0xFFu);
 }
 
-// contract_matrix_element for one term and one (I, J); *miss set when a
-// needed Γ block is absent
-__device__ double contract(const SparseArgs &a, const SparseTerm &T, const int16_t *fb,
-                           const int16_t *fk, const uint64_t *pI, const uint64_t *pJ,
-                           bool *miss) {
+// sum_p V[p0,p1,..] g0[p0] g1[p1] ...; the first Γ column (D0 values) is held
+// in registers, V is read in order (from LDS when staged there)
+template <int D0>
+__device__ __forceinline__ double contract_v(const double *V, const double *const *g,
+                                             const int32_t *d, int nbody) {
+  double g0[D0];
+#pragma unroll
+  for (int k = 0; k < D0; k++) g0[k] = g[0][k];
+  double acc = 0.0;
+  if (nbody == 2) {
+    for (int p1 = 0; p1 < d[1]; p1++) {
+      const double *Vp = V + D0 * p1;
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < D0; k++) s += Vp[k] * g0[k];
+      acc += s * g[1][p1];
+    }
+  } else if (nbody == 3) {
+    for (int p2 = 0; p2 < d[2]; p2++) {
+      double s2 = 0.0;
+      for (int p1 = 0; p1 < d[1]; p1++) {
+        const double *Vp = V + D0 * (p1 + d[1] * p2);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D0; k++) s += Vp[k] * g0[k];
+        s2 += s * g[1][p1];
+      }
+      acc += s2 * g[2][p2];
+    }
+  } else {
+    for (int p3 = 0; p3 < d[3]; p3++) {
+      double s3 = 0.0;
+      for (int p2 = 0; p2 < d[2]; p2++) {
+        double s2 = 0.0;
+        for (int p1 = 0; p1 < d[1]; p1++) {
+          const double *Vp = V + D0 * (p1 + d[1] * (p2 + d[2] * p3));
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < D0; k++) s += Vp[k] * g0[k];
+          s2 += s * g[1][p1];
+        }
+        s3 += s2 * g[2][p2];
+      }
+      acc += s3 * g[3][p3];
+    }
+  }
+  return acc;
+}
+
+constexpr int SV = 2048;   // staged integrals of a TransferConfig group (doubles)
+constexpr int SVT = 64;    // staged terms per group
+
+// per (Fock pair, term): where each active cluster's Γ block starts, its
+// leading dimension and bra state count, and the sign; blk < 0 marks a
+// missing block
+struct TermPair {
+  int64_t base[4];
+  int32_t np[4], mb[4];
+  int32_t ok;
+  float sgn;
+};
+
+__device__ void term_pair(const SparseArgs &a, const SparseTerm &T, const int16_t *fb,
+                          const int16_t *fk, TermPair *out) {
   const int SS = a.S * a.S;
-  const double *g[4];
+  out->ok = 1;
   for (int q = 0; q < T.nbody; q++) {
     const int c = T.cl[q];
     const int cb = fb[2 * c] * a.S + fb[2 * c + 1], ck = fk[2 * c] * a.S + fk[2 * c + 1];
     const int blk = a.gblock[(((int64_t)c * a.n_ops + T.op[q]) * SS + cb) * SS + ck];
     if (blk < 0) {
-      *miss = true;
-      return 0.0;
+      out->ok = 0;
+      return;
     }
-    const int np = a.gb_dims[3 * blk], mb = a.gb_dims[3 * blk + 1];
-    g[q] = a.gdata + a.gb_off[blk] +
-           (int64_t)np * ((state_of(pI, c) - 1) + (int64_t)mb * (state_of(pJ, c) - 1));
+    out->base[q] = a.gb_off[blk];
+    out->np[q] = a.gb_dims[3 * blk];
+    out->mb[q] = a.gb_dims[3 * blk + 1];
   }
-  // sign from the ket Fock configuration
   int par = 0;
   for (int q = 0; q < T.nbody; q++) {
     if (!((T.odd >> q) & 1)) continue;
@@ -108,40 +166,94 @@ __device__ double contract(const SparseArgs &a, const SparseTerm &T, const int16
     for (int c = 0; c < T.cl[q]; c++) ne += fk[2 * c] + fk[2 * c + 1];
     par ^= ne & 1;
   }
-  const double sgn = par ? -1.0 : 1.0;
-  if (T.nbody == 1) return sgn * g[0][0];
-  const double *V = a.ints + T.ioff;
-  const int d0 = T.dims[0], d1 = T.dims[1];
+  out->sgn = par ? -1.f : 1.f;
+}
+
+// sum_p V[p0,p1,p2,p3] g0[p0] g1[p1] g2[p2] g3[p3] for a fixed shape: every
+// Γ column is loaded into registers once, V (from LDS) is read in order
+template <int D0, int D1, int D2, int D3>
+__device__ __forceinline__ double contract_fixed(const double *V, const double *const *g) {
+  double a0[D0], a1[D1], a2[D2], a3[D3];
+#pragma unroll
+  for (int k = 0; k < D0; k++) a0[k] = g[0][k];
+#pragma unroll
+  for (int k = 0; k < D1; k++) a1[k] = g[1][k];
+#pragma unroll
+  for (int k = 0; k < D2; k++) a2[k] = D2 > 1 ? g[2][k] : 1.0;
+#pragma unroll
+  for (int k = 0; k < D3; k++) a3[k] = D3 > 1 ? g[3][k] : 1.0;
   double acc = 0.0;
-  if (T.nbody == 2) {
-    for (int p1 = 0; p1 < d1; p1++) {
-      double s = 0.0;
-      for (int p0 = 0; p0 < d0; p0++) s += V[p0 + d0 * p1] * g[0][p0];
-      acc += s * g[1][p1];
+#pragma unroll
+  for (int p3 = 0; p3 < D3; p3++) {
+    double s3 = 0.0;
+#pragma unroll
+    for (int p2 = 0; p2 < D2; p2++) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int p1 = 0; p1 < D1; p1++) {
+        const double *Vp = V + D0 * (p1 + D1 * (p2 + D2 * p3));
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D0; k++) s += Vp[k] * a0[k];
+        s2 += s * a1[p1];
+      }
+      s3 += s2 * a2[p2];
     }
-  } else if (T.nbody == 3) {
-    const int d2 = T.dims[2];
+    acc += s3 * a3[p3];
+  }
+  return acc;
+}
+
+// any shape: column-major V over nbody dims
+__device__ __forceinline__ double contract_any(const double *V, const double *const *g,
+                                               const int32_t *d, int nbody) {
+  const int d0 = d[0], d1 = d[1], d2 = nbody > 2 ? d[2] : 1, d3 = nbody > 3 ? d[3] : 1;
+  double acc = 0.0;
+  for (int p3 = 0; p3 < d3; p3++)
     for (int p2 = 0; p2 < d2; p2++)
       for (int p1 = 0; p1 < d1; p1++) {
-        const double *Vp = V + d0 * (p1 + d1 * p2);
+        const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
         double s = 0.0;
         for (int p0 = 0; p0 < d0; p0++) s += Vp[p0] * g[0][p0];
-        acc += s * g[1][p1] * g[2][p2];
+        double f = g[1][p1];
+        if (nbody > 2) f *= g[2][p2];
+        if (nbody > 3) f *= g[3][p3];
+        acc += s * f;
       }
-  } else {
-    const int d2 = T.dims[2], d3 = T.dims[3];
-    for (int p3 = 0; p3 < d3; p3++)
-      for (int p2 = 0; p2 < d2; p2++) {
-        const double t23 = g[2][p2] * g[3][p3];
-        for (int p1 = 0; p1 < d1; p1++) {
-          const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
-          double s = 0.0;
-          for (int p0 = 0; p0 < d0; p0++) s += Vp[p0] * g[0][p0];
-          acc += s * g[1][p1] * t23;
-        }
-      }
+  return acc;
+}
+
+// the shapes of 4-orbital clusters; 0 = no fixed shape
+__device__ __forceinline__ int shape_id(const SparseTerm &T) {
+  const int *d = T.dims;
+  if (T.nbody == 4 && d[0] == 4 && d[1] == 4 && d[2] == 4 && d[3] == 4) return 1;
+  if (T.nbody == 3) {
+    if (d[0] == 16 && d[1] == 4 && d[2] == 4) return 2;
+    if (d[0] == 4 && d[1] == 16 && d[2] == 4) return 3;
+    if (d[0] == 4 && d[1] == 4 && d[2] == 16) return 4;
   }
-  return sgn * acc;
+  if (T.nbody == 2) {
+    if (d[0] == 16 && d[1] == 16) return 5;
+    if (d[0] == 4 && d[1] == 4) return 6;
+  }
+  return 0;
+}
+
+// sum_p V[p0,p1] g0[p0] g1[p1] with the second Γ column (D1 values) in
+// registers (two-cluster terms whose first string has three operators)
+template <int D1>
+__device__ __forceinline__ double contract_v1(const double *V, const double *const *g, int d0) {
+  double g1[D1];
+#pragma unroll
+  for (int k = 0; k < D1; k++) g1[k] = g[1][k];
+  double acc = 0.0;
+  for (int p0 = 0; p0 < d0; p0++) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D1; k++) s += V[p0 + d0 * k] * g1[k];
+    acc += s * g[0][p0];
+  }
+  return acc;
 }
 
 template <bool FILL>
@@ -151,6 +263,10 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
   __shared__ int32_t sCur[MB];
   __shared__ int16_t sFb[2 * FCG_MAXN], sFk[2 * FCG_MAXN];
   __shared__ unsigned long long sTrip, sFlop, sMiss;
+  __shared__ double sV[FILL ? SV : 1];
+  __shared__ int32_t sVoff[FILL ? SVT : 1];
+  __shared__ TermPair sTP[FILL ? SVT : 1];
+  __shared__ int sChunkEnd;
   const int tid = threadIdx.x;
   const int N = a.N;
   const int64_t fb = a.fb_base + blockIdx.x;
@@ -197,31 +313,103 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
     }
     __syncthreads();
     if (FILL) {
-      for (int e = tid; e < nb * nk; e += ST) {
-        const int I = e / nk, J = e % nk;
-        const uint64_t w = sBits[I][J >> 6];
-        if (!((w >> (J & 63)) & 1ull)) continue;
-        int below = __popcll(w & ((1ull << (J & 63)) - 1ull));
-        for (int k = 0; k < (J >> 6); k++) below += __popcll(sBits[I][k]);
-        const int64_t pos = a.row_start[b0 + I] + sCur[I] + below;
-        double v = 0.0;
-        bool ms = false;
-        for (int t = t0; t < t1; t++) {
+      // the group's terms in chunks that fit the stage (integrals and
+      // per-pair Γ block data in LDS); element values accumulate in K over
+      // the chunks in term order
+      for (int c0 = t0, first = 1; c0 < t1; first = 0) {
+        if (tid == 0) {
+          int off = 0, t = c0;
+          for (; t < t1 && t - c0 < SVT; t++) {
+            const SparseTerm &T = a.terms[t];
+            int sz = 1;
+            if (T.nbody > 1)
+              for (int k = 0; k < T.nbody; k++) sz *= T.dims[k];
+            else
+              sz = 0;
+            if (T.nbody > a.nbody) sz = 0;
+            if (sz > SV) {  // larger than the stage: integrals read from global memory
+              sVoff[t - c0] = -2;
+              continue;
+            }
+            if (off + sz > SV) break;
+            sVoff[t - c0] = T.nbody > a.nbody ? -1 : off;
+            off += sz;
+          }
+          sChunkEnd = t;
+        }
+        __syncthreads();
+        const int c1 = sChunkEnd;
+        for (int t = c0 + tid; t < c1; t += ST) term_pair(a, a.terms[t], sFb, sFk, &sTP[t - c0]);
+        for (int t = c0; t < c1; t++) {
+          const int off = sVoff[t - c0];
           const SparseTerm &T = a.terms[t];
-          if (T.nbody > a.nbody) continue;
-          if (((sPI[I][0] ^ sPJ[J][0]) & T.inact[0]) | ((sPI[I][1] ^ sPJ[J][1]) & T.inact[1]))
-            continue;
-          v += contract(a, T, sFb, sFk, sPI[I], sPJ[J], &ms);
+          if (off < 0 || T.nbody == 1) continue;  // skipped, or integrals left in global memory
+          int sz = 1;
+          for (int k = 0; k < T.nbody; k++) sz *= T.dims[k];
+          for (int e = tid; e < sz; e += ST) sV[off + e] = a.ints[T.ioff + e];
         }
-        if (ms) miss++;
-        a.K[pos] = v;
-        a.colidx[pos] = (int32_t)(k0 + J);
-        if (fk == fb && J == I) {
-          a.diag_koff[b0 + I] = a.row_start[b0 + I];
-          a.diag_col[b0 + I] = sCur[I] + below;
+        __syncthreads();
+        for (int e = tid; e < nb * nk; e += ST) {
+          const int I = e / nk, J = e % nk;
+          const uint64_t w = sBits[I][J >> 6];
+          if (!((w >> (J & 63)) & 1ull)) continue;
+          int below = __popcll(w & ((1ull << (J & 63)) - 1ull));
+          for (int k = 0; k < (J >> 6); k++) below += __popcll(sBits[I][k]);
+          const int64_t pos = a.row_start[b0 + I] + sCur[I] + below;
+          double v = 0.0;
+          bool ms = false;
+          for (int t = c0; t < c1; t++) {
+            const SparseTerm &T = a.terms[t];
+            const int off = sVoff[t - c0];
+            if (off == -1) continue;
+            if (((sPI[I][0] ^ sPJ[J][0]) & T.inact[0]) | ((sPI[I][1] ^ sPJ[J][1]) & T.inact[1]))
+              continue;
+            const TermPair &tp = sTP[t - c0];
+            if (!tp.ok) {
+              ms = true;
+              continue;
+            }
+            const double *g[4];
+            for (int q = 0; q < T.nbody; q++) {
+              const int c = T.cl[q];
+              g[q] = a.gdata + tp.base[q] +
+                     (int64_t)tp.np[q] * ((state_of(sPI[I], c) - 1) +
+                                          (int64_t)tp.mb[q] * (state_of(sPJ[J], c) - 1));
+            }
+            const int d0 = T.dims[0];
+            double x;
+            switch (T.nbody == 1 ? 7 : off < 0 ? 0 : shape_id(T)) {
+              case 1: x = contract_fixed<4, 4, 4, 4>(sV + off, g); break;
+              case 2: x = contract_fixed<16, 4, 4, 1>(sV + off, g); break;
+              case 3: x = contract_fixed<4, 16, 4, 1>(sV + off, g); break;
+              case 4: x = contract_fixed<4, 4, 16, 1>(sV + off, g); break;
+              case 5: x = contract_fixed<16, 16, 1, 1>(sV + off, g); break;
+              case 6: x = contract_fixed<4, 4, 1, 1>(sV + off, g); break;
+              case 7: x = g[0][0]; break;
+              default:
+                if (off < 0) x = contract_any(a.ints + T.ioff, g, T.dims, T.nbody);
+                else if (d0 == 4) x = contract_v<4>(sV + off, g, T.dims, T.nbody);
+                else if (d0 == 16) x = contract_v<16>(sV + off, g, T.dims, T.nbody);
+                else if (T.nbody == 2 && T.dims[1] == 4) x = contract_v1<4>(sV + off, g, d0);
+                else x = contract_any(sV + off, g, T.dims, T.nbody);
+            }
+            v += (double)tp.sgn * x;
+          }
+          if (ms) miss++;
+          if (first) {
+            a.K[pos] = v;
+            a.colidx[pos] = (int32_t)(k0 + J);
+            if (fk == fb && J == I) {
+              a.diag_koff[b0 + I] = a.row_start[b0 + I];
+              a.diag_col[b0 + I] = sCur[I] + below;
+            }
+          } else {
+            a.K[pos] += v;
+          }
         }
+        __syncthreads();
+        c0 = c1;
       }
-      __syncthreads();
     }
     for (int I = tid; I < nb; I += ST) {
       int n = 0;
@@ -240,6 +428,106 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
     if (sTrip) atomicAdd(&a.stats[0], sTrip);
     if (sFlop) atomicAdd(&a.stats[1], sFlop);
     if (sMiss) atomicAdd(&a.stats[2], sMiss);
+  }
+}
+
+// ---- connected Fock pairs on the device ----
+// Fock configurations packed 4 bits per electron count (32 counts in two
+// words); ket table: open addressing, key and Fock index per slot.
+struct FockSlot {
+  uint64_t w0, w1;
+  int64_t f;
+};
+
+__device__ __forceinline__ uint64_t dmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  return x ^ (x >> 33);
+}
+
+struct PairArgs {
+  int N, S;
+  int64_t n_fock, fb_base;
+  const int16_t *fock;     // bra Fock configurations [n_fock][N][2]
+  const int16_t *tc;       // TransferConfigs of the live groups [n_live][N][2]
+  const int32_t *live;     // group index of each live TransferConfig
+  int n_live;
+  const FockSlot *table;
+  int64_t cap;
+  int32_t *count;          // count pass: pairs per bra Fock block
+  const int64_t *ptr;      // fill pass
+  int32_t *fk, *g;
+};
+
+template <bool FILL>
+__global__ __launch_bounds__(ST) void k_fock_pairs(PairArgs a) {
+  __shared__ int16_t sF[2 * FCG_MAXN];
+  __shared__ int32_t sScan[ST];
+  __shared__ int32_t sBase;
+  const int tid = threadIdx.x;
+  const int64_t fb = a.fb_base + blockIdx.x;
+  const int N2 = 2 * a.N;
+  if (tid < N2) sF[tid] = a.fock[fb * N2 + tid];
+  if (tid == 0) sBase = 0;
+  __syncthreads();
+  int total = 0;
+  for (int g0 = 0; g0 < a.n_live; g0 += ST) {
+    const int j = g0 + tid;
+    int32_t hit = -1;
+    if (j < a.n_live) {
+      const int16_t *tc = a.tc + (int64_t)j * N2;
+      uint64_t w0 = 0, w1 = 0;
+      bool ok = true;
+      for (int i = 0; i < N2; i++) {
+        const int v = sF[i] - tc[i];
+        ok = ok && v >= 0 && v < a.S;
+        const uint64_t nib = (uint64_t)(v & 0xF);
+        if (i < 16) w0 |= nib << (4 * i);
+        else w1 |= nib << (4 * (i - 16));
+      }
+      if (ok) {
+        uint64_t h = dmix64(w0 ^ dmix64(w1)) & (uint64_t)(a.cap - 1);
+        while (a.table[h].f >= 0) {
+          if (a.table[h].w0 == w0 && a.table[h].w1 == w1) {
+            hit = (int32_t)a.table[h].f;
+            break;
+          }
+          h = (h + 1) & (uint64_t)(a.cap - 1);
+        }
+      }
+    }
+    if (!FILL) {
+      total += hit >= 0;
+    } else {
+      // ordered compaction: block-wide exclusive scan of the hit flags
+      sScan[tid] = hit >= 0;
+      __syncthreads();
+      for (int off = 1; off < ST; off <<= 1) {
+        const int v = tid >= off ? sScan[tid - off] : 0;
+        __syncthreads();
+        sScan[tid] += v;
+        __syncthreads();
+      }
+      if (hit >= 0) {
+        const int64_t at = a.ptr[fb] + sBase + sScan[tid] - 1;
+        a.fk[at] = hit;
+        a.g[at] = a.live[j];
+      }
+      __syncthreads();
+      if (tid == ST - 1) sBase += sScan[ST - 1];
+      __syncthreads();
+    }
+  }
+  if (!FILL) {
+    sScan[tid] = total;
+    __syncthreads();
+    if (tid == 0) {
+      int s = 0;
+      for (int k = 0; k < ST; k++) s += sScan[k];
+      a.count[fb] = s;
+    }
   }
 }
 
@@ -263,6 +551,99 @@ int upload(fcg_ctx *ctx, DBuf<T> &b, const T *src, int64_t n) {
 }
 
 constexpr int64_t FB_CHUNK = 65536;  // bra Fock blocks per launch
+
+uint64_t hmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  return x ^ (x >> 33);
+}
+
+// Connected Fock pairs (bra block, ket block F_bra - TransferConfig, group),
+// in group order per bra block, as CSR on the device.  Same list as the
+// host fcg_fock_pairs; N <= 16 clusters, electron counts below 16.
+int device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int nbody, int S,
+                      DBuf<int64_t> &d_ptr, DBuf<int32_t> &d_fk, DBuf<int32_t> &d_g,
+                      int64_t *n_out) {
+  const int N = sp->N;
+  int64_t cap = 16;
+  while (cap < 2 * sp->n_fock) cap <<= 1;
+  std::vector<FockSlot> table(cap, FockSlot{0, 0, -1});
+  for (int64_t f = 0; f < sp->n_fock; f++) {
+    uint64_t w[2] = {0, 0};
+    for (int i = 0; i < 2 * N; i++)
+      w[i >> 4] |= (uint64_t)(sp->fock[f * 2 * N + i] & 0xF) << (4 * (i & 15));
+    uint64_t hh = hmix64(w[0] ^ hmix64(w[1])) & (cap - 1);
+    while (table[hh].f >= 0) hh = (hh + 1) & (cap - 1);
+    table[hh] = FockSlot{w[0], w[1], f};
+  }
+  std::vector<int32_t> live;
+  std::vector<int16_t> tcl;
+  for (int g = 0; g < h->n_tc; g++) {
+    bool ok = false;
+    for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++) ok |= h->nbody[t] <= nbody;
+    if (!ok) continue;
+    live.push_back(g);
+    tcl.insert(tcl.end(), &h->tc[(int64_t)g * 2 * N], &h->tc[(int64_t)(g + 1) * 2 * N]);
+  }
+  DBuf<FockSlot> d_table;
+  DBuf<int16_t> d_fock, d_tc;
+  DBuf<int32_t> d_live, d_count;
+  int rc = upload(ctx, d_table, table.data(), cap);
+  if (!rc) rc = upload(ctx, d_fock, sp->fock.data(), (int64_t)sp->fock.size());
+  if (!rc) rc = upload(ctx, d_tc, tcl.data(), (int64_t)tcl.size());
+  if (!rc) rc = upload(ctx, d_live, live.data(), (int64_t)live.size());
+  if (rc) return rc;
+  if (hipMalloc((void **)&d_count.p, sizeof(int32_t) * std::max<int64_t>(sp->n_fock, 1)) !=
+      hipSuccess) {
+    ctx->err = "hipMalloc failed in the sparse plan";
+    return FCG_ENOMEM;
+  }
+  PairArgs a{};
+  a.N = N;
+  a.S = S;
+  a.n_fock = sp->n_fock;
+  a.fock = d_fock.p;
+  a.tc = d_tc.p;
+  a.live = d_live.p;
+  a.n_live = (int)live.size();
+  a.table = d_table.p;
+  a.cap = cap;
+  a.count = d_count.p;
+  for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    hipLaunchKernelGGL(k_fock_pairs<false>, dim3((unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0)),
+                       dim3(ST), 0, ctx->stream, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<int32_t> cnt(sp->n_fock);
+  FCG_HIP(hipMemcpyAsync(cnt.data(), d_count.p, sizeof(int32_t) * sp->n_fock,
+                         hipMemcpyDeviceToHost, ctx->stream), ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  std::vector<int64_t> ptr(sp->n_fock + 1, 0);
+  for (int64_t f = 0; f < sp->n_fock; f++) ptr[f + 1] = ptr[f] + cnt[f];
+  const int64_t n = ptr.back();
+  rc = upload(ctx, d_ptr, ptr.data(), (int64_t)ptr.size());
+  if (rc) return rc;
+  if (hipMalloc((void **)&d_fk.p, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess ||
+      hipMalloc((void **)&d_g.p, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
+    ctx->err = "hipMalloc failed in the sparse plan";
+    return FCG_ENOMEM;
+  }
+  a.ptr = d_ptr.p;
+  a.fk = d_fk.p;
+  a.g = d_g.p;
+  for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    hipLaunchKernelGGL(k_fock_pairs<true>, dim3((unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0)),
+                       dim3(ST), 0, ctx->stream, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  *n_out = n;
+  return FCG_OK;
+}
 
 }  // namespace
 
@@ -295,8 +676,16 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   }
   std::vector<int64_t> fp_ptr;
   std::vector<int32_t> fp_fk, fp_g;
-  fcg_fock_pairs(h, sp, sp, nbody, S, &fp_ptr, &fp_fk, &fp_g);
-  const int64_t n_fp = fp_ptr.back();
+  DBuf<int64_t> d_fp_ptr;
+  DBuf<int32_t> d_fp_fk, d_fp_g;
+  const bool dev_pairs = S <= 16 && N <= 16;
+  int64_t n_fp = 0;
+  if (!dev_pairs) {
+    fcg_fock_pairs(h, sp, sp, nbody, S, &fp_ptr, &fp_fk, &fp_g);
+    n_fp = fp_ptr.back();
+  } else if (int rc0 = device_fock_pairs(ctx, h, sp, nbody, S, d_fp_ptr, d_fp_fk, d_fp_g, &n_fp)) {
+    return rc0;
+  }
   // term table for the device join
   std::vector<SparseTerm> terms(h->n_terms);
   for (int t = 0; t < h->n_terms; t++) {
@@ -317,15 +706,15 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
       if (q < T.nbody && h->parity[4 * t + q] == 1) T.odd |= 1 << q;
     }
   }
-  DBuf<int64_t> d_fstart, d_fp_ptr;
-  DBuf<int32_t> d_fp_fk, d_fp_g, d_tcs, d_gblock, d_nnz;
+  DBuf<int64_t> d_fstart;
+  DBuf<int32_t> d_tcs, d_gblock, d_nnz;
   DBuf<int16_t> d_fock;
   DBuf<SparseTerm> d_terms;
   DBuf<unsigned long long> d_stats;
   int rc = upload(ctx, d_fstart, fstart.data(), (int64_t)fstart.size());
-  if (!rc) rc = upload(ctx, d_fp_ptr, fp_ptr.data(), (int64_t)fp_ptr.size());
-  if (!rc) rc = upload(ctx, d_fp_fk, fp_fk.data(), n_fp);
-  if (!rc) rc = upload(ctx, d_fp_g, fp_g.data(), n_fp);
+  if (!rc && !dev_pairs) rc = upload(ctx, d_fp_ptr, fp_ptr.data(), (int64_t)fp_ptr.size());
+  if (!rc && !dev_pairs) rc = upload(ctx, d_fp_fk, fp_fk.data(), n_fp);
+  if (!rc && !dev_pairs) rc = upload(ctx, d_fp_g, fp_g.data(), n_fp);
   if (!rc) rc = upload(ctx, d_tcs, h->tc_start.data(), (int64_t)h->tc_start.size());
   if (!rc) rc = upload(ctx, d_gblock, o->gblock.data(), (int64_t)o->gblock.size());
   if (!rc) rc = upload(ctx, d_fock, sp->fock.data(), (int64_t)sp->fock.size());
