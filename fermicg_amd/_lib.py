@@ -27,7 +27,7 @@ EXPORTS = [
     "fcg_plan_stats", "fcg_sigma", "fcg_sigma_device", "fcg_diagonal", "fcg_davidson",
     "fcg_last_sigma_time", "fcg_event_record", "fcg_event_elapsed", "fcg_synchronize",
     "fcg_plan_create_pair", "fcg_tpsci_pt2", "fcg_plan_create_limited", "fcg_plan_device_bytes",
-    "fcg_set_plan_budget", "fcg_plan_budget",
+    "fcg_set_plan_budget", "fcg_plan_budget", "fcg_plan_create_pair_screened",
 ]
 
 
@@ -90,7 +90,9 @@ def load():
     L.fcg_plan_create.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
     L.fcg_plan_create_pair.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(VP)]
     L.fcg_tpsci_pt2.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64), ctypes.c_int,
-                                ctypes.c_int, P(c_f64), P(PT2Stats)]
+                                ctypes.c_int, c_f64, P(c_f64), P(PT2Stats)]
+    L.fcg_plan_create_pair_screened.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, P(c_f64), c_f64, P(VP)]
     L.fcg_plan_create_limited.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, c_i64,
                                           P(c_i64), P(VP)]
     L.fcg_plan_device_bytes.argtypes = [VP, P(c_i64)]
@@ -274,14 +276,25 @@ class Plan(_Handle):
         return b.value
 
     @classmethod
-    def create_pair(cls, ctx, ham, ops, bra, ket, nbody=4, precisions=("fp64",)):
-        """Plan of H between two spaces: sigma(bra) = H c(ket)."""
+    def create_pair(cls, ctx, ham, ops, bra, ket, nbody=4, precisions=("fp64",), screen=None):
+        """Plan of H between two spaces: sigma(bra) = H c(ket).  screen =
+        (c, thresh) applies the open matvec's screening for ket state c."""
         mask = 0
         for p in precisions:
             mask |= 1 << PRECISION[p]
         h = VP()
-        ctx.check(ctx.L.fcg_plan_create_pair(ctx.h, ham.h, ops.h, bra.h, ket.h, nbody, mask,
-                                             ctypes.byref(h)), "fcg_plan_create_pair")
+        if screen is None:
+            ctx.check(ctx.L.fcg_plan_create_pair(ctx.h, ham.h, ops.h, bra.h, ket.h, nbody, mask,
+                                                 ctypes.byref(h)), "fcg_plan_create_pair")
+        else:
+            c, thresh = screen
+            c = np.asarray(c, dtype=np.float64)
+            if c.ndim == 1:
+                c = c[:, None]
+            cc = np.ascontiguousarray(c.T)
+            ctx.check(ctx.L.fcg_plan_create_pair_screened(
+                ctx.h, ham.h, ops.h, bra.h, ket.h, nbody, mask, c.shape[1], _p(cc, c_f64),
+                float(thresh), ctypes.byref(h)), "fcg_plan_create_pair_screened")
         pl = cls(ctx, h, keep=(ham, ops, bra, ket))
         pl.nvar = bra.nvar()
         pl.nvar_ket = ket.nvar()
@@ -346,7 +359,7 @@ class Plan(_Handle):
         return ev, vec.T.copy(), stats
 
 
-def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1):
+def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
     """Second-order energies e2[r] of the TPSCI state (space, c) over its
     first-order space (fcg_tpsci_pt2); returns (e2, stats dict)."""
     c = np.asarray(c, dtype=np.float64)
@@ -358,7 +371,8 @@ def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1):
     e2 = np.zeros(R)
     st = PT2Stats()
     ctx.check(ctx.L.fcg_tpsci_pt2(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64), _p(e0, c_f64),
-                                  nbody, h0_op, _p(e2, c_f64), ctypes.byref(st)), "fcg_tpsci_pt2")
+                                  nbody, h0_op, float(thresh), _p(e2, c_f64), ctypes.byref(st)),
+              "fcg_tpsci_pt2")
     return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}
 
 

@@ -373,13 +373,14 @@ int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisi
 }
 
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
-                    int nbody, int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need) {
+                    int nbody, int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need,
+                    const double *thr) {
   if (!ctx || !h || !o || !sp || !ksp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
   auto t0 = std::chrono::steady_clock::now();
   TRY(fcg_form_boxes(ctx, o, sp));
   if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
-  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box) {
+  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box && !thr) {
     // every box one configuration: join and build on the device
     return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
   }
@@ -397,6 +398,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   }
   fcg_plan *pl = new fcg_plan();
   pl->reserved_bytes = reserved;
+  pl->d_thr = thr;
   pl->ctx = ctx;
   pl->sp = sp;
   pl->ksp = ksp;
@@ -455,6 +457,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   dev_free(pl->d_tasks);
   dev_free(pl->d_pair_t0);
   dev_free(pl->d_pair_col);
+  pl->d_thr = nullptr;
   pl->build_us =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   *out = pl;
@@ -472,6 +475,20 @@ int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int
 int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t bra,
                          fcg_space_t ket, int nbody, int precisions, fcg_plan_t *out) {
   return fcg_create_plan(ctx, h, o, bra, ket, nbody, precisions, out);
+}
+
+int fcg_plan_create_pair_screened(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t bra,
+                                  fcg_space_t ket, int nbody, int precisions, int R,
+                                  const double *c, double thresh, fcg_plan_t *out) {
+  if (!ctx || !ket || !c || R < 1 || !(thresh >= 0.0)) return FCG_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  TRY(fcg_form_boxes(ctx, o, ket));
+  double *thr = nullptr;
+  if (thresh > 0.0) TRY(fcg_screen_thresholds(ctx, ket->nvar, R, c, thresh, &thr));
+  const int rc = fcg_create_plan(ctx, h, o, bra, ket, nbody, precisions, out, INT64_MAX, nullptr,
+                                 thr);
+  if (thr) (void)hipFree(thr);
+  return rc;
 }
 
 int fcg_plan_create_limited(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t sp, int nbody,

@@ -144,6 +144,7 @@ struct fcg_plan {
   void *d_ci = nullptr;           // c interleaved ([n_var][R], FP64 or FP32) for the apply
   int64_t ci_bytes = 0;
   int64_t reserved_bytes = 0;     // held against the process-wide plan budget
+  const double *d_thr = nullptr;  // build only: per-ket-variable screening threshold
 };
 
 // host plan construction (fcg_plan_host.cpp)
@@ -198,14 +199,19 @@ int fcg_plan_reserve(fcg_ctx *ctx, int64_t bytes, int64_t max_bytes, int64_t *ne
 void fcg_plan_release(int64_t bytes);
 constexpr int FCG_SPARSE_MAX_BLOCK = 256;  // configurations per Fock block (device join)
 
+// per-ket-variable screening thresholds on the device (fcg_pt2.hip)
+int fcg_screen_thresholds(fcg_ctx *ctx, int64_t n, int R, const double *c, double thresh,
+                          double **d_thr);
+
 // shared helpers (fcg_api.hip)
 int fcg_space_to_device(fcg_space *sp);
 int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp);
 // max_bytes: refuse (FCG_ENOMEM, nothing allocated, *need set) when the
 // plan's device footprint would exceed it
+// thr (device, [ket n_var] or nullptr): screened build, see BuildArgs::thr
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
-                    int64_t *need = nullptr);
+                    int64_t *need = nullptr, const double *thr = nullptr);
 
 // kernels (fcg_kernels.hip)
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,

@@ -38,6 +38,10 @@ struct BuildArgs {
   int N;
   int64_t pair_base;
   double *K;
+  // screening (nullptr: none): a term's element enters only when its
+  // magnitude exceeds thr[ket variable] (_collect_significant_thread!)
+  const double *thr;
+  const int64_t *kvar;
 };
 
 __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int x, int i) {
@@ -224,7 +228,10 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
           break;
         }
       }
-      if (val != 0.0) acc[k] += val * cur[opidx];
+      if (val != 0.0) {
+        const double x = cur[opidx];
+        if (!a.thr || fabs(x) > a.thr[a.kvar[K] + e / db]) acc[k] += val * x;
+      }
     }
     __syncthreads();
   }
@@ -623,6 +630,8 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.facK = kp->d_factors;
   a.N = sp->N;
   a.K = pl->d_K64;
+  a.thr = pl->d_thr;
+  a.kvar = kp->d_var;
   const int64_t chunk = 1 << 20;
   for (int64_t base = 0; base < pl->n_pair; base += chunk) {
     a.pair_base = base;

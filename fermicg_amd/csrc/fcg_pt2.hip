@@ -4,9 +4,13 @@
 // Follows the reference's compute_pt2_energy (src/tpsci_pt2_energy.jl:10-175):
 //   * target Fock blocks are F_ket + TransferConfig over the state's Fock
 //     blocks, kept when every cluster sector exists (:49-79);
-//   * sigma on the target blocks (the open matvec, _pt2_job :178-266, here
-//     over every configuration of each target block: without the reference's
-//     screening, i.e. its thresh_foi -> 0 limit);
+//   * sigma on the target blocks (the open matvec, _pt2_job :178-266) over
+//     every configuration of each target block, with the reference's
+//     screening: a term's contribution from ket configuration j enters only
+//     when its magnitude exceeds thresh_foi / max_r |c[j, r]|
+//     (_collect_significant_thread!, src/tpsci_matvec_thread.jl:830-895; the
+//     reference's prescreen bounds only skip contributions that fail this
+//     test);
 //   * configurations of the variational space are projected out
 //     (project_out!, src/tpsci_outer.jl:869-889);
 //   * denominators from the Hcmf diagonal (compute_diagonal! with "Hcmf",
@@ -95,11 +99,31 @@ struct DevBuf {
 
 }  // namespace
 
+// thresholds of the reference's _collect_significant_thread!
+// (src/tpsci_matvec_thread.jl:830-895): thresh / max_r |c[j, r]| per ket
+// variable (infinite when the coefficients vanish: nothing is collected)
+int fcg_screen_thresholds(fcg_ctx *ctx, int64_t n, int R, const double *c, double thresh,
+                          double **d_thr) {
+  std::vector<double> thr(std::max<int64_t>(n, 1));
+  for (int64_t j = 0; j < n; j++) {
+    double m = 0.0;
+    for (int r = 0; r < R; r++) m = std::max(m, std::fabs(c[(int64_t)r * n + j]));
+    thr[j] = m > 0.0 ? thresh / m : INFINITY;
+  }
+  if (hipMalloc((void **)d_thr, sizeof(double) * thr.size()) != hipSuccess) {
+    *d_thr = nullptr;
+    ctx->err = "hipMalloc failed (screening thresholds)";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemcpy(*d_thr, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice), ctx);
+  return FCG_OK;
+}
+
 extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var, int R,
-                             const double *c, const double *e0, int nbody, int h0_op, double *e2,
-                             fcg_pt2_stats *st) {
+                             const double *c, const double *e0, int nbody, int h0_op,
+                             double thresh, double *e2, fcg_pt2_stats *st) {
   if (!ctx || !h || !o || !var || !c || !e0 || !e2 || R < 1 || R > FCG_MAX_R || nbody < 1 ||
-      nbody > 4 || h0_op < 0 || h0_op >= o->n_ops)
+      nbody > 4 || h0_op < 0 || h0_op >= o->n_ops || !(thresh >= 0.0))
     return FCG_EINVAL;
   if (!var->tpsci) {
     ctx->err = "fcg_tpsci_pt2 needs a TPSCI space";
@@ -120,8 +144,13 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
   const std::vector<int64_t> &hoff = tg.hoff;
   const std::vector<int32_t> &hstride = tg.hstride;
   // sigma on the first-order space
+  DevBuf<double> dthr;
+  if (thresh > 0.0)
+    if (int rc = fcg_screen_thresholds(ctx, var->nvar, R, c, thresh, &dthr.p)) return rc;
   fcg_plan *pl = nullptr;
-  if (int rc = fcg_create_plan(ctx, h, o, tsp, var, nbody, 1 << FCG_FP64, &pl)) return rc;
+  if (int rc = fcg_create_plan(ctx, h, o, tsp, var, nbody, 1 << FCG_FP64, &pl, INT64_MAX,
+                               nullptr, dthr.p))
+    return rc;
   std::unique_ptr<fcg_plan, void (*)(fcg_plan *)> pguard(pl, fcg_plan_destroy);
   DevBuf<double> dc, ds, dhd, de0, dpart;
   DevBuf<uint8_t> din;

@@ -156,10 +156,12 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     roots), E0 = <r|H0|r> with H0 the clusters' local `H0` operators unless
     given, and e2[r] = sum_x sigma[x,r]^2 / (E0[r] - H0_diag[x]) over every
     configuration of the Fock blocks F_ket + TransferConfig that is not in
-    the variational space.  sigma over those blocks runs on the GPU without
-    screening, so the result is the reference's thresh_foi -> 0 limit;
-    `thresh_foi` and `prescreen` are accepted for call compatibility.
-    Returns e2 (length R)."""
+    the variational space.  sigma over those blocks runs on the GPU with the
+    reference's screening at `thresh_foi` (a term's contribution from ket
+    configuration j enters only when its magnitude exceeds
+    thresh_foi / max_r |c[j, r]|); `prescreen` is accepted for call
+    compatibility: the reference's bounds only skip contributions that fail
+    that test.  Returns e2 (length R)."""
     from .cluster_ops import OP_NAMES
     eng = engine or default_engine()
     ci = _copy_state(ci_vector)
@@ -170,7 +172,8 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
     space = _lib.DeviceSpace.tpsci(eng.ctx, ci)
     e2, stats = _lib.tpsci_pt2(eng.ctx, eng.ham(clustered_ham, nbody), eng.ops(cluster_ops), space,
-                               ci.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0))
+                               ci.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0),
+                               thresh=thresh_foi)
     if verbose:
         evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
         print(" first-order space: %d Fock blocks, %d configurations; %.3f s"

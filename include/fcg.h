@@ -130,6 +130,15 @@ int fcg_plan_device_bytes(fcg_plan_t plan, int64_t *bytes);
  * serving loop, which admits concurrent solves by HBM footprint. */
 int fcg_set_plan_budget(int64_t bytes);
 int fcg_plan_budget(int64_t *budget, int64_t *held);
+/* The same with the reference's screening of the open matvec
+ * (contract_matvec_thread / _collect_significant_thread!,
+ * src/tpsci_matvec_thread.jl:279-895): a term's element from ket variable j
+ * enters only when its magnitude exceeds thresh / max_r |c[j, r]|, c being
+ * the ket state (host, n_ket x R) the open sigma is for.  thresh = 0 keeps
+ * every element. */
+int fcg_plan_create_pair_screened(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t bra,
+                                  fcg_space_t ket, int nbody, int precisions, int R,
+                                  const double *c, double thresh, fcg_plan_t *out);
 void fcg_plan_destroy(fcg_plan_t plan);
 /* Plan statistics: [0] block pairs, [1] (term, block pair) tasks,
  * [2] Hamiltonian-block elements, [3] build flops, [4] build time (us),
@@ -187,8 +196,9 @@ int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *v0,
  * (the reference's H0 string; 1 = "Hcmf", its default).
  * e2[r] = sum_x sigma[x,r]^2 / (e0[r] - H0_diag[x]) over every
  * configuration x of the Fock blocks F_ket + TransferConfig outside the
- * variational space, with sigma = H c: the reference's thresh_foi -> 0
- * limit (no screening). */
+ * variational space, with sigma = H c screened as the reference's open
+ * matvec does at thresh_foi = thresh (see fcg_plan_create_pair_screened;
+ * 0 keeps every contribution). */
 typedef struct {
   int64_t n_target_fock;  /* Fock blocks of the first-order space (jobs) */
   int64_t n_target_var;   /* configurations in those blocks */
@@ -199,8 +209,8 @@ typedef struct {
 } fcg_pt2_stats;
 
 int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
-                  const double *c, const double *e0, int nbody, int h0_op, double *e2,
-                  fcg_pt2_stats *stats);
+                  const double *c, const double *e0, int nbody, int h0_op, double thresh,
+                  double *e2, fcg_pt2_stats *stats);
 
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */

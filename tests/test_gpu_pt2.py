@@ -24,11 +24,12 @@ def eng():
     return Engine(0)
 
 
-def _oracle_pt2(case, nbody=4):
+def _oracle_pt2(case, nbody=4, thresh=0.0):
     st = case.state
     fo = flatten_ops(case.ops, [len(c) for c in case.clusters])
     e0 = _e0(case, st)
-    e2, _ = Oracle(fo, flatten_ham(case.H, nbody), st).pt2(st.coeffs, e0, nbody=nbody)
+    e2, _ = Oracle(fo, flatten_ham(case.H, nbody), st).pt2(st.coeffs, e0, nbody=nbody,
+                                                          thresh=thresh)
     return e0, e2
 
 
@@ -39,14 +40,30 @@ def test_pt2_parity(eng, name, nbody):
             "three_full": lambda: three_clusters_m4(R=1, full_blocks=True)}[name]()
     print("case", name, "nbody", nbody, "n_var", len(case.state), flush=True)
     e0_ref, e2_ref = _oracle_pt2(case, nbody)
-    e2 = compute_pt2_energy(case.state, case.ops, case.H, nbody=nbody, verbose=0, engine=eng)
+    e2 = compute_pt2_energy(case.state, case.ops, case.H, nbody=nbody, thresh_foi=0.0, verbose=0,
+                            engine=eng)
     st = compute_pt2_energy.last_stats
     print("e2", e2, "oracle", e2_ref, "stats", st, flush=True)
     assert np.abs(e2 - e2_ref).max() < 1e-11 * np.abs(e2_ref).max()
     # E0 computed on the GPU (Hcmf-only operator) against the oracle's
-    e2b = compute_pt2_energy(case.state, case.ops, case.H, nbody=nbody, E0=e0_ref, verbose=0,
-                             engine=eng)
+    e2b = compute_pt2_energy(case.state, case.ops, case.H, nbody=nbody, E0=e0_ref, thresh_foi=0.0,
+                             verbose=0, engine=eng)
     assert np.abs(e2b - e2).max() < 1e-11 * np.abs(e2_ref).max()
+
+
+@pytest.mark.parametrize("thresh", [1e-9, 1e-6, 1e-4, 1e-2])
+def test_pt2_screened(eng, thresh):
+    # the reference's thresh_foi screening: GPU against the oracle's
+    # restatement of _collect_significant_thread! at the same threshold
+    case = three_clusters_m4()
+    e0_ref, e2_ref = _oracle_pt2(case, thresh=thresh)
+    _, e2_exact = _oracle_pt2(case)
+    e2 = compute_pt2_energy(case.state, case.ops, case.H, thresh_foi=thresh, verbose=0,
+                            engine=eng)
+    print("thresh", thresh, "gpu", e2, "oracle", e2_ref, "exact", e2_exact, flush=True)
+    assert np.abs(e2 - e2_ref).max() < 1e-11 * np.abs(e2_ref).max()
+    if thresh >= 1e-4:   # the screening changes the answer: the test sees it
+        assert np.abs(e2_ref - e2_exact).max() > 1e-9 * np.abs(e2_exact).max()
 
 
 def test_open_sigma_pair_plan(eng):
