@@ -245,7 +245,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src, "bytes_algorithmic": bytes_alg,
-                     "kernel": "k_apply_f32 (block-sparse H c)" if prec != "fp64" else "k_apply_f64"},
+                     "kernel": "k_apply + k_apply_rows<%s> (block-sparse H c)" % prec},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
