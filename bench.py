@@ -125,7 +125,8 @@ def main():
         pl.sigma_device(c_dev.data_ptr(), s_dev.data_ptr(), R, prec)
         k_ms = pl.last_sigma_ms()
         rates[prec] = 1e3 / ms_wall
-        kernel_ms[prec] = {"step_ms": ms_wall, "stream_ms_per_step": ms_dev, "apply_kernel_ms": k_ms}
+        kernel_ms[prec] = {"step_ms": ms_wall, "stream_ms_per_step": ms_dev, "apply_kernel_ms": k_ms,
+                           "apply_variant": pl.apply_variant(prec, R)}
         log(prec, kernel_ms[prec])
 
     # parity on sampled bra blocks against the CPU oracle (and CPU baseline timing)

@@ -28,6 +28,8 @@ EXPORTS = [
     "fcg_last_sigma_time", "fcg_event_record", "fcg_event_elapsed", "fcg_synchronize",
     "fcg_plan_create_pair", "fcg_tpsci_pt2", "fcg_plan_create_limited", "fcg_plan_device_bytes",
     "fcg_set_plan_budget", "fcg_plan_budget", "fcg_plan_create_pair_screened",
+    "fcg_plan_apply_variant",
+    "fcg_plan_set_apply_variant",
 ]
 
 
@@ -97,6 +99,10 @@ def load():
                                           P(c_i64), P(VP)]
     L.fcg_plan_device_bytes.argtypes = [VP, P(c_i64)]
     L.fcg_set_plan_budget.argtypes = [c_i64]
+    L.fcg_plan_apply_variant.argtypes = [VP, ctypes.c_int, ctypes.c_int, P(ctypes.c_int),
+                                         P(ctypes.c_int)]
+    L.fcg_plan_set_apply_variant.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int]
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
@@ -268,6 +274,21 @@ class Plan(_Handle):
         pl = cls(ctx, h, keep=(ham, ops, space))
         pl.nvar = pl.nvar_ket = space.nvar()
         return pl, need.value
+
+    def apply_variant(self, precision, R):
+        """(multi-row, one-row) apply variants chosen for (precision, R); -1
+        before the first sigma."""
+        m, r = ctypes.c_int(), ctypes.c_int()
+        self.ctx.check(self.ctx.L.fcg_plan_apply_variant(self.h, PRECISION[precision], R,
+                                                         ctypes.byref(m), ctypes.byref(r)),
+                       "fcg_plan_apply_variant")
+        return m.value, r.value
+
+    def set_apply_variant(self, precision, R, multi, rows):
+        """Pin the apply variants for (precision, R) instead of timing them."""
+        self.ctx.check(self.ctx.L.fcg_plan_set_apply_variant(self.h, PRECISION[precision], R,
+                                                             multi, rows),
+                       "fcg_plan_set_apply_variant")
 
     def device_bytes(self):
         b = c_i64()

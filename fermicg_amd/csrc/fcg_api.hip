@@ -550,6 +550,73 @@ int fcg_plan_stats(fcg_plan_t pl, double *o6) {
   return FCG_OK;
 }
 
+int fcg_plan_apply_variant(fcg_plan_t pl, int precision, int R, int *multi, int *rows) {
+  if (!pl || precision < FCG_FP64 || precision > FCG_BF16X2 || R < 1 || R > FCG_MAX_R)
+    return FCG_EINVAL;
+  if (multi) *multi = pl->tuned[precision][R] ? pl->tuned_vm[precision][R] : -1;
+  if (rows) *rows = pl->tuned[precision][R] ? pl->tuned_vr[precision][R] : -1;
+  return FCG_OK;
+}
+
+int fcg_plan_set_apply_variant(fcg_plan_t pl, int precision, int R, int multi, int rows) {
+  if (!pl || precision < FCG_FP64 || precision > FCG_BF16X2 || R < 1 || R > FCG_MAX_R)
+    return FCG_EINVAL;
+  int nm = 1, nr = 1, max_r = 0;
+  fcg_apply_n_variants(&nm, &nr, &max_r);
+  if (multi < 0 || multi >= nm || rows < 0 || rows >= nr) return FCG_EINVAL;
+  // above the tuned root count each kernel has one compiled form
+  if (R > max_r && (multi != fcg_apply_default_variant(precision) || rows != 0))
+    return FCG_EINVAL;
+  pl->tuned_vm[precision][R] = (int8_t)multi;
+  pl->tuned_vr[precision][R] = (int8_t)rows;
+  pl->tuned[precision][R] = true;
+  return FCG_OK;
+}
+
+}  // extern "C"
+
+// Autotuning of the apply: on the first sigma of a plan for a precision and
+// root count (at most the tuned maximum), each variant of each apply kernel
+// runs twice on the caller's operands and the faster second run wins.  The
+// variants compute identical results, so the outputs of the timing runs are
+// simply overwritten.
+static int tune_apply(fcg_ctx *ctx, fcg_plan *pl, int R, const double *c, double *s,
+                      int precision) {
+  int nm = 1, nr = 1, max_r = 0;
+  fcg_apply_n_variants(&nm, &nr, &max_r);
+  int best_m = fcg_apply_default_variant(precision), best_r = 0;
+  if (R <= max_r) {
+    auto time_it = [&](int vm, int vr, int parts, float *ms) -> int {
+      for (int rep = 0; rep < 2; rep++) {
+        FCG_HIP(fcg_launch_apply(pl, R, c, s, precision, ctx->stream, ctx->ev0, vm, vr, parts),
+                ctx);
+        FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
+        FCG_HIP(hipEventSynchronize(ctx->ev1), ctx);
+        FCG_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1), ctx);
+      }
+      return FCG_OK;
+    };
+    float best = 1e30f, t = 0.f;
+    if (pl->n_multi > 0)
+      for (int vm = 0; vm < nm; vm++) {
+        TRY(time_it(vm, 0, 1, &t));
+        if (t < best) { best = t; best_m = vm; }
+      }
+    best = 1e30f;
+    if (pl->n_single_slots > 0)
+      for (int vr = 0; vr < nr; vr++) {
+        TRY(time_it(best_m, vr, 2, &t));
+        if (t < best) { best = t; best_r = vr; }
+      }
+  }
+  pl->tuned_vm[precision][R] = (int8_t)best_m;
+  pl->tuned_vr[precision][R] = (int8_t)best_r;
+  pl->tuned[precision][R] = true;
+  return FCG_OK;
+}
+
+extern "C" {
+
 int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, double *s_dev,
                      int precision) {
   if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c_dev || !s_dev) return FCG_EINVAL;
@@ -568,7 +635,10 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
     pl->d_ci = p;
     pl->ci_bytes = n * 8;
   }
-  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0), ctx);
+  if (!pl->tuned[precision][R]) TRY(tune_apply(ctx, pl, R, c_dev, s_dev, precision));
+  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0,
+                           pl->tuned_vm[precision][R], pl->tuned_vr[precision][R]),
+          ctx);
   FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
   ctx->last_launches = 2;
   return FCG_OK;

@@ -145,6 +145,9 @@ struct fcg_plan {
   int64_t ci_bytes = 0;
   int64_t reserved_bytes = 0;     // held against the process-wide plan budget
   const double *d_thr = nullptr;  // build only: per-ket-variable screening threshold
+  // apply variants chosen by timing on the first sigma of each precision and R
+  bool tuned[3][FCG_MAX_R + 1] = {};
+  int8_t tuned_vm[3][FCG_MAX_R + 1] = {}, tuned_vr[3][FCG_MAX_R + 1] = {};
 };
 
 // host plan construction (fcg_plan_host.cpp)
@@ -217,6 +220,11 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
                             hipStream_t s);
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
+// vm / vr: apply variants for multi-row and one-row boxes; parts: bit 0
+// multi-row boxes, bit 1 one-row boxes
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
-                            int precision, hipStream_t s, hipEvent_t before_apply);
+                            int precision, hipStream_t s, hipEvent_t before_apply, int vm, int vr,
+                            int parts = 3);
+int fcg_apply_default_variant(int precision);
+int fcg_apply_n_variants(int *n_multi, int *n_rows, int *max_r);
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s);

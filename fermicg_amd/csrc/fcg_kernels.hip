@@ -252,34 +252,23 @@ __device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
 
-// Quad loads in flight per lane (32-bit and 64-bit blocks), and wavefronts
-// (bra boxes) per workgroup.
-#ifndef FCG_APPLY_UNROLL32
-#define FCG_APPLY_UNROLL32 16
-#endif
-#ifndef FCG_APPLY_UNROLL64
-#define FCG_APPLY_UNROLL64 16
-#endif
-#ifndef FCG_APPLY_WPB32
-#define FCG_APPLY_WPB32 4
-#endif
-#ifndef FCG_APPLY_WPB64
-#define FCG_APPLY_WPB64 1
-#endif
-#ifndef FCG_APPLY_CLAMP32
-#define FCG_APPLY_CLAMP32 1
-#endif
-#ifndef FCG_APPLY_CLAMP64
-#define FCG_APPLY_CLAMP64 0
-#endif
+// Apply variants, picked per plan, precision and root count by timing each
+// once on the plan's first sigma (fcg_sigma_device).  Every variant reads the
+// same elements in the same order per lane and reduces in the same order, so
+// the result does not depend on the choice.
+//   multi-row boxes (k_apply): wavefronts per workgroup, quads in flight;
+//     32-bit blocks load branch-free from a clamped quad, 64-bit ones
+//     predicated (measured faster each way);
+//   one-row boxes (k_apply_rows): quads in flight.
+struct MultiVariant { int wpb, unroll; };
+constexpr MultiVariant kMulti[] = {{4, 16}, {1, 16}, {4, 8}};
+constexpr int kRowsUnroll[] = {8, 4};
+constexpr int N_MULTI_VARIANTS = 3, N_ROWS_VARIANTS = 2;
+constexpr int TUNED_MAX_R = 4;  // larger root counts use the default variants
 template <typename TK>
-constexpr int apply_unroll() { return sizeof(TK) == 8 ? FCG_APPLY_UNROLL64 : FCG_APPLY_UNROLL32; }
+constexpr int default_multi() { return sizeof(TK) == 8 ? 1 : 0; }
 template <typename TK>
-constexpr int apply_wpb() { return sizeof(TK) == 8 ? FCG_APPLY_WPB64 : FCG_APPLY_WPB32; }
-// clamped unconditional quad loads (measured faster for 32-bit blocks,
-// slower for 64-bit ones) or predicated loads
-template <typename TK>
-constexpr bool apply_clamp() { return sizeof(TK) == 8 ? FCG_APPLY_CLAMP64 : FCG_APPLY_CLAMP32; }
+constexpr bool apply_clamp() { return sizeof(TK) != 8; }
 
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
@@ -380,8 +369,8 @@ constexpr int stage_quads() {
 // read back with vector LDS loads broadcast across the lanes of a quad.
 // Slices are summed in a fixed order through LDS, so results are
 // reproducible bit for bit.
-template <typename TK, typename TA, typename TC, int R>
-__global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
+template <typename TK, typename TA, typename TC, int R, int WPB, int UNR, bool CLAMP>
+__global__ __launch_bounds__(64 * WPB) void k_apply(int64_t n_box,
                                                           const int32_t *__restrict__ order,
                                                           const int64_t *__restrict__ var,
                                                const int64_t *__restrict__ row_koff,
@@ -390,8 +379,8 @@ __global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
                                                const TK *__restrict__ K, const TC *__restrict__ c,
                                                int64_t nvar_s, double *__restrict__ sigma) {
   static_assert(sizeof(TA) == sizeof(TC), "stage buffer doubles as reduction buffer");
-  constexpr int APPLY_UNROLL = apply_unroll<TK>();
-  constexpr int APPLY_WPB = apply_wpb<TK>();
+  constexpr int APPLY_UNROLL = UNR;
+  constexpr int APPLY_WPB = WPB;
   constexpr int CG = stage_quads<TC, R>();
   static_assert(CG * FCG_QUAD >= 64, "reduction needs 64*R staged entries");
   __shared__ __align__(16) TC stage[APPLY_WPB][CG * FCG_QUAD * R];
@@ -444,7 +433,7 @@ __global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
       const TK *Kc = Kr + (int64_t)FCG_QUAD * db * base;
       for (int k0 = 0; k0 < nk; k0 += APPLY_UNROLL) {
         RawQuad<TK> kv[APPLY_UNROLL];
-        if constexpr (apply_clamp<TK>()) {
+        if constexpr (CLAMP) {
           // unconditional loads from a clamped (valid) quad, then a select:
           // no branch per load
           const int jmax = max(lim - 1, 0);
@@ -500,13 +489,12 @@ __global__ __launch_bounds__(64 * apply_wpb<TK>()) void k_apply(int64_t n_box,
 // quads' blocks and column indices and gathering the vector entries itself,
 // APPLY_UNROLL quads in flight.  Rows come from `order` (XCD-grouped slots,
 // -1 = empty); the 64 lane sums are combined by a fixed butterfly.
-template <typename TK, typename TA, typename TC, int R>
+template <typename TK, typename TA, typename TC, int R, int U>
 __global__ __launch_bounds__(64 * FCG_CSR_WPB) void k_apply_rows(
     int64_t n_slots, const int32_t *__restrict__ order, const int64_t *__restrict__ var,
     const int64_t *__restrict__ row_koff, const int64_t *__restrict__ col_ptr,
     const int32_t *__restrict__ colidx, const TK *__restrict__ K, const TC *__restrict__ c,
     int64_t nvar_s, double *__restrict__ sigma) {
-  constexpr int U = apply_unroll<TK>() / 2;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * FCG_CSR_WPB + wave;
   const int32_t b = w < n_slots ? order[w] : -1;
@@ -650,59 +638,91 @@ hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename TK, typename TA, typename TC, int R, int V>
+void launch_multi_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma, hipStream_t s) {
+  constexpr int W = kMulti[V].wpb, U = kMulti[V].unroll;
+  hipLaunchKernelGGL((k_apply<TK, TA, TC, R, W, U, apply_clamp<TK>()>),
+                     dim3((unsigned)((pl->n_multi + W - 1) / W)), dim3(64 * W), 0, s, pl->n_multi,
+                     pl->d_order, pl->sp->d_var, pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, K, c,
+                     pl->sp->nvar, sigma);
+}
+
+template <typename TK, typename TA, typename TC, int R, int V>
+void launch_rows_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma, hipStream_t s) {
+  hipLaunchKernelGGL((k_apply_rows<TK, TA, TC, R, kRowsUnroll[V]>),
+                     dim3((unsigned)(pl->n_single_slots / FCG_CSR_WPB)), dim3(64 * FCG_CSR_WPB), 0,
+                     s, pl->n_single_slots, pl->d_order + pl->n_multi, pl->sp->d_var,
+                     pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, K, c, pl->sp->nvar, sigma);
+}
+
+// parts: bit 0 multi-row boxes, bit 1 one-row boxes
+template <typename TK, typename TA, typename TC, int R>
+void launch_r(const fcg_plan *pl, int vm, int vr, int parts, const TK *K, const TC *c,
+              double *sigma, hipStream_t s) {
+  if ((parts & 1) && pl->n_multi > 0) {
+    if constexpr (R <= TUNED_MAX_R) {
+      if (vm == 0) launch_multi_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
+      else if (vm == 1) launch_multi_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
+      else launch_multi_v<TK, TA, TC, R, 2>(pl, K, c, sigma, s);
+    } else {
+      launch_multi_v<TK, TA, TC, R, default_multi<TK>()>(pl, K, c, sigma, s);
+    }
+  }
+  if ((parts & 2) && pl->n_single_slots > 0) {
+    if constexpr (R <= TUNED_MAX_R) {
+      if (vr == 0) launch_rows_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
+      else launch_rows_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
+    } else {
+      launch_rows_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
+    }
+  }
+}
+
+template <typename TK, typename TA, typename TC>
+hipError_t launch_by_r(const fcg_plan *pl, int R, int vm, int vr, int parts, const TK *K,
+                       const TC *c, double *sigma, hipStream_t s) {
+  switch (R) {
+#define CASE(RR) \
+  case RR: launch_r<TK, TA, TC, RR>(pl, vm, vr, parts, K, c, sigma, s); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+    CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int fcg_apply_default_variant(int precision) {
+  return precision == FCG_FP64 ? default_multi<double>() : default_multi<float>();
+}
+int fcg_apply_n_variants(int *n_multi, int *n_rows, int *max_r) {
+  *n_multi = N_MULTI_VARIANTS;
+  *n_rows = N_ROWS_VARIANTS;
+  *max_r = TUNED_MAX_R;
+  return 0;
+}
+
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
-                            int precision, hipStream_t s, hipEvent_t before_apply) {
+                            int precision, hipStream_t s, hipEvent_t before_apply, int vm, int vr,
+                            int parts) {
   const fcg_space *sp = pl->sp;
   if (sp->n_box == 0) {
     if (before_apply) (void)hipEventRecord(before_apply, s);
     return hipSuccess;
-  }
-#define LAUNCH(TK, TA, TC, KP, CP, RR)                                                        \
-  case RR:                                                                                    \
-    if (pl->n_multi > 0)                                                                      \
-      hipLaunchKernelGGL((k_apply<TK, TA, TC, RR>),                                           \
-                         dim3((unsigned)((pl->n_multi + apply_wpb<TK>() - 1) /                \
-                                         apply_wpb<TK>())),                                   \
-                         dim3(64 * apply_wpb<TK>()), 0, s, pl->n_multi, pl->d_order,          \
-                         sp->d_var, pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, KP, CP,      \
-                         sp->nvar, sigma);                                                    \
-    if (pl->n_single_slots > 0)                                                               \
-      hipLaunchKernelGGL((k_apply_rows<TK, TA, TC, RR>),                                      \
-                         dim3((unsigned)(pl->n_single_slots / FCG_CSR_WPB)),                  \
-                         dim3(64 * FCG_CSR_WPB), 0, s, pl->n_single_slots,                    \
-                         pl->d_order + pl->n_multi, sp->d_var, pl->d_row_koff,                \
-                         pl->d_col_ptr, pl->d_colidx, KP, CP, sp->nvar, sigma);               \
-    break;
-#define BY_R(TK, TA, TC, KP, CP)                                                              \
-  switch (R) {                                                                                \
-    LAUNCH(TK, TA, TC, KP, CP, 1) LAUNCH(TK, TA, TC, KP, CP, 2) LAUNCH(TK, TA, TC, KP, CP, 3)   \
-    LAUNCH(TK, TA, TC, KP, CP, 4) LAUNCH(TK, TA, TC, KP, CP, 5) LAUNCH(TK, TA, TC, KP, CP, 6)   \
-    LAUNCH(TK, TA, TC, KP, CP, 7) LAUNCH(TK, TA, TC, KP, CP, 8) LAUNCH(TK, TA, TC, KP, CP, 9)   \
-    LAUNCH(TK, TA, TC, KP, CP, 10) LAUNCH(TK, TA, TC, KP, CP, 11)                             \
-    LAUNCH(TK, TA, TC, KP, CP, 12) LAUNCH(TK, TA, TC, KP, CP, 13)                             \
-    LAUNCH(TK, TA, TC, KP, CP, 14) LAUNCH(TK, TA, TC, KP, CP, 15)                             \
-    LAUNCH(TK, TA, TC, KP, CP, 16)                                                            \
-    default: return hipErrorInvalidValue;                                                     \
   }
   const int64_t nk = pl->ksp->nvar;
   if (precision == FCG_FP64) {
     double *ci = static_cast<double *>(pl->d_ci);
     hipLaunchKernelGGL(k_interleave<double>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
     if (before_apply) (void)hipEventRecord(before_apply, s);
-    BY_R(double, double, double, pl->d_K64, ci);
-    return hipGetLastError();
+    return launch_by_r<double, double, double>(pl, R, vm, vr, parts, pl->d_K64, ci, sigma, s);
   }
   float *ci = static_cast<float *>(pl->d_ci);
   hipLaunchKernelGGL(k_interleave<float>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
   if (before_apply) (void)hipEventRecord(before_apply, s);
-  if (precision == FCG_FP32) {
-    BY_R(float, float, float, pl->d_K32, ci);
-  } else {
-    BY_R(uint32_t, float, float, pl->d_Kbf, ci);
-  }
-#undef BY_R
-#undef LAUNCH
-  return hipGetLastError();
+  if (precision == FCG_FP32)
+    return launch_by_r<float, float, float>(pl, R, vm, vr, parts, pl->d_K32, ci, sigma, s);
+  return launch_by_r<uint32_t, float, float>(pl, R, vm, vr, parts, pl->d_Kbf, ci, sigma, s);
 }
 
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s) {

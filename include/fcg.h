@@ -212,6 +212,20 @@ int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, 
                   const double *c, const double *e0, int nbody, int h0_op, double thresh,
                   double *e2, fcg_pt2_stats *stats);
 
+/* Apply variants the plan chose for a precision and root count (-1 before
+ * its first sigma there).  The first sigma of each (precision, R) times each
+ * variant of the apply kernels (workgroup shape, loads in flight) on the
+ * caller's operands and keeps the fastest; all variants give identical
+ * results.  No counterpart in the reference (its tuning is BLAS's). */
+int fcg_plan_apply_variant(fcg_plan_t plan, int precision, int R, int *multi, int *rows);
+
+/* Pin the apply variants for a precision and root count instead of timing
+ * them (multi in [0, 3), rows in [0, 2) for R <= 4; above R = 4 only the
+ * precision's default multi-row variant (1 for FP64, 0 otherwise) and rows
+ * 0 exist).  FCG_EINVAL for any other pair.  No counterpart in the
+ * reference. */
+int fcg_plan_set_apply_variant(fcg_plan_t plan, int precision, int R, int multi, int rows);
+
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */
 int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches);
