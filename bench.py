@@ -190,7 +190,7 @@ def main():
         for prec in ("fp64", args.precision if args.precision != "fp64" else "fp32"):
             ta = time.perf_counter()
             e, v, stt = pl.davidson(st.coeffs, conv_thresh=1e-8, max_iter=300, max_ss_vecs=12,
-                                    lindep_thresh=1e-12, precision=prec, refresh_k=8)
+                                    lindep_thresh=1e-12, precision=prec, refresh_k=0)
             dt = time.perf_counter() - ta
             solves[prec] = {"seconds": dt, "energies": [float(x) for x in e], **stt}
             log("solve", prec, solves[prec])

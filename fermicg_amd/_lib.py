@@ -359,7 +359,7 @@ class Plan(_Handle):
         return out
 
     def davidson(self, v0, conv_thresh=1e-8, max_iter=200, max_ss_vecs=12, lindep_thresh=1e-12,
-                 precision="fp64", refresh_k=4, precond=True):
+                 precision="fp64", refresh_k=0, precond=True):
         v0 = np.asarray(v0, dtype=np.float64)
         if v0.ndim == 1:
             v0 = v0[:, None]

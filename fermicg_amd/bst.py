@@ -242,7 +242,7 @@ def build_sigma(sigma_vector, ci_vector, cluster_ops, clustered_ham, nbody=4, ca
 
 def ci_solve(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, max_ss_vecs=12,
              max_iter=40, lindep_thresh=1e-10, precond=True, nbody=4, precision="fp64",
-             refresh_k=4, engine=None):
+             refresh_k=0, engine=None):
     """Davidson over the concatenated cores (reference ci_solve).  Returns
     (energies, BSTstate with the eigenvectors)."""
     pl = cache_hamiltonian(ci_vector, ci_vector, cluster_ops, clustered_ham, nbody=nbody,

@@ -266,7 +266,9 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
   // at most FCG_MAX_R columns); the subspace is capped at 256 vectors
   const int KR = std::min(2 * R, FCG_MAX_R);
   const int mmax = std::max(KR + R, std::min(256, o->max_ss_vecs * R));
-  const int refresh = std::max(1, o->refresh_k);
+  // refresh period; 0 = only at restarts and when the residual is below the
+  // threshold
+  const int refresh = o->refresh_k > 0 ? o->refresh_k : INT32_MAX;
   auto alloc = [&](double **p, int64_t cnt) {
     return hipMalloc((void **)p, sizeof(double) * std::max<int64_t>(cnt, 1)) == hipSuccess;
   };

@@ -104,7 +104,7 @@ def build_inputs(req):
 
 class SolveServer:
     def __init__(self, slots=4, precision="fp64", producers=8, hbm_budget=200e9,
-                 conv_thresh=1e-7, max_iter=300, refresh_k=8, device=0):
+                 conv_thresh=1e-7, max_iter=300, refresh_k=0, device=0):
         self.slots = slots
         self.precision = precision
         self.producers = producers

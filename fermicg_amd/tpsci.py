@@ -110,7 +110,7 @@ def compute_diagonal(ci_vector, cluster_ops, clustered_ham, engine=None):
 
 
 def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lindep_thresh=1e-12,
-                    max_ss_vecs=12, max_iter=40, precond=True, precision="fp64", refresh_k=4,
+                    max_ss_vecs=12, max_iter=40, precond=True, precision="fp64", refresh_k=0,
                     engine=None):
     """Lowest R eigenpairs in the space of `ci_vector`, starting from its
     coefficients.  Returns (energies, TPSCIstate with the eigenvectors)."""

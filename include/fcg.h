@@ -162,8 +162,9 @@ int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t plan, double *hd);
  * whose eigensolver is the external BlockDavidson package).  precision
  * FCG_FP64 runs every sigma in FP64; FCG_FP32 / FCG_BF16X2 run the
  * expansion sigmas in that precision and recompute the Ritz vectors'
- * sigma and residual in FP64 every refresh_k iterations, converging on the
- * FP64 residual.  v0: host (n_var x R) start vectors (orthonormalised
+ * sigma and residual in FP64 at every restart, when the reduced-precision
+ * residual falls below conv_thresh, and every refresh_k iterations if
+ * refresh_k > 0; convergence is decided on the FP64 residual.  v0: host (n_var x R) start vectors (orthonormalised
  * inside); evals (R), evecs (n_var x R) host outputs. */
 typedef struct {
   double conv_thresh;   /* residual norm threshold (reference conv_thresh) */
@@ -171,7 +172,7 @@ typedef struct {
   int max_ss_vecs;      /* subspace vectors per root before restart */
   double lindep_thresh; /* drop expansion vectors below this norm */
   int precision;        /* FCG_FP64 / FCG_FP32 / FCG_BF16X2 */
-  int refresh_k;        /* FP64 refresh period for mixed precision */
+  int refresh_k;        /* extra FP64 refresh period (0: restarts only) */
   int precond;          /* 1: diagonal (Davidson) preconditioner */
 } fcg_davidson_opts;
 
