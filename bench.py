@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve workload: arrivals per s")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
+    ap.add_argument("--order", default="size", choices=("size", "fifo"),
+                    help="serve workload: which ready request a free slot takes")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
@@ -260,7 +262,7 @@ def serve(args):
     reqs = request_stream(args.requests, rate=args.rate, seed=0)
     # producer processes start before anything in this process touches the GPU
     srv = SolveServer(slots=args.slots, producers=min(16, os.cpu_count() or 1), hbm_budget=200e9,
-                      conv_thresh=1e-7).start()
+                      conv_thresh=1e-7, order=args.order).start()
     runs, res_all = {}, {}
     for prec in ("fp64", "fp32"):
         srv.set_precision(prec)
@@ -283,13 +285,14 @@ def serve(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded integrals per request, Poisson arrivals)",
         "config": {"workload": "serve_mix_poisson%d" % int(args.rate), "requests": args.requests,
-                   "rate_per_s": args.rate, "slots": args.slots, "spaces": kinds,
+                   "rate_per_s": args.rate, "slots": args.slots, "order": args.order, "spaces": kinds,
                    "parallelism": "single GPU"},
         "serve": runs, "abs_dE_mixed_vs_fp64": dE,
         "latency_s": {p: {"p50": runs[p]["p50_latency_s"], "p99": runs[p]["p99_latency_s"]}
                       for p in runs},
-        "requests": [{k: r[k] for k in ("rid", "n_clusters", "M", "R", "kind", "n_var",
-                                        "latency", "plan_s", "solve_s", "footprint", "iterations")}
+        "requests": [{k: r[k] for k in ("rid", "n_clusters", "M", "R", "kind", "n_var", "arrival",
+                                        "ready", "start", "latency", "plan_s", "solve_s",
+                                        "footprint", "iterations")}
                      for r in res_all["fp32"]],
     }
     print(json.dumps(out))

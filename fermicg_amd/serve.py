@@ -13,8 +13,11 @@ Continuous batching:
   * producer processes (spawned, CPU only) build each request's inputs as it
     arrives: local eigenbases, ClusterOps, ClusteredTerms, space;
   * `slots` solver threads, each with its own library context (HIP stream),
-    take ready requests in arrival order, so solves of different sizes run
-    concurrently on the device; the library releases the GIL in every call;
+    take ready requests, so solves of different sizes run concurrently on
+    the device; the library releases the GIL in every call.  Slot 0 takes the
+    largest ready request (n_var x R), the others the smallest: small
+    requests do not wait behind large ones, and the largest start early
+    enough not to stretch the span (order="fifo" takes them by readiness);
   * admission by HBM footprint: the library holds a process-wide budget for
     live plans (fcg_set_plan_budget); a plan reserves its device bytes
     before allocating anything and is refused when they do not fit beside
@@ -29,7 +32,6 @@ the mix.
 """
 import concurrent.futures as cf
 import multiprocessing as mp
-import queue
 import threading
 import time
 from dataclasses import dataclass
@@ -104,7 +106,10 @@ def build_inputs(req):
 
 class SolveServer:
     def __init__(self, slots=4, precision="fp64", producers=8, hbm_budget=200e9,
-                 conv_thresh=1e-7, max_iter=300, refresh_k=0, device=0):
+                 conv_thresh=1e-7, max_iter=300, refresh_k=0, device=0, order="size"):
+        if order not in ("size", "fifo"):
+            raise ValueError("order must be 'size' or 'fifo'")
+        self.order = order
         self.slots = slots
         self.precision = precision
         self.producers = producers
@@ -150,7 +155,7 @@ class SolveServer:
     def run(self, requests, log=None):
         """Serve `requests` (arrival times in seconds from now); returns the
         per-request results in request order."""
-        ready = queue.Queue()
+        ready = _ReadyPool(self.order)
         results = {}
         mem = threading.Condition()
         state = {"in_use": 0.0, "peak": 0.0, "running": 0}
@@ -179,7 +184,7 @@ class SolveServer:
         def slot(k):
             ctx = _lib.Context(self.device)
             while True:
-                item = ready.get()
+                item = ready.get(largest=(k == 0))
                 if item is None:
                     break
                 req, inp, t_ready = item
@@ -267,6 +272,45 @@ class SolveServer:
 
 def _noop(i):
     return i
+
+
+class _ReadyPool:
+    """Requests whose inputs are built.  get() blocks until one is there and
+    returns the smallest (or, with largest=True, the largest) by n_var x R,
+    ties by arrival; with order="fifo" always the first that became ready.
+    None (one per slot, put after every request) ends a slot."""
+
+    def __init__(self, order):
+        self.order = order
+        self.items = []
+        self.stops = 0
+        self.cv = threading.Condition()
+        self.seq = 0
+
+    def put(self, item):
+        with self.cv:
+            if item is None:
+                self.stops += 1
+            else:
+                req, inp, _ = item
+                self.items.append((inp["n_var"] * req.R, req.arrival, self.seq, item))
+                self.seq += 1
+            self.cv.notify_all()
+
+    def get(self, largest=False):
+        with self.cv:
+            while not self.items and not self.stops:
+                self.cv.wait()
+            if not self.items:
+                self.stops -= 1
+                return None
+            if self.order == "fifo":
+                i = min(range(len(self.items)), key=lambda j: self.items[j][2])
+            elif largest:
+                i = max(range(len(self.items)), key=lambda j: (self.items[j][0], -self.items[j][1]))
+            else:
+                i = min(range(len(self.items)), key=lambda j: self.items[j][:2])
+            return self.items.pop(i)[3]
 
 
 def summarize(results):

@@ -8,7 +8,9 @@ same request twice gives identical arrays.
 """
 import numpy as np
 
-from fermicg_amd.serve import FULL_CAP, Request, build_inputs, request_stream, summarize
+import threading
+
+from fermicg_amd.serve import FULL_CAP, Request, _ReadyPool, build_inputs, request_stream, summarize
 
 
 def test_request_stream_seeded_poisson():
@@ -46,3 +48,32 @@ def test_summarize():
     assert s["requests"] == 2 and s["converged"] == 2
     assert abs(s["span_s"] - 4.0) < 1e-12 and abs(s["solves_per_s"] - 0.5) < 1e-12
     assert s["p50_latency_s"] == 2.0
+
+
+def test_ready_pool_order():
+    # smallest n_var x R first (ties by arrival), slot 0 the largest; fifo by
+    # readiness; one None per slot ends the slots after the requests
+    def item(rid, n_var, R, arrival):
+        return (Request(rid, 0, 2, 4, R, arrival), {"n_var": n_var}, 0.0)
+    items = [item(0, 100, 2, 0.0), item(1, 50, 1, 0.1), item(2, 50, 1, 0.05),
+             item(3, 1000, 4, 0.2), item(4, 10, 1, 0.3)]
+    pool = _ReadyPool("size")
+    for it in items:
+        pool.put(it)
+    pool.put(None)
+    pool.put(None)
+    got = [pool.get(largest=True)[0].rid, pool.get()[0].rid, pool.get()[0].rid,
+           pool.get()[0].rid, pool.get(largest=True)[0].rid]
+    assert got == [3, 4, 2, 1, 0]
+    assert pool.get() is None and pool.get(largest=True) is None
+    fifo = _ReadyPool("fifo")
+    for it in items:
+        fifo.put(it)
+    assert [fifo.get(largest=k % 2 == 0)[0].rid for k in range(5)] == [0, 1, 2, 3, 4]
+    # a waiting slot wakes on put
+    out = []
+    t = threading.Thread(target=lambda: out.append(fifo.get()))
+    t.start()
+    fifo.put(items[4])
+    t.join(5)
+    assert out and out[0][0].rid == 4
