@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve workload: arrivals per s")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
-    ap.add_argument("--order", default="size", choices=("size", "fifo"),
+    ap.add_argument("--order", default="fifo", choices=("fifo", "size"),
                     help="serve workload: which ready request a free slot takes")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -14,10 +14,10 @@ Continuous batching:
     arrives: local eigenbases, ClusterOps, ClusteredTerms, space;
   * `slots` solver threads, each with its own library context (HIP stream),
     take ready requests, so solves of different sizes run concurrently on
-    the device; the library releases the GIL in every call.  Slot 0 takes the
-    largest ready request (n_var x R), the others the smallest: small
-    requests do not wait behind large ones, and the largest start early
-    enough not to stretch the span (order="fifo" takes them by readiness);
+    the device; the library releases the GIL in every call.  A free slot
+    takes the request that became ready first (order="fifo"), or with
+    order="size" slot 0 takes the largest ready request (n_var x R) and the
+    others the smallest (measured no better on the Poisson mix);
   * admission by HBM footprint: the library holds a process-wide budget for
     live plans (fcg_set_plan_budget); a plan reserves its device bytes
     before allocating anything and is refused when they do not fit beside
@@ -106,7 +106,7 @@ def build_inputs(req):
 
 class SolveServer:
     def __init__(self, slots=4, precision="fp64", producers=8, hbm_budget=200e9,
-                 conv_thresh=1e-7, max_iter=300, refresh_k=0, device=0, order="size"):
+                 conv_thresh=1e-7, max_iter=300, refresh_k=0, device=0, order="fifo"):
         if order not in ("size", "fifo"):
             raise ValueError("order must be 'size' or 'fifo'")
         self.order = order
