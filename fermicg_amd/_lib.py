@@ -30,6 +30,7 @@ EXPORTS = [
     "fcg_set_plan_budget", "fcg_plan_budget", "fcg_plan_create_pair_screened",
     "fcg_plan_apply_variant",
     "fcg_plan_set_apply_variant",
+    "fcg_apply_variant_counts",
 ]
 
 
@@ -104,6 +105,7 @@ def load():
     L.fcg_plan_set_apply_variant.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int]
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
+    L.fcg_apply_variant_counts.argtypes = [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -409,3 +411,12 @@ def plan_budget():
     b, h = c_i64(), c_i64()
     load().fcg_plan_budget(ctypes.byref(b), ctypes.byref(h))
     return b.value, h.value
+
+
+def apply_variant_counts():
+    """(multi-row variants, one-row variants, largest root count with all of them)."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = load().fcg_apply_variant_counts(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    if rc:
+        raise FCGError("fcg_apply_variant_counts: %s" % ERRORS.get(rc, rc))
+    return a.value, b.value, c.value

@@ -558,6 +558,12 @@ int fcg_plan_apply_variant(fcg_plan_t pl, int precision, int R, int *multi, int 
   return FCG_OK;
 }
 
+int fcg_apply_variant_counts(int *n_multi, int *n_rows, int *max_r) {
+  if (!n_multi || !n_rows || !max_r) return FCG_EINVAL;
+  fcg_apply_n_variants(n_multi, n_rows, max_r);
+  return FCG_OK;
+}
+
 int fcg_plan_set_apply_variant(fcg_plan_t pl, int precision, int R, int multi, int rows) {
   if (!pl || precision < FCG_FP64 || precision > FCG_BF16X2 || R < 1 || R > FCG_MAX_R)
     return FCG_EINVAL;

@@ -221,11 +221,15 @@ int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, 
 int fcg_plan_apply_variant(fcg_plan_t plan, int precision, int R, int *multi, int *rows);
 
 /* Pin the apply variants for a precision and root count instead of timing
- * them (multi in [0, 3), rows in [0, 2) for R <= 4; above R = 4 only the
- * precision's default multi-row variant (1 for FP64, 0 otherwise) and rows
- * 0 exist).  FCG_EINVAL for any other pair.  No counterpart in the
- * reference. */
+ * them (multi in [0, n_multi), rows in [0, n_rows) of
+ * fcg_apply_variant_counts for R <= max_r; above max_r only the precision's
+ * default multi-row variant (1 for FP64, 0 otherwise) and rows 0 exist).
+ * FCG_EINVAL for any other pair.  No counterpart in the reference. */
 int fcg_plan_set_apply_variant(fcg_plan_t plan, int precision, int R, int multi, int rows);
+
+/* Number of multi-row and one-row apply variants, and the largest root
+ * count that has all of them.  No counterpart in the reference. */
+int fcg_apply_variant_counts(int *n_multi, int *n_rows, int *max_r);
 
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */

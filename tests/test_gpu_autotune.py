@@ -3,10 +3,12 @@ bit for bit, as the one it picked (the variants change the workgroup shape
 and the loads in flight, never the order of a sum).
 
 Cases: a full-block TPSCI space (multi-row boxes), a thinned one (one-row
-boxes on the sparse plan) and a space with both, at FP64, FP32 and BF16X2,
-R = 2 and R = 6 (above the tuned root count, where only the default exists
-and pinning anything else is refused).  The FP64 result is also checked
-against the oracle at 1e-12 relative.
+boxes on the sparse plan), a space with both, and a rank-1 space of 39,313
+rows (enough one-row slot groups for the persistent-grid variants to loop),
+at FP64, FP32 and BF16X2, R = 2 and R = 6 (above the tuned root count,
+where only the default exists and pinning anything else is refused).  The
+FP64 result of the small cases is also checked against the oracle at 1e-12
+relative.
 """
 import numpy as np
 import pytest
@@ -15,7 +17,7 @@ from fermicg_amd import _lib
 from fermicg_amd.marshal import flatten_ham, flatten_ops
 from fermicg_amd.state import TPSCIstate
 from fermicg_amd.tpsci import Engine, cluster_sizes_of
-from fermicg_amd.workloads import tpsci_small
+from fermicg_amd.workloads import build_problem, rank1_space, tpsci_small
 from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -29,6 +31,11 @@ def eng():
 
 
 def _space(kind, R):
+    if kind == "rank1":
+        p = build_problem(24, 6, 16, 48, 7, max_transfer=2)
+        st = rank1_space(p, R=R, frac=0.3, seed=7)
+        st.set_vector(np.random.Generator(np.random.PCG64(1)).normal(size=(len(st), R)))
+        return p, st
     p, st = tpsci_small(seed=0, R=R)
     if kind == "full":
         return p, st
@@ -47,23 +54,26 @@ def _space(kind, R):
     return p, sp
 
 
-@pytest.mark.parametrize("kind", ["full", "thinned", "mixed"])
+@pytest.mark.parametrize("kind", ["full", "thinned", "mixed", "rank1"])
 @pytest.mark.parametrize("R", [2, 6])
 def test_variants_identical(eng, kind, R):
     p, st = _space(kind, R)
     c = st.coeffs
     print("case", kind, "R", R, "rows", len(st), flush=True)
-    ref = Oracle(flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
-                 flatten_ham(p.clustered_ham), st).sigma(c)
+    ref = None if kind == "rank1" else Oracle(
+        flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
+        flatten_ham(p.clustered_ham), st).sigma(c)
     pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, precisions=PRECS)
     for prec in PRECS:
         tuned = pl.sigma(c, precision=prec)
         vm, vr = pl.apply_variant(prec, R)
         print(prec, "tuned variant", (vm, vr), flush=True)
         assert vm >= 0 and vr >= 0
-        if prec == "fp64":
+        if prec == "fp64" and ref is not None:
             assert np.abs(tuned - ref).max() < 1e-12 * np.abs(ref).max()
-        pairs = [(m, r) for m in range(3) for r in range(2)] if R <= 4 else [(vm, vr)]
+        nm, nr, max_r = _lib.apply_variant_counts()
+        assert (nm, nr, max_r) == (3, 2, 4)
+        pairs = [(m, r) for m in range(nm) for r in range(nr)] if R <= max_r else [(vm, vr)]
         for m, r in pairs:
             pl.set_apply_variant(prec, R, m, r)
             assert pl.apply_variant(prec, R) == (m, r)
@@ -73,4 +83,6 @@ def test_variants_identical(eng, kind, R):
             with pytest.raises(_lib.FCGError):
                 pl.set_apply_variant(prec, R, (vm + 1) % 3, 0)
         with pytest.raises(_lib.FCGError):
-            pl.set_apply_variant(prec, R, 3, 0)
+            pl.set_apply_variant(prec, R, nm, 0)
+        with pytest.raises(_lib.FCGError):
+            pl.set_apply_variant(prec, R, 0, nr)
