@@ -369,6 +369,7 @@ int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisi
   int64_t per = 8;
   if (precisions & (1 << FCG_FP32)) per += 4;
   if (precisions & (1 << FCG_BF16X2)) per += 4;
+  if (precisions & (1 << FCG_BF16)) per += 2;
   return n_kel * per + n_cols * 4 + n_box * 48;
 }
 
@@ -434,6 +435,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
   if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
   if (pl->precisions & (1 << FCG_BF16X2)) PTRY(dev_alloc(ctx, &pl->d_Kbf, pl->n_kel));
+  if (pl->precisions & (1 << FCG_BF16)) PTRY(dev_alloc(ctx, &pl->d_Kb16, pl->n_kel));
   if (pl->n_kel > 0) {
     // zero columns pad each row panel to whole quads
     const hipError_t e = hipMemsetAsync(pl->d_K64, 0, sizeof(double) * pl->n_kel, ctx->stream);
@@ -535,6 +537,7 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   dev_free(pl->d_K64);
   dev_free(pl->d_K32);
   dev_free(pl->d_Kbf);
+  dev_free(pl->d_Kb16);
   if (pl->d_ci) (void)hipFree(pl->d_ci);
   delete pl;
 }
@@ -551,7 +554,7 @@ int fcg_plan_stats(fcg_plan_t pl, double *o6) {
 }
 
 int fcg_plan_apply_variant(fcg_plan_t pl, int precision, int R, int *multi, int *rows) {
-  if (!pl || precision < FCG_FP64 || precision > FCG_BF16X2 || R < 1 || R > FCG_MAX_R)
+  if (!pl || precision < FCG_FP64 || precision >= FCG_N_PREC || R < 1 || R > FCG_MAX_R)
     return FCG_EINVAL;
   if (multi) *multi = pl->tuned[precision][R] ? pl->tuned_vm[precision][R] : -1;
   if (rows) *rows = pl->tuned[precision][R] ? pl->tuned_vr[precision][R] : -1;
@@ -565,7 +568,7 @@ int fcg_apply_variant_counts(int *n_multi, int *n_rows, int *max_r) {
 }
 
 int fcg_plan_set_apply_variant(fcg_plan_t pl, int precision, int R, int multi, int rows) {
-  if (!pl || precision < FCG_FP64 || precision > FCG_BF16X2 || R < 1 || R > FCG_MAX_R)
+  if (!pl || precision < FCG_FP64 || precision >= FCG_N_PREC || R < 1 || R > FCG_MAX_R)
     return FCG_EINVAL;
   int nm = 1, nr = 1, max_r = 0;
   fcg_apply_n_variants(&nm, &nr, &max_r);
@@ -626,7 +629,7 @@ extern "C" {
 int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, double *s_dev,
                      int precision) {
   if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c_dev || !s_dev) return FCG_EINVAL;
-  if (precision < FCG_FP64 || precision > FCG_BF16X2 || !(pl->precisions & (1 << precision))) {
+  if (precision < FCG_FP64 || precision >= FCG_N_PREC || !(pl->precisions & (1 << precision))) {
     ctx->err = "precision not built into this plan";
     return FCG_EINVAL;
   }

@@ -6,11 +6,11 @@
 // max_ss_vecs per root, lindep_thresh, diagonal preconditioner).
 //
 // Restarts are thick: the subspace collapses onto its lowest min(2R, 16) Ritz
-// vectors.  Mixed precision: expansion vectors get their sigma in FP32 or
-// split-BF16; every refresh_k iterations, and before convergence is
-// accepted, the subspace collapses onto those Ritz vectors and their sigma
-// is recomputed in FP64 (one launch), so the residual that decides
-// convergence is always an FP64 one.
+// vectors.  Mixed precision: expansion vectors get their sigma in FP32,
+// split-BF16 or BF16; at every restart, before convergence is accepted and
+// (optionally) every refresh_k iterations, the subspace collapses onto those
+// Ritz vectors and their sigma is recomputed in FP64 (one launch), so the
+// residual that decides convergence is always an FP64 one.
 // Orthogonalisation (block Gram-Schmidt against the subspace, twice, then
 // symmetric (Loewdin) orthonormalisation of the new block) and the
 // Rayleigh-Ritz projection are FP64 on the GPU with this file's own
@@ -257,6 +257,10 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
   d.n = n;
   d.nchunk = (int)((n + GCHUNK - 1) / GCHUNK);
   const int low = o->precision;
+  if (low < FCG_FP64 || low >= FCG_N_PREC) {
+    ctx->err = "unknown sigma precision";
+    return FCG_EINVAL;
+  }
   const bool mixed = low != FCG_FP64;
   if (mixed && !(pl->precisions & (1 << low))) {
     ctx->err = "plan lacks the requested sigma precision";

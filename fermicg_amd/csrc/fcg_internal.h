@@ -18,6 +18,7 @@ constexpr int FCG_LDS_X = 4096;       // chain intermediates (doubles)
 constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
+constexpr int FCG_N_PREC = 4;  // FCG_FP64 .. FCG_BF16
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
 constexpr int FCG_CSR_WPB = 4;        // one-row boxes: wavefronts (rows) per workgroup
 constexpr int FCG_XCDS = 8;           // MI355X accelerator complex dies (workgroups round-robin)
@@ -141,13 +142,14 @@ struct fcg_plan {
   double *d_K64 = nullptr;
   float *d_K32 = nullptr;
   uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
+  uint16_t *d_Kb16 = nullptr;     // bf16 (the hi part alone)
   void *d_ci = nullptr;           // c interleaved ([n_var][R], FP64 or FP32) for the apply
   int64_t ci_bytes = 0;
   int64_t reserved_bytes = 0;     // held against the process-wide plan budget
   const double *d_thr = nullptr;  // build only: per-ket-variable screening threshold
   // apply variants chosen by timing on the first sigma of each precision and R
-  bool tuned[3][FCG_MAX_R + 1] = {};
-  int8_t tuned_vm[3][FCG_MAX_R + 1] = {}, tuned_vr[3][FCG_MAX_R + 1] = {};
+  bool tuned[FCG_N_PREC][FCG_MAX_R + 1] = {};
+  int8_t tuned_vm[FCG_N_PREC][FCG_MAX_R + 1] = {}, tuned_vr[FCG_N_PREC][FCG_MAX_R + 1] = {};
 };
 
 // host plan construction (fcg_plan_host.cpp)

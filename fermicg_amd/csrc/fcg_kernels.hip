@@ -251,6 +251,9 @@ __device__ __forceinline__ float kval(float v) { return v; }
 __device__ __forceinline__ float kval(uint32_t u) {  // hi bf16 | lo bf16
   return __uint_as_float(u & 0xffff0000u) + __uint_as_float(u << 16);
 }
+__device__ __forceinline__ float kval(uint16_t u) {  // bf16
+  return __uint_as_float((uint32_t)u << 16);
+}
 
 // Apply variants, picked per plan, precision and root count by timing each
 // once on the plan's first sigma (fcg_sigma_device).  Every variant reads the
@@ -337,6 +340,15 @@ template <> struct RawQuad<uint32_t> {
   __device__ void zero() { v = make_uint4(0u, 0u, 0u, 0u); }
   __device__ void get(float (&k)[4]) const {
     k[0] = kval(v.x); k[1] = kval(v.y); k[2] = kval(v.z); k[3] = kval(v.w);
+  }
+};
+template <> struct RawQuad<uint16_t> {  // four bf16 in 8 bytes
+  uint2 v;
+  __device__ void load(const uint16_t *p) { v = *reinterpret_cast<const uint2 *>(p); }
+  __device__ void zero() { v = make_uint2(0u, 0u); }
+  __device__ void get(float (&k)[4]) const {
+    k[0] = __uint_as_float(v.x << 16); k[1] = __uint_as_float(v.x & 0xffff0000u);
+    k[2] = __uint_as_float(v.y << 16); k[3] = __uint_as_float(v.y & 0xffff0000u);
   }
 };
 template <> struct RawQuad<double> {
@@ -556,7 +568,7 @@ __device__ __forceinline__ uint32_t bf16_rne_bits(float f) {
 }
 
 __global__ void k_convert(int64_t n, const double *__restrict__ K64, float *__restrict__ K32,
-                          uint32_t *__restrict__ Kbf) {
+                          uint32_t *__restrict__ Kbf, uint16_t *__restrict__ Kb16) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
     const float f = (float)K64[e];
@@ -567,6 +579,7 @@ __global__ void k_convert(int64_t n, const double *__restrict__ K64, float *__re
       const uint32_t lo = bf16_rne_bits(r);
       Kbf[e] = hi | (lo >> 16);
     }
+    if (Kb16) Kb16[e] = (uint16_t)(bf16_rne_bits(f) >> 16);
   }
 }
 
@@ -634,9 +647,9 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
 }
 
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
-  if (!pl->d_K32 && !pl->d_Kbf) return hipSuccess;
+  if (!pl->d_K32 && !pl->d_Kbf && !pl->d_Kb16) return hipSuccess;
   hipLaunchKernelGGL(k_convert, dim3(4096), dim3(256), 0, s, pl->n_kel, pl->d_K64, pl->d_K32,
-                     pl->d_Kbf);
+                     pl->d_Kbf, pl->d_Kb16);
   return hipGetLastError();
 }
 
@@ -724,6 +737,8 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
   if (before_apply) (void)hipEventRecord(before_apply, s);
   if (precision == FCG_FP32)
     return launch_by_r<float, float, float>(pl, R, vm, vr, parts, pl->d_K32, ci, sigma, s);
+  if (precision == FCG_BF16)
+    return launch_by_r<uint16_t, float, float>(pl, R, vm, vr, parts, pl->d_Kb16, ci, sigma, s);
   return launch_by_r<uint32_t, float, float>(pl, R, vm, vr, parts, pl->d_Kbf, ci, sigma, s);
 }
 

@@ -802,6 +802,8 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
     return fail(FCG_ENOMEM, "device allocation");
   if ((pl->precisions & (1 << FCG_BF16X2)) && !alloc(&pl->d_Kbf, n_cols))
     return fail(FCG_ENOMEM, "device allocation");
+  if ((pl->precisions & (1 << FCG_BF16)) && !alloc(&pl->d_Kb16, n_cols))
+    return fail(FCG_ENOMEM, "device allocation");
   hipStream_t s = ctx->stream;
   if (hipMemcpyAsync(pl->d_row_koff, col_ptr.data(), sizeof(int64_t) * n_row,
                      hipMemcpyHostToDevice, s) != hipSuccess ||

@@ -69,7 +69,7 @@ def test_ci_solve_vs_dense(eng):
                     engine=eng)
     print("ci_solve", e - w, v.davidson_stats, flush=True)
     assert np.abs(e - w).max() < 1e-10
-    for prec in ("fp32", "bf16x2"):
+    for prec in ("fp32", "bf16x2", "bf16"):
         e2, v2 = ci_solve(bst, p.cluster_ops, p.clustered_ham, conv_thresh=1e-8, max_iter=400,
                           precision=prec, engine=eng)
         print(prec, e2 - w, v2.davidson_stats, flush=True)
