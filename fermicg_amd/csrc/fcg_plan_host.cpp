@@ -13,6 +13,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -355,28 +356,28 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     }
     std::vector<LocalTask>().swap(v);
   }
-  std::stable_sort(pairs.begin(), pairs.end(), [](const PairRec &a, const PairRec &b) {
+  auto pair_less = [](const PairRec &a, const PairRec &b) {
     return a.B != b.B ? a.B < b.B : a.K < b.K;
-  });
+  };
+  if (!std::is_sorted(pairs.begin(), pairs.end(), pair_less))  // boxes grouped by Fock block
+    std::stable_sort(pairs.begin(), pairs.end(), pair_less);
   // tasks contiguous in pair order; per-task chain order; limits; flop count
   const int64_t np = (int64_t)pairs.size();
   out->tasks.clear();
-  out->tasks.reserve(tasks.size());
-  out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.resize(np + 1);
+  out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.assign(np + 1, 0);
   out->pair_koff.resize(np);
   out->pair_col.resize(np);
   out->row_ptr.assign(bra->n_box + 1, 0);
   out->diag_koff.assign(bra->n_box, -1);
   out->diag_col.assign(bra->n_box, 0);
   std::vector<int64_t> row_cols(bra->n_box, 0);
-  double flops = 0.0;
+  // serial pass: pair layout (panel columns run along each bra row)
   for (int64_t p = 0; p < np; p++) {
     const PairRec &r = pairs[p];
     out->pair_B[p] = r.B;
     out->pair_K[p] = r.K;
-    out->pair_t0[p] = (int64_t)out->tasks.size();
-    for (int64_t t = r.t0; t < r.t1; t++) out->tasks.push_back(tasks[t]);
-    const int64_t db = dim_b(r.B), dk = dim_k(r.K);
+    out->pair_t0[p + 1] = out->pair_t0[p] + (r.t1 - r.t0);
+    const int64_t dk = dim_k(r.K);
     if (row_cols[r.B] + dk > INT32_MAX / FCG_MAX_BOX) {
       *err = "row panel too wide";
       return FCG_EOVERFLOW;
@@ -385,6 +386,26 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     if (ket == bra && r.B == r.K) out->diag_col[r.B] = (int32_t)row_cols[r.B];
     row_cols[r.B] += dk;
     out->row_ptr[r.B + 1]++;
+  }
+  out->tasks.resize(tasks.size());
+  // parallel pass: tasks in pair order, their chain order, LDS limits, flops
+  double flops = 0.0;
+  std::atomic<int> lim_err{FCG_OK};
+  const char *lim_msg = nullptr;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : flops)
+  for (int64_t p = 0; p < np; p++) {
+    if (lim_err) continue;
+    const PairRec &r = pairs[p];
+    auto fail = [&](const char *m) {
+#pragma omp critical
+      {
+        if (!lim_err) {
+          lim_err = FCG_EOVERFLOW;
+          lim_msg = m;
+        }
+      }
+    };
+    const int64_t db = dim_b(r.B), dk = dim_k(r.K);
     const int64_t B = r.B, K = r.K;
     int64_t ssum = 0;
     for (int c = 0; c < N; c++)
@@ -392,14 +413,16 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
           !(bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0))
         ssum += (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
     if (ssum > FCG_LDS_S) {
-      *err = "inactive-cluster overlaps exceed LDS budget";
-      return FCG_EOVERFLOW;
+      fail("inactive-cluster overlaps exceed LDS budget");
+      continue;
     }
     for (int64_t t = r.t0; t < r.t1; t++) {
       TaskDev &d = out->tasks[out->pair_t0[p] + (t - r.t0)];
+      d = tasks[t];
       const int tm = d.term, nb = h->nbody[tm];
       int64_t np_[4], rr_[4];
       int64_t rest0 = 1;
+      const char *lds_err = nullptr;
       for (int a = 0; a < nb; a++) {
         const int c = h->clusters[4 * tm + a];
         np_[a] = (nb == 1) ? 1 : h->dims[4 * tm + a];
@@ -408,16 +431,17 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         const int64_t nI = bra->hi[B * N + c] - bra->lo[B * N + c] + 1;
         const int64_t nJ = ket->hi[K * N + c] - ket->lo[K * N + c] + 1;
         const int64_t rk = ket->rank[K * N + c];
-        if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) {
-          *err = "Γ slice exceeds LDS budget";
-          return FCG_EOVERFLOW;
-        }
+        if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) lds_err = "Γ slice exceeds LDS budget";
         const bool ident = bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0;
         if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * bra->rank[B * N + c] * rk);
       }
+      if (lds_err) {
+        fail(lds_err);
+        break;
+      }
       if (rest0 > FCG_LDS_X) {
-        *err = "term integrals exceed LDS budget";
-        return FCG_EOVERFLOW;
+        fail("term integrals exceed LDS budget");
+        break;
       }
       // chain order: the permutation with the smallest peak intermediate
       int perm[4] = {0, 1, 2, 3}, best[4] = {0, 1, 2, 3};
@@ -439,14 +463,17 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         }
       } while (std::next_permutation(perm, perm + nb));
       if (best_peak > FCG_LDS_X) {
-        *err = "chain intermediate exceeds LDS budget";
-        return FCG_EOVERFLOW;
+        fail("chain intermediate exceeds LDS budget");
+        break;
       }
       for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? best[s2] : 0) << (8 + 4 * s2);
       flops += best_fl + 2.0 * db * dk;
     }
   }
-  out->pair_t0[np] = (int64_t)out->tasks.size();
+  if (lim_err.load()) {
+    *err = lim_msg;
+    return lim_err.load();
+  }
   for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
   // row panels: the blocks of one bra row side by side form a db x ncol
   // panel (ncol padded to a multiple of FCG_QUAD with zero columns), stored
@@ -467,6 +494,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   out->col_ptr[bra->n_box] = ncol;
   for (int64_t p = 0; p < np; p++) out->pair_koff[p] = out->row_koff[out->pair_B[p]];
   out->colidx.assign(ncol, 0);
+#pragma omp parallel for schedule(dynamic, 64)
   for (int64_t b = 0; b < bra->n_box; b++) {
     int64_t at = out->col_ptr[b];
     for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) {
