@@ -393,7 +393,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
     // the pair and task tables live on the device during the build
     const int64_t bytes = fcg_plan_bytes(hp.n_kel, (int64_t)hp.colidx.size(), sp->n_box,
                                          precisions | (1 << FCG_FP64)) +
-                          (int64_t)hp.pair_B.size() * 36 + (int64_t)hp.tasks.size() * sizeof(TaskDev);
+                          (int64_t)hp.pair_B.size() * 44 + (int64_t)hp.tasks.size() * sizeof(TaskDev);
     if (int rc2 = fcg_plan_reserve(ctx, bytes, max_bytes, need)) return rc2;
     reserved = bytes;
   }
@@ -424,6 +424,8 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_pair_col, hp.pair_col.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
+  PTRY(dev_upload(ctx, &pl->d_pair_list, hp.pair_list.data(), pl->n_pair));
+  pl->n_small = hp.n_small;
   PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
   PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
@@ -457,6 +459,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   }
   // device copies of the task bookkeeping are only needed for the build
   dev_free(pl->d_tasks);
+  dev_free(pl->d_pair_list);
   dev_free(pl->d_pair_t0);
   dev_free(pl->d_pair_col);
   pl->d_thr = nullptr;
@@ -528,6 +531,7 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   dev_free(pl->d_pair_koff);
   dev_free(pl->d_pair_col);
   dev_free(pl->d_tasks);
+  dev_free(pl->d_pair_list);
   dev_free(pl->d_diag_koff);
   dev_free(pl->d_diag_col);
   dev_free(pl->d_row_koff);

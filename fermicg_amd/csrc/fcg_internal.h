@@ -17,6 +17,9 @@ constexpr int FCG_MAX_BOX = 64;       // rows/cols of one Hamiltonian block
 constexpr int FCG_LDS_X = 4096;       // chain intermediates (doubles)
 constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
+// a smaller LDS footprint for the block pairs that fit it (several
+// workgroups per CU instead of one)
+constexpr int FCG_LDS_X_SMALL = 1024, FCG_LDS_T_SMALL = 512, FCG_LDS_S_SMALL = 256;
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
 constexpr int FCG_N_PREC = 4;  // FCG_FP64 .. FCG_BF16
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
@@ -128,6 +131,8 @@ struct fcg_plan {
   int64_t *d_pair_koff = nullptr; // [n_pair] panel of the pair's bra box
   int32_t *d_pair_col = nullptr;  // [n_pair] first panel column of the pair
   TaskDev *d_tasks = nullptr;
+  int64_t *d_pair_list = nullptr;  // build only: pair indices, small-LDS pairs first
+  int64_t n_small = 0;
   int64_t *d_diag_koff = nullptr; // [n_box] panel holding the (B,B) block or -1
   int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
   int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
@@ -159,6 +164,10 @@ struct HostPlan {
   std::vector<int32_t> pair_col, diag_col, colidx, order;
   int64_t n_multi = 0, n_single_slots = 0;
   std::vector<TaskDev> tasks;
+  // pair indices for the block build: pairs within the small LDS limits
+  // first (n_small of them), then the rest, each in pair order
+  std::vector<int64_t> pair_list;
+  int64_t n_small = 0;
   int64_t n_kel = 0;
   double flops = 0.0;
 };

@@ -36,6 +36,7 @@ struct BuildArgs {
   const int64_t *foffB, *foffK;
   const double *facB, *facK;
   int N;
+  const int64_t *pair_list;  // pair index of workgroup pair_base + blockIdx.x
   int64_t pair_base;
   double *K;
   // screening (nullptr: none): a term's element enters only when its
@@ -51,11 +52,15 @@ __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int
 constexpr int BT = 256;                       // threads of the build kernel
 constexpr int KPT = FCG_MAX_BOX * FCG_MAX_BOX / BT;  // block elements per thread
 
-__global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
-  __shared__ double sX[2][FCG_LDS_X];
-  __shared__ double sG[FCG_LDS_X];
-  __shared__ double sT[FCG_LDS_T];
-  __shared__ double sS[FCG_LDS_S];
+// LX / LT / LS: LDS capacities (doubles) of the chain intermediates and Γ
+// slices, the factor-compression rows and the overlaps; the host puts each
+// pair in the smallest class that holds it
+template <int LX, int LT, int LS>
+__global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
+  __shared__ double sX[2][LX];
+  __shared__ double sG[LX];
+  __shared__ double sT[LT];
+  __shared__ double sS[LS];
   __shared__ int sLoB[FCG_MAXN], sLoK[FCG_MAXN], sNB[FCG_MAXN], sNK[FCG_MAXN];
   __shared__ int sRB[FCG_MAXN], sRK[FCG_MAXN], sKind[FCG_MAXN], sSoff[FCG_MAXN];
   __shared__ int64_t sFB[FCG_MAXN], sFK[FCG_MAXN];
@@ -63,7 +68,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
   __shared__ int sStride[FCG_MAXN];
 
   const int tid = threadIdx.x;
-  const int64_t p = a.pair_base + blockIdx.x;
+  const int64_t p = a.pair_list[a.pair_base + blockIdx.x];
   const int64_t B = a.pair_B[p], K = a.pair_K[p];
   const int N = a.N;
   if (tid < N) {
@@ -161,7 +166,7 @@ __global__ __launch_bounds__(BT) void k_build_blocks(BuildArgs a) {
         }
         __syncthreads();
       } else {
-        int Pc = FCG_LDS_T / (nI * rk);
+        int Pc = LT / (nI * rk);
         if (Pc > np) Pc = np;
         for (int p0 = 0; p0 < np; p0 += Pc) {
           const int pc = min(Pc, np - p0);
@@ -635,13 +640,22 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.K = pl->d_K64;
   a.thr = pl->d_thr;
   a.kvar = kp->d_var;
+  a.pair_list = pl->d_pair_list;
   const int64_t chunk = 1 << 20;
-  for (int64_t base = 0; base < pl->n_pair; base += chunk) {
+  // pairs [0, n_small) of the list fit the small LDS class, the rest the full one
+  for (int64_t base = 0; base < pl->n_pair;) {
     a.pair_base = base;
-    const int64_t n = std::min(chunk, pl->n_pair - base);
-    hipLaunchKernelGGL(k_build_blocks, dim3((unsigned)n), dim3(BT), 0, s, a);
+    const int64_t end = std::min(base + chunk, base < pl->n_small ? pl->n_small : pl->n_pair);
+    const int64_t n = end - base;
+    if (base < pl->n_small)
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL>),
+                         dim3((unsigned)n), dim3(BT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S>), dim3((unsigned)n),
+                         dim3(BT), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    base = end;
   }
   return hipSuccess;
 }

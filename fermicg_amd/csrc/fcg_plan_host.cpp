@@ -392,6 +392,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   double flops = 0.0;
   std::atomic<int> lim_err{FCG_OK};
   const char *lim_msg = nullptr;
+  std::vector<uint8_t> small(np, 0);
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : flops)
   for (int64_t p = 0; p < np; p++) {
     if (lim_err) continue;
@@ -416,6 +417,9 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       fail("inactive-cluster overlaps exceed LDS budget");
       continue;
     }
+    // largest LDS use of the pair's tasks: chain intermediates and staged
+    // Γ slices (X), factor-compression rows (T), overlaps (S)
+    int64_t need_x = 0, need_t = 0;
     for (int64_t t = r.t0; t < r.t1; t++) {
       TaskDev &d = out->tasks[out->pair_t0[p] + (t - r.t0)];
       d = tasks[t];
@@ -432,6 +436,8 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         const int64_t nJ = ket->hi[K * N + c] - ket->lo[K * N + c] + 1;
         const int64_t rk = ket->rank[K * N + c];
         if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) lds_err = "Γ slice exceeds LDS budget";
+        need_x = std::max(need_x, np_[a] * rr_[a]);
+        need_t = std::max(need_t, nI * rk);
         const bool ident = bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0;
         if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * bra->rank[B * N + c] * rk);
       }
@@ -466,14 +472,23 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         fail("chain intermediate exceeds LDS budget");
         break;
       }
+      need_x = std::max(need_x, std::max(best_peak, rest0));
       for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? best[s2] : 0) << (8 + 4 * s2);
       flops += best_fl + 2.0 * db * dk;
     }
+    small[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL;
   }
   if (lim_err.load()) {
     *err = lim_msg;
     return lim_err.load();
   }
+  out->pair_list.resize(np);
+  int64_t ns = 0;
+  for (int64_t p = 0; p < np; p++)
+    if (small[p]) out->pair_list[ns++] = p;
+  out->n_small = ns;
+  for (int64_t p = 0, at = ns; p < np; p++)
+    if (!small[p]) out->pair_list[at++] = p;
   for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
   // row panels: the blocks of one bra row side by side form a db x ncol
   // panel (ncol padded to a multiple of FCG_QUAD with zero columns), stored
