@@ -57,6 +57,11 @@ struct SparseArgs {
   int32_t *row_nnz;                  // count pass
   unsigned long long *stats;         // [0] triples, [1] flops, [2] missing blocks
   const int64_t *row_start;          // fill pass
+  // fill pass: each TransferConfig group's terms in stage-sized chunks
+  // (chunk k of group g ends at term chunk_end[grp_chunk[g] + k]) and each
+  // term's integral offset in the stage (-1 dropped by nbody, -2 too large
+  // for the stage: read from global memory)
+  const int32_t *grp_chunk, *chunk_end, *voff;
   double *K;
   int32_t *colidx;
   int64_t *diag_koff;
@@ -151,10 +156,15 @@ __device__ void term_pair(const SparseArgs &a, const SparseTerm &T, const int16_
     const int c = T.cl[q];
     const int cb = fb[2 * c] * a.S + fb[2 * c + 1], ck = fk[2 * c] * a.S + fk[2 * c + 1];
     const int blk = a.gblock[(((int64_t)c * a.n_ops + T.op[q]) * SS + cb) * SS + ck];
-    if (blk < 0) {
+    out->base[q] = blk;  // block ids first: independent loads
+  }
+  for (int q = 0; q < T.nbody; q++)
+    if (out->base[q] < 0) {
       out->ok = 0;
       return;
     }
+  for (int q = 0; q < T.nbody; q++) {
+    const int blk = (int)out->base[q];
     out->base[q] = a.gb_off[blk];
     out->np[q] = a.gb_dims[3 * blk];
     out->mb[q] = a.gb_dims[3 * blk + 1];
@@ -266,7 +276,6 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
   __shared__ double sV[FILL ? SV : 1];
   __shared__ int32_t sVoff[FILL ? SVT : 1];
   __shared__ TermPair sTP[FILL ? SVT : 1];
-  __shared__ int sChunkEnd;
   const int tid = threadIdx.x;
   const int N = a.N;
   const int64_t fb = a.fb_base + blockIdx.x;
@@ -316,30 +325,13 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
       // the group's terms in chunks that fit the stage (integrals and
       // per-pair Γ block data in LDS); element values accumulate in K over
       // the chunks in term order
-      for (int c0 = t0, first = 1; c0 < t1; first = 0) {
-        if (tid == 0) {
-          int off = 0, t = c0;
-          for (; t < t1 && t - c0 < SVT; t++) {
-            const SparseTerm &T = a.terms[t];
-            int sz = 1;
-            if (T.nbody > 1)
-              for (int k = 0; k < T.nbody; k++) sz *= T.dims[k];
-            else
-              sz = 0;
-            if (T.nbody > a.nbody) sz = 0;
-            if (sz > SV) {  // larger than the stage: integrals read from global memory
-              sVoff[t - c0] = -2;
-              continue;
-            }
-            if (off + sz > SV) break;
-            sVoff[t - c0] = T.nbody > a.nbody ? -1 : off;
-            off += sz;
-          }
-          sChunkEnd = t;
+      for (int c0 = t0, first = 1, kc = a.grp_chunk[g]; c0 < t1; first = 0, kc++) {
+        const int c1 = a.chunk_end[kc];
+        for (int t = c0 + tid; t < c1; t += ST) {
+          sVoff[t - c0] = a.voff[t];
+          term_pair(a, a.terms[t], sFb, sFk, &sTP[t - c0]);
         }
         __syncthreads();
-        const int c1 = sChunkEnd;
-        for (int t = c0 + tid; t < c1; t += ST) term_pair(a, a.terms[t], sFb, sFk, &sTP[t - c0]);
         for (int t = c0; t < c1; t++) {
           const int off = sVoff[t - c0];
           const SparseTerm &T = a.terms[t];
@@ -710,6 +702,7 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   DBuf<int32_t> d_tcs, d_gblock, d_nnz;
   DBuf<int16_t> d_fock;
   DBuf<SparseTerm> d_terms;
+  DBuf<int32_t> d_grp_chunk, d_chunk_end, d_voff;
   DBuf<unsigned long long> d_stats;
   int rc = upload(ctx, d_fstart, fstart.data(), (int64_t)fstart.size());
   if (!rc && !dev_pairs) rc = upload(ctx, d_fp_ptr, fp_ptr.data(), (int64_t)fp_ptr.size());
@@ -719,6 +712,38 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   if (!rc) rc = upload(ctx, d_gblock, o->gblock.data(), (int64_t)o->gblock.size());
   if (!rc) rc = upload(ctx, d_fock, sp->fock.data(), (int64_t)sp->fock.size());
   if (!rc) rc = upload(ctx, d_terms, terms.data(), (int64_t)terms.size());
+  // each group's terms in chunks that fit the fill pass's stage: integrals
+  // of at most SV doubles and at most SVT terms per chunk, in term order
+  std::vector<int32_t> grp_chunk(h->n_tc + 1, 0), chunk_end, voff(terms.size(), 0);
+  for (int g = 0; g < h->n_tc; g++) {
+    grp_chunk[g] = (int32_t)chunk_end.size();
+    const int t1 = h->tc_start[g + 1];
+    for (int c0 = h->tc_start[g]; c0 < t1;) {
+      int off = 0, t = c0;
+      for (; t < t1 && t - c0 < SVT; t++) {
+        const SparseTerm &T = terms[t];
+        int sz = 0;
+        if (T.nbody > 1) {
+          sz = 1;
+          for (int k = 0; k < T.nbody; k++) sz *= T.dims[k];
+        }
+        if (T.nbody > nbody) sz = 0;
+        if (sz > SV) {  // larger than the stage: integrals read from global memory
+          voff[t] = -2;
+          continue;
+        }
+        if (off + sz > SV) break;
+        voff[t] = T.nbody > nbody ? -1 : off;
+        off += sz;
+      }
+      chunk_end.push_back(t);
+      c0 = t;
+    }
+  }
+  grp_chunk[h->n_tc] = (int32_t)chunk_end.size();
+  if (!rc) rc = upload(ctx, d_grp_chunk, grp_chunk.data(), (int64_t)grp_chunk.size());
+  if (!rc) rc = upload(ctx, d_chunk_end, chunk_end.data(), (int64_t)chunk_end.size());
+  if (!rc) rc = upload(ctx, d_voff, voff.data(), (int64_t)voff.size());
   if (rc) return rc;
   if (hipMalloc((void **)&d_nnz.p, sizeof(int32_t) * std::max<int64_t>(n_row, 1)) != hipSuccess ||
       hipMalloc((void **)&d_stats.p, 3 * sizeof(unsigned long long)) != hipSuccess) {
@@ -739,6 +764,9 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   a.fp_g = d_fp_g.p;
   a.tc_start = d_tcs.p;
   a.terms = d_terms.p;
+  a.grp_chunk = d_grp_chunk.p;
+  a.chunk_end = d_chunk_end.p;
+  a.voff = d_voff.p;
   a.ints = h->d_ints;
   a.gblock = d_gblock.p;
   a.gb_off = o->d_gb_off;
