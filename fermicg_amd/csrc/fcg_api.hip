@@ -6,36 +6,47 @@
 
 #include "fcg_internal.h"
 
-namespace {
-
-template <typename T>
-int dev_upload(fcg_ctx *ctx, T **dst, const T *src, int64_t n) {
-  *dst = nullptr;
-  if (n <= 0) return FCG_OK;
-  if (hipMalloc((void **)dst, sizeof(T) * n) != hipSuccess) {
-    *dst = nullptr;
-    ctx->err = "hipMalloc failed (" + std::to_string(sizeof(T) * n) + " bytes)";
-    return FCG_ENOMEM;
+void *fcg_dmalloc(fcg_ctx *ctx, size_t bytes) {
+  void *p = nullptr;
+  if (hipMallocAsync(&p, std::max<size_t>(bytes, 1), ctx->stream) != hipSuccess) {
+    ctx->err = "hipMallocAsync failed (" + std::to_string(bytes) + " bytes)";
+    return nullptr;
   }
-  FCG_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream), ctx);
-  return FCG_OK;
+  return p;
 }
+
+void fcg_dfree(fcg_ctx *ctx, void *p) {
+  if (p) (void)hipFreeAsync(p, ctx->stream);
+}
+
+void fcg_plan_note_user(fcg_plan *pl, fcg_ctx *ctx) {
+  if (ctx == pl->ctx) return;
+  std::lock_guard<std::mutex> g(pl->users_mu);
+  if (std::find(pl->users.begin(), pl->users.end(), ctx) == pl->users.end())
+    pl->users.push_back(ctx);
+}
+
+namespace {
 
 template <typename T>
 int dev_alloc(fcg_ctx *ctx, T **dst, int64_t n) {
   *dst = nullptr;
   if (n <= 0) return FCG_OK;
-  if (hipMalloc((void **)dst, sizeof(T) * n) != hipSuccess) {
-    *dst = nullptr;
-    ctx->err = "hipMalloc failed (" + std::to_string(sizeof(T) * n) + " bytes)";
-    return FCG_ENOMEM;
-  }
+  *dst = static_cast<T *>(fcg_dmalloc(ctx, sizeof(T) * n));
+  return *dst ? FCG_OK : FCG_ENOMEM;
+}
+
+template <typename T>
+int dev_upload(fcg_ctx *ctx, T **dst, const T *src, int64_t n) {
+  if (int rc = dev_alloc(ctx, dst, n)) return rc;
+  if (n > 0)
+    FCG_HIP(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream), ctx);
   return FCG_OK;
 }
 
 template <typename T>
-void dev_free(T *&p) {
-  if (p) (void)hipFree(p);
+void dev_free(fcg_ctx *ctx, T *&p) {
+  if (p) fcg_dfree(ctx, p);
   p = nullptr;
 }
 
@@ -47,8 +58,8 @@ void dev_free(T *&p) {
 
 int ensure_scratch(fcg_ctx *ctx, int64_t n) {
   if (n <= ctx->scratch_elems) return FCG_OK;
-  dev_free(ctx->d_c);
-  dev_free(ctx->d_s);
+  dev_free(ctx, ctx->d_c);
+  dev_free(ctx, ctx->d_s);
   ctx->scratch_elems = 0;
   TRY(dev_alloc(ctx, &ctx->d_c, n));
   TRY(dev_alloc(ctx, &ctx->d_s, n));
@@ -93,8 +104,9 @@ int fcg_create(int device, fcg_ctx_t *out) {
 void fcg_destroy(fcg_ctx_t ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  dev_free(ctx->d_c);
-  dev_free(ctx->d_s);
+  dev_free(ctx, ctx->d_c);
+  dev_free(ctx, ctx->d_s);
+  (void)hipStreamSynchronize(ctx->stream);
   for (auto &e : ctx->slots)
     if (e) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -148,9 +160,9 @@ int fcg_ops_upload(fcg_ctx_t ctx, int n_clusters, int S, int n_ops, const int32_
 
 void fcg_ops_destroy(fcg_ops_t o) {
   if (!o) return;
-  dev_free(o->d_gdata);
-  dev_free(o->d_gb_off);
-  dev_free(o->d_gb_dims);
+  dev_free(o->ctx, o->d_gdata);
+  dev_free(o->ctx, o->d_gb_off);
+  dev_free(o->ctx, o->d_gb_dims);
   delete o;
 }
 
@@ -220,8 +232,8 @@ int fcg_ham_upload(fcg_ctx_t ctx, int n_clusters, int n_terms, const int32_t *nb
 
 void fcg_ham_destroy(fcg_ham_t h) {
   if (!h) return;
-  dev_free(h->d_terms);
-  dev_free(h->d_ints);
+  dev_free(h->ctx, h->d_terms);
+  dev_free(h->ctx, h->d_ints);
   delete h;
 }
 
@@ -304,13 +316,13 @@ int64_t fcg_space_nvar(fcg_space_t sp) { return sp ? sp->nvar : -1; }
 
 void fcg_space_destroy(fcg_space_t sp) {
   if (!sp) return;
-  dev_free(sp->d_lo);
-  dev_free(sp->d_hi);
-  dev_free(sp->d_rank);
-  dev_free(sp->d_foff);
-  dev_free(sp->d_var);
-  dev_free(sp->d_dim);
-  dev_free(sp->d_factors);
+  dev_free(sp->ctx, sp->d_lo);
+  dev_free(sp->ctx, sp->d_hi);
+  dev_free(sp->ctx, sp->d_rank);
+  dev_free(sp->ctx, sp->d_foff);
+  dev_free(sp->ctx, sp->d_var);
+  dev_free(sp->ctx, sp->d_dim);
+  dev_free(sp->ctx, sp->d_factors);
   delete sp;
 }
 
@@ -458,10 +470,10 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
     }
   }
   // device copies of the task bookkeeping are only needed for the build
-  dev_free(pl->d_tasks);
-  dev_free(pl->d_pair_list);
-  dev_free(pl->d_pair_t0);
-  dev_free(pl->d_pair_col);
+  dev_free(ctx, pl->d_tasks);
+  dev_free(ctx, pl->d_pair_list);
+  dev_free(ctx, pl->d_pair_t0);
+  dev_free(ctx, pl->d_pair_col);
   pl->d_thr = nullptr;
   pl->build_us =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
@@ -492,7 +504,7 @@ int fcg_plan_create_pair_screened(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_s
   if (thresh > 0.0) TRY(fcg_screen_thresholds(ctx, ket->nvar, R, c, thresh, &thr));
   const int rc = fcg_create_plan(ctx, h, o, bra, ket, nbody, precisions, out, INT64_MAX, nullptr,
                                  thr);
-  if (thr) (void)hipFree(thr);
+  if (thr) fcg_dfree(ctx, thr);
   return rc;
 }
 
@@ -523,26 +535,30 @@ int fcg_plan_device_bytes(fcg_plan_t pl, int64_t *bytes) {
 
 void fcg_plan_destroy(fcg_plan_t pl) {
   if (!pl) return;
+  // other contexts that read the plan finish first; the plan's own stream
+  // orders the frees after its work
+  for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
+  fcg_ctx *ctx = pl->ctx;
+  dev_free(ctx, pl->d_row_ptr);
+  dev_free(ctx, pl->d_pair_B);
+  dev_free(ctx, pl->d_pair_K);
+  dev_free(ctx, pl->d_pair_t0);
+  dev_free(ctx, pl->d_pair_koff);
+  dev_free(ctx, pl->d_pair_col);
+  dev_free(ctx, pl->d_tasks);
+  dev_free(ctx, pl->d_pair_list);
+  dev_free(ctx, pl->d_diag_koff);
+  dev_free(ctx, pl->d_diag_col);
+  dev_free(ctx, pl->d_row_koff);
+  dev_free(ctx, pl->d_col_ptr);
+  dev_free(ctx, pl->d_order);
+  dev_free(ctx, pl->d_colidx);
+  dev_free(ctx, pl->d_K64);
+  dev_free(ctx, pl->d_K32);
+  dev_free(ctx, pl->d_Kbf);
+  dev_free(ctx, pl->d_Kb16);
+  if (pl->d_ci) fcg_dfree(ctx, pl->d_ci);
   fcg_plan_release(pl->reserved_bytes);
-  dev_free(pl->d_row_ptr);
-  dev_free(pl->d_pair_B);
-  dev_free(pl->d_pair_K);
-  dev_free(pl->d_pair_t0);
-  dev_free(pl->d_pair_koff);
-  dev_free(pl->d_pair_col);
-  dev_free(pl->d_tasks);
-  dev_free(pl->d_pair_list);
-  dev_free(pl->d_diag_koff);
-  dev_free(pl->d_diag_col);
-  dev_free(pl->d_row_koff);
-  dev_free(pl->d_col_ptr);
-  dev_free(pl->d_order);
-  dev_free(pl->d_colidx);
-  dev_free(pl->d_K64);
-  dev_free(pl->d_K32);
-  dev_free(pl->d_Kbf);
-  dev_free(pl->d_Kb16);
-  if (pl->d_ci) (void)hipFree(pl->d_ci);
   delete pl;
 }
 
@@ -638,9 +654,15 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
     return FCG_EINVAL;
   }
   (void)hipSetDevice(ctx->device);
+  fcg_plan_note_user(pl, ctx);
   const int64_t n = (int64_t)R * pl->ksp->nvar;
   if (pl->ci_bytes < n * 8) {  // interleaved copy of c, sized for the FP64 form
-    if (pl->d_ci) (void)hipFree(pl->d_ci);
+    if (pl->d_ci) {
+      // the old copy may still be read on the streams that used the plan
+      (void)hipStreamSynchronize(pl->ctx->stream);
+      for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
+      fcg_dfree(ctx, pl->d_ci);
+    }
     pl->d_ci = nullptr;
     pl->ci_bytes = 0;
     double *p = nullptr;
@@ -680,6 +702,7 @@ int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t pl, double *hd) {
     return FCG_EINVAL;
   }
   (void)hipSetDevice(ctx->device);
+  fcg_plan_note_user(pl, ctx);
   const int64_t n = pl->sp->nvar;
   TRY(ensure_scratch(ctx, n));
   FCG_HIP(fcg_launch_diag(pl, ctx->d_s, ctx->stream), ctx);

@@ -161,9 +161,9 @@ struct Dav {
   int nchunk = 0;
   double *V = nullptr, *W = nullptr, *X = nullptr, *AX = nullptr, *T = nullptr, *T2 = nullptr;
   double *hd = nullptr, *small = nullptr, *theta = nullptr, *part = nullptr;
-  ~Dav() {
+  ~Dav() {  // stream-ordered: waits for no other context's work
     for (double *p : {V, W, X, AX, T, T2, hd, small, theta, part})
-      if (p) (void)hipFree(p);
+      if (p) fcg_dfree(ctx, p);
   }
 };
 
@@ -251,6 +251,7 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
     ctx->err = "Davidson needs a plan within one space";
     return FCG_EINVAL;
   }
+  fcg_plan_note_user(pl, ctx);
   Dav d;
   d.ctx = ctx;
   const int64_t n = pl->sp->nvar;
@@ -274,7 +275,8 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
   // threshold
   const int refresh = o->refresh_k > 0 ? o->refresh_k : INT32_MAX;
   auto alloc = [&](double **p, int64_t cnt) {
-    return hipMalloc((void **)p, sizeof(double) * std::max<int64_t>(cnt, 1)) == hipSuccess;
+    *p = static_cast<double *>(fcg_dmalloc(ctx, sizeof(double) * std::max<int64_t>(cnt, 1)));
+    return *p != nullptr;
   };
   if (!alloc(&d.V, n * mmax) || !alloc(&d.W, n * mmax) || !alloc(&d.X, n * KR) ||
       !alloc(&d.AX, n * KR) || !alloc(&d.T, n * KR) || !alloc(&d.T2, n * KR) ||

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -119,6 +120,10 @@ struct fcg_space {
 
 struct fcg_plan {
   fcg_ctx *ctx = nullptr;
+  // contexts other than ctx that launched work reading this plan: their
+  // streams are synchronised before the plan's memory is freed
+  std::mutex users_mu;
+  std::vector<fcg_ctx *> users;
   fcg_space *sp = nullptr;   // bra space (rows of H, sigma)
   fcg_space *ksp = nullptr;  // ket space (columns of H, c); == sp for H within one space
   int64_t n_pair = 0, n_task = 0, n_kel = 0;
@@ -210,6 +215,18 @@ int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisi
 // reserve a plan's bytes against max_bytes and the process-wide plan budget
 // (FCG_ENOMEM, nothing held, when either would be exceeded); release on destroy
 int fcg_plan_reserve(fcg_ctx *ctx, int64_t bytes, int64_t max_bytes, int64_t *need);
+
+// Device memory of the library is stream-ordered on the owning context's
+// stream (hipMallocAsync / hipFreeAsync): freeing it waits for no other
+// context's work, where hipFree would synchronise the whole device.  Memory
+// read on other contexts' streams is freed only after those streams are
+// synchronised (plans record such users; uploads and plan creation end
+// with a stream synchronisation).  fcg_dmalloc returns nullptr on failure
+// with ctx->err set.
+void *fcg_dmalloc(fcg_ctx *ctx, size_t bytes);
+void fcg_dfree(fcg_ctx *ctx, void *p);
+// note that ctx launches work reading pl (no-op for the owning context)
+void fcg_plan_note_user(fcg_plan *pl, fcg_ctx *ctx);
 void fcg_plan_release(int64_t bytes);
 constexpr int FCG_SPARSE_MAX_BLOCK = 256;  // configurations per Fock block (device join)
 

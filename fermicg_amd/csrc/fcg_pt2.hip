@@ -88,13 +88,19 @@ __global__ __launch_bounds__(256) void k_pt2_sum(int64_t n, int R, const double 
   }
 }
 
+// scratch of one PT2 call, stream-ordered on its context
 template <typename T>
 struct DevBuf {
+  fcg_ctx *ctx = nullptr;
   T *p = nullptr;
+  explicit DevBuf(fcg_ctx *c) : ctx(c) {}
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) fcg_dfree(ctx, p);
   }
-  bool alloc(int64_t n) { return hipMalloc((void **)&p, sizeof(T) * std::max<int64_t>(n, 1)) == hipSuccess; }
+  bool alloc(int64_t n) {
+    p = static_cast<T *>(fcg_dmalloc(ctx, sizeof(T) * std::max<int64_t>(n, 1)));
+    return p != nullptr;
+  }
 };
 
 }  // namespace
@@ -110,12 +116,12 @@ int fcg_screen_thresholds(fcg_ctx *ctx, int64_t n, int R, const double *c, doubl
     for (int r = 0; r < R; r++) m = std::max(m, std::fabs(c[(int64_t)r * n + j]));
     thr[j] = m > 0.0 ? thresh / m : INFINITY;
   }
-  if (hipMalloc((void **)d_thr, sizeof(double) * thr.size()) != hipSuccess) {
-    *d_thr = nullptr;
-    ctx->err = "hipMalloc failed (screening thresholds)";
-    return FCG_ENOMEM;
-  }
-  FCG_HIP(hipMemcpy(*d_thr, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice), ctx);
+  *d_thr = static_cast<double *>(fcg_dmalloc(ctx, sizeof(double) * thr.size()));
+  if (!*d_thr) return FCG_ENOMEM;
+  FCG_HIP(hipMemcpyAsync(*d_thr, thr.data(), sizeof(double) * thr.size(), hipMemcpyHostToDevice,
+                         ctx->stream),
+          ctx);
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
   return FCG_OK;
 }
 
@@ -144,7 +150,7 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
   const std::vector<int64_t> &hoff = tg.hoff;
   const std::vector<int32_t> &hstride = tg.hstride;
   // sigma on the first-order space
-  DevBuf<double> dthr;
+  DevBuf<double> dthr(ctx);
   if (thresh > 0.0)
     if (int rc = fcg_screen_thresholds(ctx, var->nvar, R, c, thresh, &dthr.p)) return rc;
   fcg_plan *pl = nullptr;
@@ -152,10 +158,10 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
                                nullptr, dthr.p))
     return rc;
   std::unique_ptr<fcg_plan, void (*)(fcg_plan *)> pguard(pl, fcg_plan_destroy);
-  DevBuf<double> dc, ds, dhd, de0, dpart;
-  DevBuf<uint8_t> din;
-  DevBuf<int64_t> dhoff;
-  DevBuf<int32_t> dhstr;
+  DevBuf<double> dc(ctx), ds(ctx), dhd(ctx), de0(ctx), dpart(ctx);
+  DevBuf<uint8_t> din(ctx);
+  DevBuf<int64_t> dhoff(ctx);
+  DevBuf<int32_t> dhstr(ctx);
   if (!dc.alloc((int64_t)R * var->nvar) || !ds.alloc((int64_t)R * nt) || !dhd.alloc(nt) ||
       !de0.alloc(R) || !dpart.alloc((int64_t)PT2_BLOCKS * R * 2) || !din.alloc(nt) ||
       !dhoff.alloc(tsp->n_box * N) || !dhstr.alloc(tsp->n_box * N)) {

@@ -523,18 +523,25 @@ __global__ __launch_bounds__(ST) void k_fock_pairs(PairArgs a) {
   }
 }
 
+// device scratch of the plan build, stream-ordered on the building context
 template <typename T>
 struct DBuf {
+  fcg_ctx *ctx = nullptr;
   T *p = nullptr;
+  bool alloc(fcg_ctx *c, int64_t n) {
+    ctx = c;
+    p = static_cast<T *>(fcg_dmalloc(c, sizeof(T) * std::max<int64_t>(n, 1)));
+    return p != nullptr;
+  }
   ~DBuf() {
-    if (p) (void)hipFree(p);
+    if (p) fcg_dfree(ctx, p);
   }
 };
 
 template <typename T>
 int upload(fcg_ctx *ctx, DBuf<T> &b, const T *src, int64_t n) {
-  if (hipMalloc((void **)&b.p, sizeof(T) * std::max<int64_t>(n, 1)) != hipSuccess) {
-    ctx->err = "hipMalloc failed in the sparse plan";
+  if (!b.alloc(ctx, n)) {
+    ctx->err = "device allocation failed in the sparse plan";
     return FCG_ENOMEM;
   }
   if (n > 0)
@@ -587,9 +594,8 @@ int device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int n
   if (!rc) rc = upload(ctx, d_tc, tcl.data(), (int64_t)tcl.size());
   if (!rc) rc = upload(ctx, d_live, live.data(), (int64_t)live.size());
   if (rc) return rc;
-  if (hipMalloc((void **)&d_count.p, sizeof(int32_t) * std::max<int64_t>(sp->n_fock, 1)) !=
-      hipSuccess) {
-    ctx->err = "hipMalloc failed in the sparse plan";
+  if (!d_count.alloc(ctx, sp->n_fock)) {
+    ctx->err = "device allocation failed in the sparse plan";
     return FCG_ENOMEM;
   }
   PairArgs a{};
@@ -618,9 +624,8 @@ int device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int n
   const int64_t n = ptr.back();
   rc = upload(ctx, d_ptr, ptr.data(), (int64_t)ptr.size());
   if (rc) return rc;
-  if (hipMalloc((void **)&d_fk.p, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess ||
-      hipMalloc((void **)&d_g.p, sizeof(int32_t) * std::max<int64_t>(n, 1)) != hipSuccess) {
-    ctx->err = "hipMalloc failed in the sparse plan";
+  if (!d_fk.alloc(ctx, n) || !d_g.alloc(ctx, n)) {
+    ctx->err = "device allocation failed in the sparse plan";
     return FCG_ENOMEM;
   }
   a.ptr = d_ptr.p;
@@ -745,9 +750,8 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   if (!rc) rc = upload(ctx, d_chunk_end, chunk_end.data(), (int64_t)chunk_end.size());
   if (!rc) rc = upload(ctx, d_voff, voff.data(), (int64_t)voff.size());
   if (rc) return rc;
-  if (hipMalloc((void **)&d_nnz.p, sizeof(int32_t) * std::max<int64_t>(n_row, 1)) != hipSuccess ||
-      hipMalloc((void **)&d_stats.p, 3 * sizeof(unsigned long long)) != hipSuccess) {
-    ctx->err = "hipMalloc failed in the sparse plan";
+  if (!d_nnz.alloc(ctx, n_row) || !d_stats.alloc(ctx, 3)) {
+    ctx->err = "device allocation failed in the sparse plan";
     return FCG_ENOMEM;
   }
   FCG_HIP(hipMemsetAsync(d_stats.p, 0, 3 * sizeof(unsigned long long), ctx->stream), ctx);
@@ -819,7 +823,8 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   };
   auto alloc = [&](auto **p, int64_t n) {
     using T = std::remove_reference_t<decltype(**p)>;
-    return hipMalloc((void **)p, sizeof(T) * std::max<int64_t>(n, 1)) == hipSuccess;
+    *p = static_cast<T *>(fcg_dmalloc(ctx, sizeof(T) * std::max<int64_t>(n, 1)));
+    return *p != nullptr;
   };
   if (!alloc(&pl->d_row_koff, n_row) || !alloc(&pl->d_col_ptr, n_row + 1) ||
       !alloc(&pl->d_order, (int64_t)order.size()) || !alloc(&pl->d_colidx, n_cols) ||
