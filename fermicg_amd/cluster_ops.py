@@ -146,7 +146,7 @@ class LocalFockSpace:
                 if r is None:
                     return None
                 sec_next, M = r
-                blocks.append(np.einsum("ij,bjm->bim", M, Y))
+                blocks.append(np.matmul(M, Y))  # M applied to every batch slice
             # new leading axis is this operator's index; earlier operators
             # (to the left) will be prepended, so stack as (p, batch, ...)
             Y = np.stack(blocks, axis=0).reshape(n * Y.shape[0], -1, X.shape[1])
@@ -236,7 +236,7 @@ def compute_cluster_ops(bases, ints: InCoreInts):
                     # an intermediate sector is empty: the block exists and is zero
                     G = np.zeros((n,) * k + (Vb.shape[1], cb.n_states(sec_ket)))
                 else:
-                    G = np.einsum("im,...in->...mn", Vb, r[1])
+                    G = np.matmul(Vb.T, r[1])  # Vb' applied to every (p1..pk) slice
                 # (p1..pk, I, J) -> column-major (n^k, Mb, Mk), p1 fastest
                 Gf = np.asfortranarray(G).reshape((n ** k, Vb.shape[1], cb.n_states(sec_ket)), order="F")
                 ops[name][(sec_bra, sec_ket)] = np.asfortranarray(Gf)
