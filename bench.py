@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2", "bf16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
+    ap.add_argument("--ss-vecs", type=int, default=12,
+                    help="Davidson subspace vectors per root before a restart")
     ap.add_argument("--cpu-rows", type=int, default=2000)
     ap.add_argument("--clusters", type=int, default=6, help="bst workload: number of clusters")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -192,7 +194,8 @@ def main():
     if not args.no_solve:
         for prec in dict.fromkeys(("fp64", "fp32", "bf16", args.precision)):
             ta = time.perf_counter()
-            e, v, stt = pl.davidson(st.coeffs, conv_thresh=1e-8, max_iter=600, max_ss_vecs=12,
+            e, v, stt = pl.davidson(st.coeffs, conv_thresh=1e-8, max_iter=600,
+                                    max_ss_vecs=args.ss_vecs,
                                     lindep_thresh=1e-12, precision=prec, refresh_k=0)
             dt = time.perf_counter() - ta
             solves[prec] = {"seconds": dt, "energies": [float(x) for x in e], **stt}
