@@ -48,7 +48,8 @@ class DavidsonStats(ctypes.Structure):
 
 class PT2Stats(ctypes.Structure):
     _fields_ = [("n_target_fock", c_i64), ("n_target_var", c_i64), ("n_first_order", c_i64),
-                ("block_pairs", c_i64), ("block_elems", c_i64), ("seconds", c_f64)]
+                ("block_pairs", c_i64), ("block_elems", c_i64), ("seconds", c_f64),
+                ("build_flops", c_f64), ("build_seconds", c_f64)]
 
 
 class FCGError(RuntimeError):

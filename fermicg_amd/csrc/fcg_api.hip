@@ -387,13 +387,13 @@ int64_t fcg_plan_bytes(int64_t n_kel, int64_t n_cols, int64_t n_box, int precisi
 
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need,
-                    const double *thr) {
+                    const double *thr, const FusedSigma *fused) {
   if (!ctx || !h || !o || !sp || !ksp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
   auto t0 = std::chrono::steady_clock::now();
   TRY(fcg_form_boxes(ctx, o, sp));
   if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
-  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box && !thr) {
+  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box && !thr && !fused) {
     // every box one configuration: join and build on the device
     return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
   }
@@ -403,8 +403,9 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   int64_t reserved = 0;
   {
     // the pair and task tables live on the device during the build
-    const int64_t bytes = fcg_plan_bytes(hp.n_kel, (int64_t)hp.colidx.size(), sp->n_box,
-                                         precisions | (1 << FCG_FP64)) +
+    const int64_t bytes = (fused ? 0
+                                 : fcg_plan_bytes(hp.n_kel, (int64_t)hp.colidx.size(), sp->n_box,
+                                                  precisions | (1 << FCG_FP64))) +
                           (int64_t)hp.pair_B.size() * 44 + (int64_t)hp.tasks.size() * sizeof(TaskDev);
     if (int rc2 = fcg_plan_reserve(ctx, bytes, max_bytes, need)) return rc2;
     reserved = bytes;
@@ -438,19 +439,25 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
   PTRY(dev_upload(ctx, &pl->d_pair_list, hp.pair_list.data(), pl->n_pair));
   pl->n_small = hp.n_small;
-  PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
-  PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
-  PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
-  PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
-  PTRY(dev_upload(ctx, &pl->d_order, hp.order.data(), (int64_t)hp.order.size()));
-  pl->n_multi = hp.n_multi;
-  pl->n_single_slots = hp.n_single_slots;
-  PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
-  PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
-  if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
-  if (pl->precisions & (1 << FCG_BF16X2)) PTRY(dev_alloc(ctx, &pl->d_Kbf, pl->n_kel));
-  if (pl->precisions & (1 << FCG_BF16)) PTRY(dev_alloc(ctx, &pl->d_Kb16, pl->n_kel));
-  if (pl->n_kel > 0) {
+  const bool store = fused == nullptr;
+  pl->holds_blocks = store;
+  if (store) {  // the apply's layout: only for a plan that keeps its blocks
+    PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
+    PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
+    PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));
+    PTRY(dev_upload(ctx, &pl->d_col_ptr, hp.col_ptr.data(), sp->n_box + 1));
+    PTRY(dev_upload(ctx, &pl->d_order, hp.order.data(), (int64_t)hp.order.size()));
+    pl->n_multi = hp.n_multi;
+    pl->n_single_slots = hp.n_single_slots;
+    PTRY(dev_upload(ctx, &pl->d_colidx, hp.colidx.data(), pl->n_cols));
+  }
+  if (store) {
+    PTRY(dev_alloc(ctx, &pl->d_K64, pl->n_kel));
+    if (pl->precisions & (1 << FCG_FP32)) PTRY(dev_alloc(ctx, &pl->d_K32, pl->n_kel));
+    if (pl->precisions & (1 << FCG_BF16X2)) PTRY(dev_alloc(ctx, &pl->d_Kbf, pl->n_kel));
+    if (pl->precisions & (1 << FCG_BF16)) PTRY(dev_alloc(ctx, &pl->d_Kb16, pl->n_kel));
+  }
+  if (store && pl->n_kel > 0) {
     // zero columns pad each row panel to whole quads
     const hipError_t e = hipMemsetAsync(pl->d_K64, 0, sizeof(double) * pl->n_kel, ctx->stream);
     if (e != hipSuccess) {
@@ -460,8 +467,8 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
     }
   }
   if (pl->n_pair > 0) {
-    hipError_t e = fcg_launch_build(pl, h, o, ctx->stream);
-    if (e == hipSuccess) e = fcg_launch_convert(pl, ctx->stream);
+    hipError_t e = fcg_launch_build(pl, h, o, ctx->stream, fused);
+    if (e == hipSuccess && store) e = fcg_launch_convert(pl, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
       ctx->err = std::string("block build: ") + hipGetErrorString(e);
@@ -653,6 +660,10 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
     ctx->err = "precision not built into this plan";
     return FCG_EINVAL;
   }
+  if (!pl->holds_blocks) {
+    ctx->err = "plan holds no blocks";
+    return FCG_EINVAL;
+  }
   (void)hipSetDevice(ctx->device);
   fcg_plan_note_user(pl, ctx);
   const int64_t n = (int64_t)R * pl->ksp->nvar;
@@ -697,8 +708,8 @@ int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c, double *sigm
 
 int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t pl, double *hd) {
   if (!ctx || !pl || !hd) return FCG_EINVAL;
-  if (pl->ksp != pl->sp) {
-    ctx->err = "diagonal needs a plan within one space";
+  if (pl->ksp != pl->sp || !pl->holds_blocks) {
+    ctx->err = "diagonal needs a plan within one space that holds its blocks";
     return FCG_EINVAL;
   }
   (void)hipSetDevice(ctx->device);

@@ -247,8 +247,8 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
                             fcg_davidson_stats *st) {
   if (!ctx || !pl || !v0 || !o || !evals || !evecs || R < 1 || R > FCG_MAX_R) return FCG_EINVAL;
   (void)hipSetDevice(ctx->device);
-  if (pl->ksp != pl->sp) {
-    ctx->err = "Davidson needs a plan within one space";
+  if (pl->ksp != pl->sp || !pl->holds_blocks) {
+    ctx->err = "Davidson needs a plan within one space that holds its blocks";
     return FCG_EINVAL;
   }
   fcg_plan_note_user(pl, ctx);

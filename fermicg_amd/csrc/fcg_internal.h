@@ -137,6 +137,7 @@ struct fcg_plan {
   int32_t *d_pair_col = nullptr;  // [n_pair] first panel column of the pair
   TaskDev *d_tasks = nullptr;
   int64_t *d_pair_list = nullptr;  // build only: pair indices, small-LDS pairs first
+  bool holds_blocks = true;        // false: built with a fused sigma, nothing stored
   int64_t n_small = 0;
   int64_t *d_diag_koff = nullptr; // [n_box] panel holding the (B,B) block or -1
   int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
@@ -240,13 +241,21 @@ int fcg_form_boxes(fcg_ctx *ctx, fcg_ops *o, fcg_space *sp);
 // max_bytes: refuse (FCG_ENOMEM, nothing allocated, *need set) when the
 // plan's device footprint would exceed it
 // thr (device, [ket n_var] or nullptr): screened build, see BuildArgs::thr
+// fused (or nullptr): sigma += H c during the build, no block stored; the
+// plan then holds no blocks (for a sigma needed once, as in PT2)
+struct FusedSigma {
+  int R;
+  const double *c;  // device, ket n_var x R, column-major
+  double *sigma;    // device, bra n_var x R, column-major, accumulated into
+};
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
-                    int64_t *need = nullptr, const double *thr = nullptr);
+                    int64_t *need = nullptr, const double *thr = nullptr,
+                    const FusedSigma *fused = nullptr);
 
 // kernels (fcg_kernels.hip)
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
-                            hipStream_t s);
+                            hipStream_t s, const FusedSigma *fused = nullptr);
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
 // vm / vr: apply variants for multi-row and one-row boxes; parts: bit 0
 // multi-row boxes, bit 1 one-row boxes

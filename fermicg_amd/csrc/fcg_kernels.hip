@@ -43,6 +43,13 @@ struct BuildArgs {
   // magnitude exceeds thr[ket variable] (_collect_significant_thread!)
   const double *thr;
   const int64_t *kvar;
+  // fused sigma (fsig != nullptr): fsig[bra var, r] += block x fc[ket var, r]
+  // instead of storing the block (column-major, nb / nk variables)
+  const double *fc;
+  double *fsig;
+  int fR;
+  int64_t nb_var, nk_var;
+  const int64_t *bvar;
 };
 
 __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int x, int i) {
@@ -239,6 +246,29 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
       }
     }
     __syncthreads();
+  }
+  if (a.fsig) {
+    // fused sigma: the block's products summed per bra row in LDS (the chain
+    // buffer is free now), then one global add per (row, root)
+    double *sY = sX[0];
+    const int nY = db * a.fR;
+    for (int e = tid; e < nY; e += BT) sY[e] = 0.0;
+    __syncthreads();
+    const int64_t kv0 = a.kvar[K];
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+      const int e = tid + BT * k;
+      if (e >= db * dk) break;
+      if (acc[k] == 0.0) continue;
+      const int r = e % db, q = e / db;
+      for (int rr = 0; rr < a.fR; rr++)
+        atomicAdd(&sY[r * a.fR + rr], acc[k] * a.fc[(int64_t)rr * a.nk_var + kv0 + q]);
+    }
+    __syncthreads();
+    const int64_t bv0 = a.bvar[B];
+    for (int e = tid; e < nY; e += BT)
+      atomicAdd(&a.fsig[(int64_t)(e % a.fR) * a.nb_var + bv0 + e / a.fR], sY[e]);
+    return;
   }
   double *Kout = a.K + a.pair_koff[p];
   const int64_t col0 = a.pair_col[p];
@@ -611,7 +641,7 @@ __global__ void k_diag(int64_t n_box, const int64_t *__restrict__ var,
 }  // namespace
 
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
-                            hipStream_t s) {
+                            hipStream_t s, const FusedSigma *fused) {
   const fcg_space *sp = pl->sp;
   BuildArgs a;
   a.pair_B = pl->d_pair_B;
@@ -641,6 +671,12 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.thr = pl->d_thr;
   a.kvar = kp->d_var;
   a.pair_list = pl->d_pair_list;
+  a.fc = fused ? fused->c : nullptr;
+  a.fsig = fused ? fused->sigma : nullptr;
+  a.fR = fused ? fused->R : 0;
+  a.nb_var = sp->nvar;
+  a.nk_var = kp->nvar;
+  a.bvar = sp->d_var;
   const int64_t chunk = 1 << 20;
   // pairs [0, n_small) of the list fit the small LDS class, the rest the full one
   for (int64_t base = 0; base < pl->n_pair;) {

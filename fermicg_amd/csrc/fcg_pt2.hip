@@ -149,15 +149,9 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
   const std::vector<uint8_t> &inspace = tg.inspace;
   const std::vector<int64_t> &hoff = tg.hoff;
   const std::vector<int32_t> &hstride = tg.hstride;
-  // sigma on the first-order space
   DevBuf<double> dthr(ctx);
   if (thresh > 0.0)
     if (int rc = fcg_screen_thresholds(ctx, var->nvar, R, c, thresh, &dthr.p)) return rc;
-  fcg_plan *pl = nullptr;
-  if (int rc = fcg_create_plan(ctx, h, o, tsp, var, nbody, 1 << FCG_FP64, &pl, INT64_MAX,
-                               nullptr, dthr.p))
-    return rc;
-  std::unique_ptr<fcg_plan, void (*)(fcg_plan *)> pguard(pl, fcg_plan_destroy);
   DevBuf<double> dc(ctx), ds(ctx), dhd(ctx), de0(ctx), dpart(ctx);
   DevBuf<uint8_t> din(ctx);
   DevBuf<int64_t> dhoff(ctx);
@@ -173,7 +167,15 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
   FCG_HIP(hipMemcpyAsync(din.p, inspace.data(), nt, hipMemcpyHostToDevice, ctx->stream), ctx);
   FCG_HIP(hipMemcpyAsync(dhoff.p, hoff.data(), sizeof(int64_t) * hoff.size(), hipMemcpyHostToDevice, ctx->stream), ctx);
   FCG_HIP(hipMemcpyAsync(dhstr.p, hstride.data(), sizeof(int32_t) * hstride.size(), hipMemcpyHostToDevice, ctx->stream), ctx);
-  if (int rc = fcg_sigma_device(ctx, pl, R, dc.p, ds.p, FCG_FP64)) return rc;
+  // sigma on the first-order space, accumulated during the block build (the
+  // blocks are used once, so none is stored)
+  FCG_HIP(hipMemsetAsync(ds.p, 0, sizeof(double) * R * nt, ctx->stream), ctx);
+  const FusedSigma fused{R, dc.p, ds.p};
+  fcg_plan *pl = nullptr;
+  if (int rc = fcg_create_plan(ctx, h, o, tsp, var, nbody, 1 << FCG_FP64, &pl, INT64_MAX,
+                               nullptr, dthr.p, &fused))
+    return rc;
+  std::unique_ptr<fcg_plan, void (*)(fcg_plan *)> pguard(pl, fcg_plan_destroy);
   if (tsp->n_box > 0)
     hipLaunchKernelGGL(k_h0_diag, dim3((unsigned)tsp->n_box), dim3(64), 0, ctx->stream,
                        tsp->n_box, N, tsp->d_var, tsp->d_rank, dhoff.p, dhstr.p, o->d_gdata, dhd.p);
@@ -202,6 +204,8 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
     st->n_first_order = nt - nin;
     st->block_pairs = pl->n_pair;
     st->block_elems = pl->n_kel;
+    st->build_flops = pl->build_flops;
+    st->build_seconds = pl->build_us * 1e-6;
     st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return FCG_OK;

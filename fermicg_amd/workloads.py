@@ -60,6 +60,21 @@ def tpsci_small(seed=0, R=1):
     return p, st
 
 
+def pt2_medium(seed=11, M=7, R=2):
+    """PT2 workload: 16 orbitals / 4 clusters of 4 at half filling / M states
+    per sector; variational space = every configuration of the Fock blocks
+    within 1 transferred electron (25 blocks, M=7: 60,025 configurations),
+    R orthonormal seeded vectors.  The first-order space is every
+    configuration of the Fock blocks reached by one TransferConfig."""
+    from .tpsci import orthonormalize
+    p = build_problem(16, 4, M, 32, seed, max_transfer=1)
+    st = TPSCIstate.full_blocks(p.bases, p.focks, R=R)
+    rng = np.random.Generator(np.random.PCG64(seed + 3000))
+    st.set_vector(orthonormalize(rng.normal(size=(len(st), R))))
+    p.meta = {"workload": "pt2_16o_4c_M%d" % M, "n_fock": len(p.focks), "n_cfg": len(st)}
+    return p, st
+
+
 def bst_medium(seed=1, R=2, n_clusters=6, max_roots=16, max_rank=8):
     """24 orbitals / 6 clusters / M=16 / Tucker P-Q blocks with <= 2 Q
     clusters and ranks <= 8 over the Fock configurations with <= 2

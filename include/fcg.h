@@ -212,6 +212,8 @@ typedef struct {
   int64_t block_pairs;    /* Hamiltonian blocks built */
   int64_t block_elems;    /* their elements */
   double seconds;         /* wall time of the call */
+  double build_flops;     /* planner flop count of the block build */
+  double build_seconds;   /* wall time of the join and block build */
 } fcg_pt2_stats;
 
 int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
