@@ -18,9 +18,10 @@ constexpr int FCG_MAX_BOX = 64;       // rows/cols of one Hamiltonian block
 constexpr int FCG_LDS_X = 4096;       // chain intermediates (doubles)
 constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
-// a smaller LDS footprint for the block pairs that fit it (several
+// smaller LDS footprints for the block pairs that fit them (several
 // workgroups per CU instead of one)
 constexpr int FCG_LDS_X_SMALL = 1024, FCG_LDS_T_SMALL = 512, FCG_LDS_S_SMALL = 256;
+constexpr int FCG_LDS_X_MID = 2048, FCG_LDS_T_MID = 1024, FCG_LDS_S_MID = 512;
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
 constexpr int FCG_N_PREC = 4;  // FCG_FP64 .. FCG_BF16
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
@@ -137,6 +138,7 @@ struct fcg_plan {
   int32_t *d_pair_col = nullptr;  // [n_pair] first panel column of the pair
   TaskDev *d_tasks = nullptr;
   int64_t *d_pair_list = nullptr;  // build only: pair indices, small-LDS pairs first
+  int64_t n_mid = 0;
   bool holds_blocks = true;        // false: built with a fused sigma, nothing stored
   int64_t n_small = 0;
   int64_t *d_diag_koff = nullptr; // [n_box] panel holding the (B,B) block or -1
@@ -171,9 +173,10 @@ struct HostPlan {
   int64_t n_multi = 0, n_single_slots = 0;
   std::vector<TaskDev> tasks;
   // pair indices for the block build: pairs within the small LDS limits
-  // first (n_small of them), then the rest, each in pair order
+  // first (n_small of them), then those within the middle limits (n_mid),
+  // then the rest, each in pair order
   std::vector<int64_t> pair_list;
-  int64_t n_small = 0;
+  int64_t n_small = 0, n_mid = 0;
   int64_t n_kel = 0;
   double flops = 0.0;
 };

@@ -678,13 +678,19 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.nk_var = kp->nvar;
   a.bvar = sp->d_var;
   const int64_t chunk = 1 << 20;
-  // pairs [0, n_small) of the list fit the small LDS class, the rest the full one
+  // the list holds the small LDS class's pairs [0, n_small), then the middle
+  // class's (n_mid), then the full class's
+  const int64_t mid0 = pl->n_small, full0 = pl->n_small + pl->n_mid;
   for (int64_t base = 0; base < pl->n_pair;) {
     a.pair_base = base;
-    const int64_t end = std::min(base + chunk, base < pl->n_small ? pl->n_small : pl->n_pair);
+    const int64_t lim = base < mid0 ? mid0 : base < full0 ? full0 : pl->n_pair;
+    const int64_t end = std::min(base + chunk, lim);
     const int64_t n = end - base;
-    if (base < pl->n_small)
+    if (base < mid0)
       hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL>),
+                         dim3((unsigned)n), dim3(BT), 0, s, a);
+    else if (base < full0)
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else
       hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S>), dim3((unsigned)n),

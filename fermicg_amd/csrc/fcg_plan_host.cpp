@@ -392,7 +392,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   double flops = 0.0;
   std::atomic<int> lim_err{FCG_OK};
   const char *lim_msg = nullptr;
-  std::vector<uint8_t> small(np, 0);
+  std::vector<uint8_t> lds_class(np, 0);  // 2 small, 1 middle, 0 full
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : flops)
   for (int64_t p = 0; p < np; p++) {
     if (lim_err) continue;
@@ -476,19 +476,24 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? best[s2] : 0) << (8 + 4 * s2);
       flops += best_fl + 2.0 * db * dk;
     }
-    small[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL;
+    lds_class[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
+                   ? 2
+                   : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
   }
   if (lim_err.load()) {
     *err = lim_msg;
     return lim_err.load();
   }
+  // LDS classes: 2 small, 1 middle, 0 full
   out->pair_list.resize(np);
-  int64_t ns = 0;
-  for (int64_t p = 0; p < np; p++)
-    if (small[p]) out->pair_list[ns++] = p;
-  out->n_small = ns;
-  for (int64_t p = 0, at = ns; p < np; p++)
-    if (!small[p]) out->pair_list[at++] = p;
+  int64_t at = 0;
+  for (int cls = 2; cls >= 0; cls--) {
+    const int64_t start = at;
+    for (int64_t p = 0; p < np; p++)
+      if (lds_class[p] == cls) out->pair_list[at++] = p;
+    if (cls == 2) out->n_small = at - start;
+    if (cls == 1) out->n_mid = at - start;
+  }
   for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
   // row panels: the blocks of one bra row side by side form a db x ncol
   // panel (ncol padded to a multiple of FCG_QUAD with zero columns), stored
