@@ -337,33 +337,60 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     *err = errmsg;
     return errcode;
   }
-  // assemble pairs ordered by bra box then ket box
+  // assemble pairs ordered by bra box then ket box, and their tasks
+  // contiguous in that order: each bra Fock block's runs of equal (B, K)
+  // are counted, offset by prefix sums and written in parallel
   struct PairRec { int32_t B, K; int64_t t0, t1; };
-  std::vector<PairRec> pairs;
-  std::vector<TaskDev> tasks;
-  size_t ntask = 0;
-  for (auto &v : per_fock) ntask += v.size();
-  tasks.reserve(ntask);
-  for (int64_t fb = 0; fb < bra->n_fock; fb++) {
+  const int64_t nfb = bra->n_fock;
+  std::vector<int64_t> pair_off(nfb + 1, 0), task_off(nfb + 1, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t fb = 0; fb < nfb; fb++) {
+    const auto &v = per_fock[fb];
+    int64_t n = 0;
+    for (size_t i = 0; i < v.size(); i++)
+      if (i == 0 || v[i].B != v[i - 1].B || v[i].K != v[i - 1].K) n++;
+    pair_off[fb + 1] = n;
+    task_off[fb + 1] = (int64_t)v.size();
+  }
+  for (int64_t fb = 0; fb < nfb; fb++) {
+    pair_off[fb + 1] += pair_off[fb];
+    task_off[fb + 1] += task_off[fb];
+  }
+  std::vector<PairRec> pairs(pair_off[nfb]);
+  out->tasks.resize(task_off[nfb]);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t fb = 0; fb < nfb; fb++) {
     auto &v = per_fock[fb];
-    size_t i = 0;
-    while (i < v.size()) {
-      size_t j = i;
-      const int64_t t0 = (int64_t)tasks.size();
-      while (j < v.size() && v[j].B == v[i].B && v[j].K == v[i].K) tasks.push_back(v[j++].t);
-      pairs.push_back({v[i].B, v[i].K, t0, (int64_t)tasks.size()});
-      i = j;
+    int64_t pp = pair_off[fb], tt = task_off[fb];
+    for (size_t i = 0; i < v.size(); i++) {
+      if (i == 0 || v[i].B != v[i - 1].B || v[i].K != v[i - 1].K) {
+        pairs[pp++] = {v[i].B, v[i].K, tt, tt};
+      }
+      out->tasks[tt++] = v[i].t;
+      pairs[pp - 1].t1 = tt;
     }
     std::vector<LocalTask>().swap(v);
   }
   auto pair_less = [](const PairRec &a, const PairRec &b) {
     return a.B != b.B ? a.B < b.B : a.K < b.K;
   };
-  if (!std::is_sorted(pairs.begin(), pairs.end(), pair_less))  // boxes grouped by Fock block
+  if (!std::is_sorted(pairs.begin(), pairs.end(), pair_less)) {
+    // bra boxes not grouped by Fock block: sort the pairs, then regroup
+    // their tasks in the new order
     std::stable_sort(pairs.begin(), pairs.end(), pair_less);
-  // tasks contiguous in pair order; per-task chain order; limits; flop count
+    std::vector<TaskDev> moved(out->tasks.size());
+    int64_t at = 0;
+    for (PairRec &r : pairs) {
+      const int64_t n = r.t1 - r.t0;
+      std::copy(out->tasks.begin() + r.t0, out->tasks.begin() + r.t1, moved.begin() + at);
+      r.t0 = at;
+      r.t1 = at + n;
+      at += n;
+    }
+    out->tasks.swap(moved);
+  }
+  // per-task chain order; limits; flop count
   const int64_t np = (int64_t)pairs.size();
-  out->tasks.clear();
   out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.assign(np + 1, 0);
   out->pair_koff.resize(np);
   out->pair_col.resize(np);
@@ -387,8 +414,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     row_cols[r.B] += dk;
     out->row_ptr[r.B + 1]++;
   }
-  out->tasks.resize(tasks.size());
-  // parallel pass: tasks in pair order, their chain order, LDS limits, flops
+  // parallel pass: each task's chain order, LDS limits, flops
   double flops = 0.0;
   std::atomic<int> lim_err{FCG_OK};
   const char *lim_msg = nullptr;
@@ -421,8 +447,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     // Γ slices (X), factor-compression rows (T), overlaps (S)
     int64_t need_x = 0, need_t = 0;
     for (int64_t t = r.t0; t < r.t1; t++) {
-      TaskDev &d = out->tasks[out->pair_t0[p] + (t - r.t0)];
-      d = tasks[t];
+      TaskDev &d = out->tasks[t];
       const int tm = d.term, nb = h->nbody[tm];
       int64_t np_[4], rr_[4];
       int64_t rest0 = 1;
