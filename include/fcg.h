@@ -10,9 +10,13 @@
  *   - host pointers are read during the call only; outputs are
  *     caller-allocated host memory unless the name ends in _device.
  *   - handles are device-resident and owned by the library; free them with
- *     fcg_destroy_*; destroy dependants (plans) before their inputs.
+ *     fcg_destroy_*; destroy dependants (plans) before their inputs, and all
+ *     handles before their context.
  *   - one fcg_ctx_t per host thread; calls on one ctx are serialised on its
- *     HIP stream.
+ *     HIP stream.  Device memory is stream-ordered on the context that made
+ *     the handle, so destroying a handle waits for no other context's work;
+ *     a plan may serve several contexts, one call at a time, and
+ *     fcg_plan_destroy first synchronises each of them.
  *   - layouts are column-major: Γ blocks (np, m_bra, m_ket), integrals as
  *     the ClusteredTerm ints array, vectors (n_var x R) with element (i,r)
  *     at r*n_var + i.  State indices and ranges are 1-based (Julia).
