@@ -298,10 +298,12 @@ __device__ __forceinline__ float kval(uint16_t u) {  // bf16
 //     32-bit blocks load branch-free from a clamped quad, 64-bit ones
 //     predicated (measured faster each way);
 //   one-row boxes (k_apply_rows): quads in flight.
-struct MultiVariant { int wpb, unroll; };
-constexpr MultiVariant kMulti[] = {{4, 16}, {1, 16}, {4, 8}};
-constexpr int kRowsUnroll[] = {8, 4};
-constexpr int N_MULTI_VARIANTS = 3, N_ROWS_VARIANTS = 2;
+struct MultiVariant { int wpb, unroll; bool nt; };
+constexpr MultiVariant kMulti[] = {{4, 16, false}, {1, 16, false}, {4, 8, false},
+                                   {4, 16, true}, {1, 16, true}};
+struct RowsVariant { int unroll; bool nt; };
+constexpr RowsVariant kRows[] = {{8, false}, {4, false}, {8, true}, {4, true}};
+constexpr int N_MULTI_VARIANTS = 5, N_ROWS_VARIANTS = 4;
 constexpr int TUNED_MAX_R = 4;  // larger root counts use the default variants
 template <typename TK>
 constexpr int default_multi() { return sizeof(TK) == 8 ? 1 : 0; }
@@ -361,17 +363,38 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Loads of data read once per sigma (blocks, column indices)
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef unsigned nu4 __attribute__((ext_vector_type(4)));
+typedef unsigned nu2 __attribute__((ext_vector_type(2)));
+typedef double nd2 __attribute__((ext_vector_type(2)));
+typedef int ni4 __attribute__((ext_vector_type(4)));
+// NT: non-temporal (the stream does not displace the gathered vector from
+// the caches); a variant choice, measured faster for some precisions and
+// plans and slower for others
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // One column quad of one panel row as loaded from HBM (16 or 32 bytes).
 template <typename TK> struct RawQuad;
 template <> struct RawQuad<float> {
   float4 v;
-  __device__ void load(const float *p) { v = *reinterpret_cast<const float4 *>(p); }
+  template <bool NT> __device__ void load(const float *p) {
+    const nf4 t = ld_stream<NT>(reinterpret_cast<const nf4 *>(p));
+    v = make_float4(t.x, t.y, t.z, t.w);
+  }
   __device__ void zero() { v = make_float4(0.f, 0.f, 0.f, 0.f); }
   __device__ void get(float (&k)[4]) const { k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w; }
 };
 template <> struct RawQuad<uint32_t> {
   uint4 v;
-  __device__ void load(const uint32_t *p) { v = *reinterpret_cast<const uint4 *>(p); }
+  template <bool NT> __device__ void load(const uint32_t *p) {
+    const nu4 t = ld_stream<NT>(reinterpret_cast<const nu4 *>(p));
+    v = make_uint4(t.x, t.y, t.z, t.w);
+  }
   __device__ void zero() { v = make_uint4(0u, 0u, 0u, 0u); }
   __device__ void get(float (&k)[4]) const {
     k[0] = kval(v.x); k[1] = kval(v.y); k[2] = kval(v.z); k[3] = kval(v.w);
@@ -379,7 +402,10 @@ template <> struct RawQuad<uint32_t> {
 };
 template <> struct RawQuad<uint16_t> {  // four bf16 in 8 bytes
   uint2 v;
-  __device__ void load(const uint16_t *p) { v = *reinterpret_cast<const uint2 *>(p); }
+  template <bool NT> __device__ void load(const uint16_t *p) {
+    const nu2 t = ld_stream<NT>(reinterpret_cast<const nu2 *>(p));
+    v = make_uint2(t.x, t.y);
+  }
   __device__ void zero() { v = make_uint2(0u, 0u); }
   __device__ void get(float (&k)[4]) const {
     k[0] = __uint_as_float(v.x << 16); k[1] = __uint_as_float(v.x & 0xffff0000u);
@@ -388,9 +414,11 @@ template <> struct RawQuad<uint16_t> {  // four bf16 in 8 bytes
 };
 template <> struct RawQuad<double> {
   double2 a, b;
-  __device__ void load(const double *p) {
-    a = reinterpret_cast<const double2 *>(p)[0];
-    b = reinterpret_cast<const double2 *>(p)[1];
+  template <bool NT> __device__ void load(const double *p) {
+    const nd2 x = ld_stream<NT>(reinterpret_cast<const nd2 *>(p));
+    const nd2 y = ld_stream<NT>(reinterpret_cast<const nd2 *>(p) + 1);
+    a = make_double2(x.x, x.y);
+    b = make_double2(y.x, y.y);
   }
   __device__ void zero() { a = b = make_double2(0.0, 0.0); }
   __device__ void get(double (&k)[4]) const { k[0] = a.x; k[1] = a.y; k[2] = b.x; k[3] = b.y; }
@@ -416,7 +444,7 @@ constexpr int stage_quads() {
 // read back with vector LDS loads broadcast across the lanes of a quad.
 // Slices are summed in a fixed order through LDS, so results are
 // reproducible bit for bit.
-template <typename TK, typename TA, typename TC, int R, int WPB, int UNR, bool CLAMP>
+template <typename TK, typename TA, typename TC, int R, int WPB, int UNR, bool CLAMP, bool NT>
 __global__ __launch_bounds__(64 * WPB) void k_apply(int64_t n_box,
                                                           const int32_t *__restrict__ order,
                                                           const int64_t *__restrict__ var,
@@ -486,7 +514,7 @@ __global__ __launch_bounds__(64 * WPB) void k_apply(int64_t n_box,
           const int jmax = max(lim - 1, 0);
 #pragma unroll
           for (int u = 0; u < APPLY_UNROLL; u++)
-            kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * min(s + sl * (k0 + u), jmax)));
+            kv[u].template load<NT>(Kc + (uint32_t)(FCG_QUAD * db * min(s + sl * (k0 + u), jmax)));
 #pragma unroll
           for (int u = 0; u < APPLY_UNROLL; u++)
             if (s + sl * (k0 + u) >= lim) kv[u].zero();
@@ -494,7 +522,7 @@ __global__ __launch_bounds__(64 * WPB) void k_apply(int64_t n_box,
 #pragma unroll
           for (int u = 0; u < APPLY_UNROLL; u++) {
             const int j = s + sl * (k0 + u);
-            if (j < lim) kv[u].load(Kc + (uint32_t)(FCG_QUAD * db * j));
+            if (j < lim) kv[u].template load<NT>(Kc + (uint32_t)(FCG_QUAD * db * j));
             else kv[u].zero();
           }
         }
@@ -539,7 +567,7 @@ __global__ __launch_bounds__(64 * WPB) void k_apply(int64_t n_box,
 // instruction (1.45x over quad-per-lane gathers on the 2e6-configuration
 // rank-1 space).  Rows come from `order` (XCD-grouped slots, -1 = empty);
 // the 64 lane sums are combined by a fixed butterfly.
-template <typename TK, typename TA, typename TC, int R, int U>
+template <typename TK, typename TA, typename TC, int R, int U, bool NT>
 __global__ __launch_bounds__(64 * FCG_CSR_WPB) void k_apply_rows(
     int64_t n_slots, const int32_t *__restrict__ order, const int64_t *__restrict__ var,
     const int64_t *__restrict__ row_koff, const int64_t *__restrict__ col_ptr,
@@ -567,8 +595,8 @@ __global__ __launch_bounds__(64 * FCG_CSR_WPB) void k_apply_rows(
     for (int e = 0; e < NE; e++) {
       const int j = s0 + 64 * e;
       if (j + lane < ncol) {
-        kv[e] = Kl[j];
-        idx[e] = cl[j];
+        kv[e] = ld_stream<NT>(Kl + j);
+        idx[e] = ld_stream<NT>(cl + j);
       } else {
         kv[e] = (TK)0;
         idx[e] = 0;
@@ -712,7 +740,8 @@ hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {
 template <typename TK, typename TA, typename TC, int R, int V>
 void launch_multi_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma, hipStream_t s) {
   constexpr int W = kMulti[V].wpb, U = kMulti[V].unroll;
-  hipLaunchKernelGGL((k_apply<TK, TA, TC, R, W, U, apply_clamp<TK>()>),
+  constexpr bool NT = kMulti[V].nt;
+  hipLaunchKernelGGL((k_apply<TK, TA, TC, R, W, U, apply_clamp<TK>(), NT>),
                      dim3((unsigned)((pl->n_multi + W - 1) / W)), dim3(64 * W), 0, s, pl->n_multi,
                      pl->d_order, pl->sp->d_var, pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, K, c,
                      pl->sp->nvar, sigma);
@@ -720,7 +749,7 @@ void launch_multi_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma,
 
 template <typename TK, typename TA, typename TC, int R, int V>
 void launch_rows_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma, hipStream_t s) {
-  hipLaunchKernelGGL((k_apply_rows<TK, TA, TC, R, kRowsUnroll[V]>),
+  hipLaunchKernelGGL((k_apply_rows<TK, TA, TC, R, kRows[V].unroll, kRows[V].nt>),
                      dim3((unsigned)(pl->n_single_slots / FCG_CSR_WPB)), dim3(64 * FCG_CSR_WPB), 0,
                      s, pl->n_single_slots, pl->d_order + pl->n_multi, pl->sp->d_var,
                      pl->d_row_koff, pl->d_col_ptr, pl->d_colidx, K, c, pl->sp->nvar, sigma);
@@ -734,7 +763,9 @@ void launch_r(const fcg_plan *pl, int vm, int vr, int parts, const TK *K, const 
     if constexpr (R <= TUNED_MAX_R) {
       if (vm == 0) launch_multi_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
       else if (vm == 1) launch_multi_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
-      else launch_multi_v<TK, TA, TC, R, 2>(pl, K, c, sigma, s);
+      else if (vm == 2) launch_multi_v<TK, TA, TC, R, 2>(pl, K, c, sigma, s);
+      else if (vm == 3) launch_multi_v<TK, TA, TC, R, 3>(pl, K, c, sigma, s);
+      else launch_multi_v<TK, TA, TC, R, 4>(pl, K, c, sigma, s);
     } else {
       launch_multi_v<TK, TA, TC, R, default_multi<TK>()>(pl, K, c, sigma, s);
     }
@@ -742,7 +773,9 @@ void launch_r(const fcg_plan *pl, int vm, int vr, int parts, const TK *K, const 
   if ((parts & 2) && pl->n_single_slots > 0) {
     if constexpr (R <= TUNED_MAX_R) {
       if (vr == 0) launch_rows_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
-      else launch_rows_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
+      else if (vr == 1) launch_rows_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
+      else if (vr == 2) launch_rows_v<TK, TA, TC, R, 2>(pl, K, c, sigma, s);
+      else launch_rows_v<TK, TA, TC, R, 3>(pl, K, c, sigma, s);
     } else {
       launch_rows_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
     }

@@ -72,7 +72,7 @@ def test_variants_identical(eng, kind, R):
         if prec == "fp64" and ref is not None:
             assert np.abs(tuned - ref).max() < 1e-12 * np.abs(ref).max()
         nm, nr, max_r = _lib.apply_variant_counts()
-        assert (nm, nr, max_r) == (3, 2, 4)
+        assert (nm, nr, max_r) == (5, 4, 4)
         pairs = [(m, r) for m in range(nm) for r in range(nr)] if R <= max_r else [(vm, vr)]
         for m, r in pairs:
             pl.set_apply_variant(prec, R, m, r)
