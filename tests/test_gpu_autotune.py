@@ -4,7 +4,7 @@ and the loads in flight, never the order of a sum).
 
 Cases: a full-block TPSCI space (multi-row boxes), a thinned one (one-row
 boxes on the sparse plan), a space with both, and a rank-1 space of 39,313
-rows (enough one-row slot groups for the persistent-grid variants to loop),
+rows (many one-row boxes per workgroup for every one-row variant),
 at FP64, FP32, BF16X2 and BF16, R = 2 and R = 6 (above the tuned root count,
 where only the default exists and pinning anything else is refused).  The
 FP64 result of the small cases is also checked against the oracle at 1e-12
