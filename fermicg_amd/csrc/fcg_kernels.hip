@@ -58,6 +58,9 @@ __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int
 
 constexpr int BT = 256;                       // threads of the build kernel
 constexpr int KPT = FCG_MAX_BOX * FCG_MAX_BOX / BT;  // block elements per thread
+static_assert(2 * FCG_MAX_BOX <= BT, "one thread per bra row and ket column");
+static_assert(FCG_MAX_BOX <= 256, "state digits are bytes");
+static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL, "fused sigma stages c and y in LDS");
 
 // LX / LT / LS: LDS capacities (doubles) of the chain intermediates and Γ
 // slices, the factor-compression rows and the overlaps; the host puts each
@@ -73,6 +76,13 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
   __shared__ int64_t sFB[FCG_MAXN], sFK[FCG_MAXN];
   __shared__ int sAct[FCG_MAXN];
   __shared__ int sStride[FCG_MAXN];
+  // per pair: each bra row's / ket column's state digit per cluster, and the
+  // screening threshold of each ket column; per task: each row's and column's
+  // share of the chain-output index and the key of its inactive digits
+  __shared__ uint8_t sDigB[FCG_MAX_BOX * FCG_MAXN], sDigK[FCG_MAX_BOX * FCG_MAXN];
+  __shared__ int2 sRowT[FCG_MAX_BOX], sColT[FCG_MAX_BOX];
+  __shared__ double sThr[FCG_MAX_BOX];
+  __shared__ int sFast;
 
   const int tid = threadIdx.x;
   const int64_t p = a.pair_list[a.pair_base + blockIdx.x];
@@ -114,6 +124,21 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
       sS[sSoff[c] + e] = s;
     }
   }
+  if (tid < db + dk) {  // mixed-radix digits, cluster 0 fastest
+    const bool row = tid < db;
+    int x = row ? tid : tid - db;
+    uint8_t *dg = row ? sDigB + x * N : sDigK + x * N;
+    for (int c = 0; c < N; c++) {
+      const int rc = row ? sRB[c] : sRK[c];
+      dg[c] = (uint8_t)(x % rc);
+      x /= rc;
+    }
+  }
+  if (a.thr)
+    for (int q = tid; q < dk; q += BT) sThr[q] = a.thr[a.kvar[K] + q];
+  // this thread's block elements e = tid + BT k (column-major, db rows) are
+  // visited as (row r, column q), advanced by (dr, dq) per step
+  const int r0 = tid % db, q0 = tid / db, dr = BT % db, dq = BT / db;
   double acc[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; k++) acc[k] = 0.0;
@@ -155,6 +180,42 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
         sStride[c] = st;
         st *= sRB[c] * sRK[c];
       }
+      // the separable form needs every inactive cluster to be a Kronecker
+      // delta of absolute state indices (identity factors on both sides)
+      int fast = 1;
+      for (int c = 0; c < N; c++)
+        if (sAct[c] < 0 && sKind[c] != 0) fast = 0;
+      sFast = fast;
+    }
+    if (tid < db + dk) {
+      // element (r, q) of the task's term reads cur[rowoff[r] + coloff[q]]
+      // and is nonzero when rowkey[r] == colkey[q]: the inactive clusters'
+      // bra states shifted to ket numbering, in the ket digits' mixed radix
+      // (-1 when one falls outside the ket box)
+      const bool row = tid < db;
+      const int x = row ? tid : tid - db;
+      const uint8_t *dg = row ? sDigB + x * N : sDigK + x * N;
+      int off = 0, st = 1;
+      for (int s = 0; s < nb; s++) {
+        const int c = tm.cl[ord[s]];
+        off += (row ? dg[c] : sRB[c] * dg[c]) * st;
+        st *= sRB[c] * sRK[c];
+      }
+      int key = 0, w = 1;
+      bool out = false;
+      for (int c = 0; c < N; c++) {
+        bool act = false;
+        for (int s = 0; s < nb; s++) act |= tm.cl[s] == c;
+        if (act) continue;
+        int v = dg[c];
+        if (row) {
+          v += sLoB[c] - sLoK[c];
+          out |= v < 0 || v >= sRK[c];
+        }
+        key += v * w;
+        w *= sRK[c];
+      }
+      (row ? sRowT[x] : sColT[x]) = make_int2(off, out ? -1 : key);
     }
     for (int s = 0; s < nb; s++) {
       const int tp = ord[s];
@@ -218,51 +279,82 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
     }
     // Kronecker accumulate into this thread's block elements
     const double sign = (td.meta & 1) ? -1.0 : 1.0;
+    if (sFast) {
+      int r = r0, q = q0;
 #pragma unroll
-    for (int k = 0; k < KPT; k++) {
-      const int e = tid + BT * k;
-      if (e >= db * dk) break;
-      int r = e % db, q = e / db;
-      double val = sign;
-      int opidx = 0;
-      for (int c = 0; c < N; c++) {
-        const int rb = sRB[c], rk = sRK[c];
-        const int ic = r % rb, jc = q % rk;
-        r /= rb; q /= rk;
-        if (sAct[c] >= 0) {
-          opidx += (ic + rb * jc) * sStride[c];
-        } else if (sKind[c] == 0) {
-          if (sLoB[c] + ic != sLoK[c] + jc) { val = 0.0; break; }
-        } else if (sKind[c] == 1) {
-          val *= sS[sSoff[c] + ic + rb * jc];
-        } else {
-          val = 0.0;
-          break;
+      for (int k = 0; k < KPT; k++) {
+        if (q >= dk) break;
+        const int2 R = sRowT[r], C = sColT[q];
+        if (R.y == C.y) {
+          const double x = cur[R.x + C.x];
+          if (!a.thr || fabs(x) > sThr[q]) acc[k] += sign * x;
         }
+        r += dr;
+        q += dq;
+        if (r >= db) { r -= db; q++; }
       }
-      if (val != 0.0) {
-        const double x = cur[opidx];
-        if (!a.thr || fabs(x) > a.thr[a.kvar[K] + e / db]) acc[k] += val * x;
+    } else {
+      int r = r0, q = q0;
+#pragma unroll
+      for (int k = 0; k < KPT; k++) {
+        if (q >= dk) break;
+        const uint8_t *dB = sDigB + r * N, *dK = sDigK + q * N;
+        double val = sign;
+        int opidx = 0;
+        for (int c = 0; c < N; c++) {
+          const int ic = dB[c], jc = dK[c];
+          if (sAct[c] >= 0) {
+            opidx += (ic + sRB[c] * jc) * sStride[c];
+          } else if (sKind[c] == 0) {
+            if (sLoB[c] + ic != sLoK[c] + jc) { val = 0.0; break; }
+          } else if (sKind[c] == 1) {
+            val *= sS[sSoff[c] + ic + sRB[c] * jc];
+          } else {
+            val = 0.0;
+            break;
+          }
+        }
+        if (val != 0.0) {
+          const double x = cur[opidx];
+          if (!a.thr || fabs(x) > sThr[q]) acc[k] += val * x;
+        }
+        r += dr;
+        q += dq;
+        if (r >= db) { r -= db; q++; }
       }
     }
     __syncthreads();
   }
   if (a.fsig) {
     // fused sigma: the block's products summed per bra row in LDS (the chain
-    // buffer is free now), then one global add per (row, root)
-    double *sY = sX[0];
-    const int nY = db * a.fR;
-    for (int e = tid; e < nY; e += BT) sY[e] = 0.0;
-    __syncthreads();
+    // buffers are free now; the ket box's c staged beside them), a thread's
+    // run of elements in one row added once, then one global add per
+    // (row, root)
+    double *sY = sX[0], *sC = sX[1];
+    const int fR = a.fR, nY = db * fR;
     const int64_t kv0 = a.kvar[K];
+    for (int e = tid; e < nY; e += BT) sY[e] = 0.0;
+    for (int e = tid; e < dk * fR; e += BT)
+      sC[e] = a.fc[(int64_t)(e / dk) * a.nk_var + kv0 + e % dk];
+    __syncthreads();
+    for (int rr = 0; rr < fR; rr++) {
+      const double *cq = sC + rr * dk;
+      int r = r0, q = q0, rc = r0;
+      double run = 0.0;
 #pragma unroll
-    for (int k = 0; k < KPT; k++) {
-      const int e = tid + BT * k;
-      if (e >= db * dk) break;
-      if (acc[k] == 0.0) continue;
-      const int r = e % db, q = e / db;
-      for (int rr = 0; rr < a.fR; rr++)
-        atomicAdd(&sY[r * a.fR + rr], acc[k] * a.fc[(int64_t)rr * a.nk_var + kv0 + q]);
+      for (int k = 0; k < KPT; k++) {
+        if (q >= dk) break;
+        if (r != rc) {
+          if (run != 0.0) atomicAdd(&sY[rc * fR + rr], run);
+          run = 0.0;
+          rc = r;
+        }
+        run += acc[k] * cq[q];
+        r += dr;
+        q += dq;
+        if (r >= db) { r -= db; q++; }
+      }
+      if (run != 0.0) atomicAdd(&sY[rc * fR + rr], run);
     }
     __syncthreads();
     const int64_t bv0 = a.bvar[B];
