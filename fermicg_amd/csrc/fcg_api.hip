@@ -398,7 +398,8 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
     return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
   }
   HostPlan hp;
-  int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, &hp, &ctx->err);
+  const bool store = fused == nullptr;
+  int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, store, &hp, &ctx->err);
   if (rc) return rc;
   int64_t reserved = 0;
   {
@@ -430,19 +431,18 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
       return rc_;               \
     }                           \
   } while (0)
-  PTRY(dev_upload(ctx, &pl->d_row_ptr, hp.row_ptr.data(), sp->n_box + 1));
   PTRY(dev_upload(ctx, &pl->d_pair_B, hp.pair_B.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_pair_K, hp.pair_K.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_pair_t0, hp.pair_t0.data(), pl->n_pair + 1));
-  PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
-  PTRY(dev_upload(ctx, &pl->d_pair_col, hp.pair_col.data(), pl->n_pair));
   PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
   PTRY(dev_upload(ctx, &pl->d_pair_list, hp.pair_list.data(), pl->n_pair));
   pl->n_small = hp.n_small;
   pl->n_mid = hp.n_mid;
-  const bool store = fused == nullptr;
   pl->holds_blocks = store;
   if (store) {  // the apply's layout: only for a plan that keeps its blocks
+    PTRY(dev_upload(ctx, &pl->d_row_ptr, hp.row_ptr.data(), sp->n_box + 1));
+    PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
+    PTRY(dev_upload(ctx, &pl->d_pair_col, hp.pair_col.data(), pl->n_pair));
     PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
     PTRY(dev_upload(ctx, &pl->d_diag_col, hp.diag_col.data(), sp->n_box));
     PTRY(dev_upload(ctx, &pl->d_row_koff, hp.row_koff.data(), sp->n_box));

@@ -3,8 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/fcg.h"
@@ -166,25 +169,55 @@ struct fcg_plan {
 };
 
 // host plan construction (fcg_plan_host.cpp)
+// Allocator whose resize() leaves new trivial elements unwritten (default-
+// instead of value-initialised): the host plan's large tables are written in
+// full by parallel passes, so a serial zero fill first would only cost time.
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = DefaultInitAlloc<U>;
+  };
+  DefaultInitAlloc() = default;
+  template <class U>
+  DefaultInitAlloc(const DefaultInitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a) {
+    ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using RawVec = std::vector<T, DefaultInitAlloc<T>>;
+
 struct HostPlan {
-  std::vector<int32_t> pair_B, pair_K;
-  std::vector<int64_t> pair_t0, pair_koff, row_ptr, diag_koff, row_koff, col_ptr;
-  std::vector<int32_t> pair_col, diag_col, colidx, order;
+  RawVec<int32_t> pair_B, pair_K, pair_col, colidx;
+  RawVec<int64_t> pair_t0, pair_koff;
+  std::vector<int64_t> row_ptr, diag_koff, row_koff, col_ptr;
+  std::vector<int32_t> diag_col, order;
   int64_t n_multi = 0, n_single_slots = 0;
-  std::vector<TaskDev> tasks;
+  RawVec<TaskDev> tasks;
   // pair indices for the block build: pairs within the small LDS limits
   // first (n_small of them), then those within the middle limits (n_mid),
   // then the rest, each in pair order
-  std::vector<int64_t> pair_list;
+  RawVec<int64_t> pair_list;
   int64_t n_small = 0, n_mid = 0;
+  // stored elements: panel layout included (layout), else sum of db * dk
   int64_t n_kel = 0;
   double flops = 0.0;
 };
 // apply schedule of a plan (see fcg_plan::d_order)
 void fcg_apply_order(const std::vector<int64_t> &var_off, const std::vector<int64_t> &col_ptr,
                      std::vector<int32_t> *order, int64_t *n_multi, int64_t *n_single_slots);
+// layout = false: pairs, tasks and LDS classes only (a plan whose blocks are
+// used once during the build); pair_col, pair_koff, the row panels, colidx,
+// the apply order and the diagonal tables are left empty
 int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
-                        const fcg_space *ket, int nbody, HostPlan *out, std::string *err);
+                        const fcg_space *ket, int nbody, bool layout, HostPlan *out,
+                        std::string *err);
 int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
                     const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
                     fcg_space *sp, std::string *err);

@@ -64,9 +64,10 @@ static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL, "fused sigma stages c 
 
 // LX / LT / LS: LDS capacities (doubles) of the chain intermediates and Γ
 // slices, the factor-compression rows and the overlaps; the host puts each
-// pair in the smallest class that holds it
-template <int LX, int LT, int LS>
-__global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
+// pair in the smallest class that holds it.  MINB: workgroups per CU the
+// class's LDS allows (the register budget follows)
+template <int LX, int LT, int LS, int MINB>
+__global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
   __shared__ double sX[2][LX];
   __shared__ double sG[LX];
   __shared__ double sT[LT];
@@ -281,7 +282,6 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
     const double sign = (td.meta & 1) ? -1.0 : 1.0;
     if (sFast) {
       int r = r0, q = q0;
-#pragma unroll
       for (int k = 0; k < KPT; k++) {
         if (q >= dk) break;
         const int2 R = sRowT[r], C = sColT[q];
@@ -295,7 +295,6 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
       }
     } else {
       int r = r0, q = q0;
-#pragma unroll
       for (int k = 0; k < KPT; k++) {
         if (q >= dk) break;
         const uint8_t *dB = sDigB + r * N, *dK = sDigK + q * N;
@@ -341,7 +340,6 @@ __global__ __launch_bounds__(BT, 3) void k_build_blocks(BuildArgs a) {
       const double *cq = sC + rr * dk;
       int r = r0, q = q0, rc = r0;
       double run = 0.0;
-#pragma unroll
       for (int k = 0; k < KPT; k++) {
         if (q >= dk) break;
         if (r != rc) {
@@ -807,13 +805,13 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
     const int64_t end = std::min(base + chunk, lim);
     const int64_t n = end - base;
     if (base < mid0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL, 3>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else if (base < full0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID, 2>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S>), dim3((unsigned)n),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S, 1>), dim3((unsigned)n),
                          dim3(BT), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

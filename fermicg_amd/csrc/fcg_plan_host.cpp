@@ -219,7 +219,8 @@ static int check_space(const fcg_ops *o, const fcg_space *sp, std::string *err) 
 }
 
 int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
-                        const fcg_space *ket, int nbody, HostPlan *out, std::string *err) {
+                        const fcg_space *ket, int nbody, bool layout, HostPlan *out,
+                        std::string *err) {
   const int N = bra->N, S = o->S;
   if (h->N != N || o->N != N || ket->N != N) {
     *err = "cluster counts of ham/ops/spaces differ";
@@ -356,7 +357,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     pair_off[fb + 1] += pair_off[fb];
     task_off[fb + 1] += task_off[fb];
   }
-  std::vector<PairRec> pairs(pair_off[nfb]);
+  RawVec<PairRec> pairs(pair_off[nfb]);
   out->tasks.resize(task_off[nfb]);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t fb = 0; fb < nfb; fb++) {
@@ -378,7 +379,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     // bra boxes not grouped by Fock block: sort the pairs, then regroup
     // their tasks in the new order
     std::stable_sort(pairs.begin(), pairs.end(), pair_less);
-    std::vector<TaskDev> moved(out->tasks.size());
+    RawVec<TaskDev> moved(out->tasks.size());
     int64_t at = 0;
     for (PairRec &r : pairs) {
       const int64_t n = r.t1 - r.t0;
@@ -389,167 +390,229 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     }
     out->tasks.swap(moved);
   }
-  // per-task chain order; limits; flop count
-  const int64_t np = (int64_t)pairs.size();
-  out->pair_B.resize(np); out->pair_K.resize(np); out->pair_t0.assign(np + 1, 0);
-  out->pair_koff.resize(np);
-  out->pair_col.resize(np);
-  out->row_ptr.assign(bra->n_box + 1, 0);
-  out->diag_koff.assign(bra->n_box, -1);
-  out->diag_col.assign(bra->n_box, 0);
-  std::vector<int64_t> row_cols(bra->n_box, 0);
-  // serial pass: pair layout (panel columns run along each bra row)
+  // pair tables (tasks are contiguous in pair order, so a pair's first task
+  // is its t0); row b of the bra holds pairs [row_ptr[b], row_ptr[b + 1])
+  const int64_t np = (int64_t)pairs.size(), nbb = bra->n_box;
+  out->pair_B.resize(np);
+  out->pair_K.resize(np);
+  out->pair_t0.resize(np + 1);
+#pragma omp parallel for schedule(static)
   for (int64_t p = 0; p < np; p++) {
-    const PairRec &r = pairs[p];
-    out->pair_B[p] = r.B;
-    out->pair_K[p] = r.K;
-    out->pair_t0[p + 1] = out->pair_t0[p] + (r.t1 - r.t0);
-    const int64_t dk = dim_k(r.K);
-    if (row_cols[r.B] + dk > INT32_MAX / FCG_MAX_BOX) {
-      *err = "row panel too wide";
-      return FCG_EOVERFLOW;
-    }
-    out->pair_col[p] = (int32_t)row_cols[r.B];
-    if (ket == bra && r.B == r.K) out->diag_col[r.B] = (int32_t)row_cols[r.B];
-    row_cols[r.B] += dk;
-    out->row_ptr[r.B + 1]++;
+    out->pair_B[p] = pairs[p].B;
+    out->pair_K[p] = pairs[p].K;
+    out->pair_t0[p] = pairs[p].t0;
   }
+  out->pair_t0[np] = np ? pairs[np - 1].t1 : 0;
+  out->row_ptr.assign(nbb + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b <= nbb; b++)
+    out->row_ptr[b] =
+        std::lower_bound(out->pair_B.begin(), out->pair_B.end(), (int32_t)b) - out->pair_B.begin();
   // parallel pass: each task's chain order, LDS limits, flops
   double flops = 0.0;
+  int64_t elems = 0;
   std::atomic<int> lim_err{FCG_OK};
   const char *lim_msg = nullptr;
-  std::vector<uint8_t> lds_class(np, 0);  // 2 small, 1 middle, 0 full
-#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : flops)
-  for (int64_t p = 0; p < np; p++) {
-    if (lim_err) continue;
-    const PairRec &r = pairs[p];
-    auto fail = [&](const char *m) {
+  RawVec<uint8_t> lds_class(np);  // 2 small, 1 middle, 0 full
+#pragma omp parallel reduction(+ : flops, elems)
+  {
+    // the previous task's chain order, reused while its shape repeats (the
+    // tasks of one pair share their clusters, so mostly it does)
+    int m_nb = -1;
+    int64_t m_np[4] = {0, 0, 0, 0}, m_rr[4] = {0, 0, 0, 0}, m_peak = 0;
+    int m_best[4] = {0, 1, 2, 3};
+    double m_fl = 0.0;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t p = 0; p < np; p++) {
+      if (lim_err) continue;
+      const PairRec &r = pairs[p];
+      auto fail = [&](const char *m) {
 #pragma omp critical
-      {
-        if (!lim_err) {
-          lim_err = FCG_EOVERFLOW;
-          lim_msg = m;
+        {
+          if (!lim_err) {
+            lim_err = FCG_EOVERFLOW;
+            lim_msg = m;
+          }
         }
+      };
+      const int64_t db = dim_b(r.B), dk = dim_k(r.K);
+      const int64_t B = r.B, K = r.K;
+      elems += db * dk;
+      int64_t ssum = 0;
+      for (int c = 0; c < N; c++)
+        if (bra->lo[B * N + c] == ket->lo[K * N + c] && bra->hi[B * N + c] == ket->hi[K * N + c] &&
+            !(bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0))
+          ssum += (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
+      if (ssum > FCG_LDS_S) {
+        fail("inactive-cluster overlaps exceed LDS budget");
+        continue;
       }
-    };
-    const int64_t db = dim_b(r.B), dk = dim_k(r.K);
-    const int64_t B = r.B, K = r.K;
-    int64_t ssum = 0;
-    for (int c = 0; c < N; c++)
-      if (bra->lo[B * N + c] == ket->lo[K * N + c] && bra->hi[B * N + c] == ket->hi[K * N + c] &&
-          !(bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0))
-        ssum += (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
-    if (ssum > FCG_LDS_S) {
-      fail("inactive-cluster overlaps exceed LDS budget");
-      continue;
+      // largest LDS use of the pair's tasks: chain intermediates and staged
+      // Γ slices (X), factor-compression rows (T), overlaps (S)
+      int64_t need_x = 0, need_t = 0;
+      for (int64_t t = r.t0; t < r.t1; t++) {
+        TaskDev &d = out->tasks[t];
+        const int tm = d.term, nb = h->nbody[tm];
+        int64_t np_[4] = {0, 0, 0, 0}, rr_[4] = {0, 0, 0, 0};
+        int64_t rest0 = 1;
+        const char *lds_err = nullptr;
+        for (int a = 0; a < nb; a++) {
+          const int c = h->clusters[4 * tm + a];
+          np_[a] = (nb == 1) ? 1 : h->dims[4 * tm + a];
+          rr_[a] = (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
+          rest0 *= np_[a];
+          const int64_t nI = bra->hi[B * N + c] - bra->lo[B * N + c] + 1;
+          const int64_t nJ = ket->hi[K * N + c] - ket->lo[K * N + c] + 1;
+          const int64_t rk = ket->rank[K * N + c];
+          if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) lds_err = "Γ slice exceeds LDS budget";
+          need_x = std::max(need_x, np_[a] * rr_[a]);
+          need_t = std::max(need_t, nI * rk);
+          const bool ident = bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0;
+          if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * bra->rank[B * N + c] * rk);
+        }
+        if (lds_err) {
+          fail(lds_err);
+          break;
+        }
+        if (rest0 > FCG_LDS_X) {
+          fail("term integrals exceed LDS budget");
+          break;
+        }
+        // chain order: the permutation with the smallest peak intermediate
+        // (ties: fewest flops)
+        if (nb != m_nb || !std::equal(np_, np_ + 4, m_np) || !std::equal(rr_, rr_ + 4, m_rr)) {
+          int perm[4] = {0, 1, 2, 3};
+          int64_t best_peak = INT64_MAX;
+          double best_fl = 0.0;
+          do {
+            int64_t rest = rest0, peak = rest0;
+            double fl = 0.0;
+            for (int s2 = 0; s2 < nb; s2++) {
+              const int a = perm[s2];
+              rest = rest / np_[a] * rr_[a];
+              fl += 2.0 * np_[a] * (double)rest;
+              peak = std::max(peak, rest);
+            }
+            if (peak < best_peak || (peak == best_peak && fl < best_fl)) {
+              best_peak = peak;
+              best_fl = fl;
+              std::copy(perm, perm + 4, m_best);
+            }
+          } while (std::next_permutation(perm, perm + nb));
+          m_nb = nb;
+          std::copy(np_, np_ + 4, m_np);
+          std::copy(rr_, rr_ + 4, m_rr);
+          m_peak = best_peak;
+          m_fl = best_fl;
+        }
+        if (m_peak > FCG_LDS_X) {
+          fail("chain intermediate exceeds LDS budget");
+          break;
+        }
+        need_x = std::max(need_x, std::max(m_peak, rest0));
+        for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? m_best[s2] : 0) << (8 + 4 * s2);
+        flops += m_fl + 2.0 * db * dk;
+      }
+      lds_class[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
+                     ? 2
+                     : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
     }
-    // largest LDS use of the pair's tasks: chain intermediates and staged
-    // Γ slices (X), factor-compression rows (T), overlaps (S)
-    int64_t need_x = 0, need_t = 0;
-    for (int64_t t = r.t0; t < r.t1; t++) {
-      TaskDev &d = out->tasks[t];
-      const int tm = d.term, nb = h->nbody[tm];
-      int64_t np_[4], rr_[4];
-      int64_t rest0 = 1;
-      const char *lds_err = nullptr;
-      for (int a = 0; a < nb; a++) {
-        const int c = h->clusters[4 * tm + a];
-        np_[a] = (nb == 1) ? 1 : h->dims[4 * tm + a];
-        rr_[a] = (int64_t)bra->rank[B * N + c] * ket->rank[K * N + c];
-        rest0 *= np_[a];
-        const int64_t nI = bra->hi[B * N + c] - bra->lo[B * N + c] + 1;
-        const int64_t nJ = ket->hi[K * N + c] - ket->lo[K * N + c] + 1;
-        const int64_t rk = ket->rank[K * N + c];
-        if (np_[a] * rr_[a] > FCG_LDS_X || nI * rk > FCG_LDS_T) lds_err = "Γ slice exceeds LDS budget";
-        need_x = std::max(need_x, np_[a] * rr_[a]);
-        need_t = std::max(need_t, nI * rk);
-        const bool ident = bra->foff[B * N + c] < 0 && ket->foff[K * N + c] < 0;
-        if (!ident) flops += 2.0 * np_[a] * (double)(nI * nJ * rk + nI * bra->rank[B * N + c] * rk);
-      }
-      if (lds_err) {
-        fail(lds_err);
-        break;
-      }
-      if (rest0 > FCG_LDS_X) {
-        fail("term integrals exceed LDS budget");
-        break;
-      }
-      // chain order: the permutation with the smallest peak intermediate
-      int perm[4] = {0, 1, 2, 3}, best[4] = {0, 1, 2, 3};
-      int64_t best_peak = INT64_MAX;
-      double best_fl = 0.0;
-      do {
-        int64_t rest = rest0, peak = rest0;
-        double fl = 0.0;
-        for (int s2 = 0; s2 < nb; s2++) {
-          const int a = perm[s2];
-          rest = rest / np_[a] * rr_[a];
-          fl += 2.0 * np_[a] * (double)rest;
-          peak = std::max(peak, rest);
-        }
-        if (peak < best_peak || (peak == best_peak && fl < best_fl)) {
-          best_peak = peak;
-          best_fl = fl;
-          std::copy(perm, perm + 4, best);
-        }
-      } while (std::next_permutation(perm, perm + nb));
-      if (best_peak > FCG_LDS_X) {
-        fail("chain intermediate exceeds LDS budget");
-        break;
-      }
-      need_x = std::max(need_x, std::max(best_peak, rest0));
-      for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? best[s2] : 0) << (8 + 4 * s2);
-      flops += best_fl + 2.0 * db * dk;
-    }
-    lds_class[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
-                   ? 2
-                   : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
   }
   if (lim_err.load()) {
     *err = lim_msg;
     return lim_err.load();
   }
-  // LDS classes: 2 small, 1 middle, 0 full
-  out->pair_list.resize(np);
-  int64_t at = 0;
-  for (int cls = 2; cls >= 0; cls--) {
-    const int64_t start = at;
-    for (int64_t p = 0; p < np; p++)
-      if (lds_class[p] == cls) out->pair_list[at++] = p;
-    if (cls == 2) out->n_small = at - start;
-    if (cls == 1) out->n_mid = at - start;
+  // LDS classes, each in pair order: 2 small, 1 middle, 0 full (a stable
+  // partition over per-thread chunks)
+  {
+    const int nth = std::max(1, omp_get_max_threads());
+    const int64_t chunk = (np + nth - 1) / nth;
+    std::vector<int64_t> cnt(3 * (size_t)nth, 0);
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < nth; t++)
+      for (int64_t p = t * chunk; p < std::min(np, (t + 1) * chunk); p++) cnt[lds_class[p] * nth + t]++;
+    int64_t at = 0;
+    for (int cls = 2; cls >= 0; cls--) {
+      const int64_t start = at;
+      for (int t = 0; t < nth; t++) {
+        const int64_t n = cnt[cls * nth + t];
+        cnt[cls * nth + t] = at;
+        at += n;
+      }
+      if (cls == 2) out->n_small = at - start;
+      if (cls == 1) out->n_mid = at - start;
+    }
+    out->pair_list.resize(np);
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < nth; t++) {
+      int64_t pos[3] = {cnt[t], cnt[nth + t], cnt[2 * nth + t]};
+      for (int64_t p = t * chunk; p < std::min(np, (t + 1) * chunk); p++)
+        out->pair_list[pos[lds_class[p]]++] = p;
+    }
   }
-  for (int64_t b = 0; b < bra->n_box; b++) out->row_ptr[b + 1] += out->row_ptr[b];
+  out->flops = flops;
+  if (!layout) {
+    out->n_kel = elems;
+    return FCG_OK;
+  }
   // row panels: the blocks of one bra row side by side form a db x ncol
   // panel (ncol padded to a multiple of FCG_QUAD with zero columns), stored
   // in column quads (panel_index); colidx maps its columns to ket variables
-  out->row_koff.assign(bra->n_box, 0);
-  out->col_ptr.assign(bra->n_box + 1, 0);
+  out->pair_col.resize(np);
+  out->pair_koff.resize(np);
+  out->diag_col.assign(nbb, 0);
+  out->diag_koff.assign(nbb, -1);
+  std::vector<int64_t> row_cols(nbb, 0);
+  std::vector<uint8_t> has_diag(nbb, 0);
+  int wide = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : wide)
+  for (int64_t b = 0; b < nbb; b++) {
+    int64_t w = 0;
+    for (int64_t p = out->row_ptr[b]; p < out->row_ptr[b + 1]; p++) {
+      const int64_t dk = dim_k(out->pair_K[p]);
+      if (w + dk > INT32_MAX / FCG_MAX_BOX) {
+        wide = 1;
+        break;
+      }
+      out->pair_col[p] = (int32_t)w;
+      if (ket == bra && out->pair_K[p] == b) {
+        out->diag_col[b] = (int32_t)w;
+        has_diag[b] = 1;
+      }
+      w += dk;
+    }
+    row_cols[b] = w;
+  }
+  if (wide) {
+    *err = "row panel too wide";
+    return FCG_EOVERFLOW;
+  }
+  out->row_koff.assign(nbb, 0);
+  out->col_ptr.assign(nbb + 1, 0);
   int64_t ncol = 0, koff = 0;
-  for (int64_t b = 0; b < bra->n_box; b++) {
+  for (int64_t b = 0; b < nbb; b++) {
     const int64_t w = (row_cols[b] + FCG_QUAD - 1) / FCG_QUAD * FCG_QUAD;
     out->col_ptr[b] = ncol;
     out->row_koff[b] = koff;
-    if (ket == bra)
-      for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++)
-        if (out->pair_K[q] == b) out->diag_koff[b] = koff;
+    if (has_diag[b]) out->diag_koff[b] = koff;
     ncol += w;
     koff += dim_b(b) * w;
   }
-  out->col_ptr[bra->n_box] = ncol;
+  out->col_ptr[nbb] = ncol;
+#pragma omp parallel for schedule(static)
   for (int64_t p = 0; p < np; p++) out->pair_koff[p] = out->row_koff[out->pair_B[p]];
-  out->colidx.assign(ncol, 0);
+  out->colidx.resize(ncol);
 #pragma omp parallel for schedule(dynamic, 64)
-  for (int64_t b = 0; b < bra->n_box; b++) {
+  for (int64_t b = 0; b < nbb; b++) {
     int64_t at = out->col_ptr[b];
     for (int64_t q = out->row_ptr[b]; q < out->row_ptr[b + 1]; q++) {
       const int32_t K = out->pair_K[q];
       for (int64_t j = 0; j < dim_k(K); j++) out->colidx[at++] = (int32_t)(ket->var_off[K] + j);
     }
+    for (; at < out->col_ptr[b + 1]; at++) out->colidx[at] = 0;  // padding columns
   }
   fcg_apply_order(bra->var_off, out->col_ptr, &out->order, &out->n_multi, &out->n_single_slots);
   out->n_kel = koff;
-  out->flops = flops;
   return FCG_OK;
 }
 
