@@ -436,6 +436,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   PTRY(dev_upload(ctx, &pl->d_pair_t0, hp.pair_t0.data(), pl->n_pair + 1));
   PTRY(dev_upload(ctx, &pl->d_tasks, hp.tasks.data(), pl->n_task));
   PTRY(dev_upload(ctx, &pl->d_pair_list, hp.pair_list.data(), pl->n_pair));
+  pl->n_tiny = hp.n_tiny;
   pl->n_small = hp.n_small;
   pl->n_mid = hp.n_mid;
   pl->holds_blocks = store;

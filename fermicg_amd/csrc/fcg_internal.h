@@ -23,6 +23,9 @@ constexpr int FCG_LDS_T = 2048;       // factor-compression scratch (doubles)
 constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
 // smaller LDS footprints for the block pairs that fit them (several
 // workgroups per CU instead of one)
+// tiny class: blocks of at most FCG_TINY_ELEMS elements, built by 64 threads
+constexpr int FCG_LDS_X_TINY = 256, FCG_LDS_T_TINY = 128, FCG_LDS_S_TINY = 64;
+constexpr int FCG_TINY_ELEMS = 512;
 constexpr int FCG_LDS_X_SMALL = 1024, FCG_LDS_T_SMALL = 512, FCG_LDS_S_SMALL = 256;
 constexpr int FCG_LDS_X_MID = 2048, FCG_LDS_T_MID = 1024, FCG_LDS_S_MID = 512;
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
@@ -140,10 +143,9 @@ struct fcg_plan {
   int64_t *d_pair_koff = nullptr; // [n_pair] panel of the pair's bra box
   int32_t *d_pair_col = nullptr;  // [n_pair] first panel column of the pair
   TaskDev *d_tasks = nullptr;
-  int64_t *d_pair_list = nullptr;  // build only: pair indices, small-LDS pairs first
-  int64_t n_mid = 0;
+  int64_t *d_pair_list = nullptr;  // build only: pair indices by LDS class, smallest first
+  int64_t n_tiny = 0, n_small = 0, n_mid = 0;
   bool holds_blocks = true;        // false: built with a fused sigma, nothing stored
-  int64_t n_small = 0;
   int64_t *d_diag_koff = nullptr; // [n_box] panel holding the (B,B) block or -1
   int32_t *d_diag_col = nullptr;  // [n_box] its first panel column
   int64_t *d_row_koff = nullptr;  // [n_box] first block of each bra row
@@ -200,11 +202,12 @@ struct HostPlan {
   std::vector<int32_t> diag_col, order;
   int64_t n_multi = 0, n_single_slots = 0;
   RawVec<TaskDev> tasks;
-  // pair indices for the block build: pairs within the small LDS limits
-  // first (n_small of them), then those within the middle limits (n_mid),
+  // pair indices for the block build: pairs within the tiny class's limits
+  // first (n_tiny of them; plans that store their blocks only), then those
+  // within the small LDS limits (n_small), then the middle limits (n_mid),
   // then the rest, each in pair order
   RawVec<int64_t> pair_list;
-  int64_t n_small = 0, n_mid = 0;
+  int64_t n_tiny = 0, n_small = 0, n_mid = 0;
   // stored elements: panel layout included (layout), else sum of db * dk
   int64_t n_kel = 0;
   double flops = 0.0;

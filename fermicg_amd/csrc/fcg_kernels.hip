@@ -58,16 +58,18 @@ __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int
 
 constexpr int BT = 256;                       // threads of the build kernel
 constexpr int KPT = FCG_MAX_BOX * FCG_MAX_BOX / BT;  // block elements per thread
-static_assert(2 * FCG_MAX_BOX <= BT, "one thread per bra row and ket column");
+constexpr int BT_TINY = 64, KPT_TINY = FCG_TINY_ELEMS / BT_TINY;  // the tiny class
 static_assert(FCG_MAX_BOX <= 256, "state digits are bytes");
-static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL, "fused sigma stages c and y in LDS");
+static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL,
+              "fused sigma stages c and y in LDS (fused plans use no tiny class)");
 
 // LX / LT / LS: LDS capacities (doubles) of the chain intermediates and Γ
 // slices, the factor-compression rows and the overlaps; the host puts each
-// pair in the smallest class that holds it.  MINB: workgroups per CU the
-// class's LDS allows (the register budget follows)
-template <int LX, int LT, int LS, int MINB>
-__global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
+// pair in the smallest class that holds it.  MINW: waves per SIMD that the
+// class's LDS allows (sets the register budget).  NT threads hold EPT block
+// elements each (NT * EPT >= the class's largest db * dk).
+template <int LX, int LT, int LS, int MINW, int NT, int EPT>
+__global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
   __shared__ double sX[2][LX];
   __shared__ double sG[LX];
   __shared__ double sT[LT];
@@ -117,7 +119,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
   for (int c = 0; c < N; c++) {
     if (sKind[c] != 1) continue;
     const int rb = sRB[c], rk = sRK[c], nrg = sNB[c];
-    for (int e = tid; e < rb * rk; e += BT) {
+    for (int e = tid; e < rb * rk; e += NT) {
       const int i = e % rb, j = e / rb;
       double s = 0.0;
       for (int x = 0; x < nrg; x++)
@@ -125,9 +127,9 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
       sS[sSoff[c] + e] = s;
     }
   }
-  if (tid < db + dk) {  // mixed-radix digits, cluster 0 fastest
-    const bool row = tid < db;
-    int x = row ? tid : tid - db;
+  for (int x0 = tid; x0 < db + dk; x0 += NT) {  // mixed-radix digits, cluster 0 fastest
+    const bool row = x0 < db;
+    int x = row ? x0 : x0 - db;
     uint8_t *dg = row ? sDigB + x * N : sDigK + x * N;
     for (int c = 0; c < N; c++) {
       const int rc = row ? sRB[c] : sRK[c];
@@ -136,13 +138,13 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
     }
   }
   if (a.thr)
-    for (int q = tid; q < dk; q += BT) sThr[q] = a.thr[a.kvar[K] + q];
-  // this thread's block elements e = tid + BT k (column-major, db rows) are
+    for (int q = tid; q < dk; q += NT) sThr[q] = a.thr[a.kvar[K] + q];
+  // this thread's block elements e = tid + NT k (column-major, db rows) are
   // visited as (row r, column q), advanced by (dr, dq) per step
-  const int r0 = tid % db, q0 = tid / db, dr = BT % db, dq = BT / db;
-  double acc[KPT];
+  const int r0 = tid % db, q0 = tid / db, dr = NT % db, dq = NT / db;
+  double acc[EPT];
 #pragma unroll
-  for (int k = 0; k < KPT; k++) acc[k] = 0.0;
+  for (int k = 0; k < EPT; k++) acc[k] = 0.0;
   __syncthreads();
 
   for (int64_t t = a.pair_t0[p]; t < a.pair_t0[p + 1]; t++) {
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
       ostr[0] = 1;
       for (int s = 1; s < nb; s++) ostr[s] = ostr[s - 1] * tm.dims[s - 1];
       for (int s = 0; s < nb; s++) { pd[s] = tm.dims[ord[s]]; rest *= pd[s]; }
-      for (int e = tid; e < rest; e += BT) {
+      for (int e = tid; e < rest; e += NT) {
         int r = e, src = 0;
         for (int s = 0; s < nb; s++) {
           src += (r % pd[s]) * ostr[ord[s]];
@@ -188,13 +190,13 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
         if (sAct[c] < 0 && sKind[c] != 0) fast = 0;
       sFast = fast;
     }
-    if (tid < db + dk) {
+    for (int x0 = tid; x0 < db + dk; x0 += NT) {
       // element (r, q) of the task's term reads cur[rowoff[r] + coloff[q]]
       // and is nonzero when rowkey[r] == colkey[q]: the inactive clusters'
       // bra states shifted to ket numbering, in the ket digits' mixed radix
       // (-1 when one falls outside the ket box)
-      const bool row = tid < db;
-      const int x = row ? tid : tid - db;
+      const bool row = x0 < db;
+      const int x = row ? x0 : x0 - db;
       const uint8_t *dg = row ? sDigB + x * N : sDigK + x * N;
       int off = 0, st = 1;
       for (int s = 0; s < nb; s++) {
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
       const int nI = sNB[c], nJ = sNK[c];
       const int64_t fB = sFB[c], fK = sFK[c];
       if (fB < 0 && fK < 0) {
-        for (int e = tid; e < np * rb * rk; e += BT) {
+        for (int e = tid; e < np * rb * rk; e += NT) {
           const int pp = e % np, ij = e / np, i = ij % rb, j = ij / rb;
           sG[e] = G[pp + (int64_t)np * ((lb + i) + (int64_t)mb * (lk + j))];
         }
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
         if (Pc > np) Pc = np;
         for (int p0 = 0; p0 < np; p0 += Pc) {
           const int pc = min(Pc, np - p0);
-          for (int e = tid; e < pc * nI * rk; e += BT) {
+          for (int e = tid; e < pc * nI * rk; e += NT) {
             const int pp = e % pc, r2 = e / pc, I = r2 % nI, j = r2 / nI;
             double v = 0.0;
             const double *Gp = G + (p0 + pp) + (int64_t)np * (lb + I);
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
             sT[e] = v;
           }
           __syncthreads();
-          for (int e = tid; e < pc * rb * rk; e += BT) {
+          for (int e = tid; e < pc * rb * rk; e += NT) {
             const int pp = e % pc, r2 = e / pc, i = r2 % rb, j = r2 / rb;
             double v = 0.0;
             if (fB < 0) {
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
       }
       // contract the leading index: nxt[q, P] = sum_p cur[p, q] g[p, P]
       const int rest2 = rest / np, PP = rb * rk, osz = rest2 * PP;
-      for (int e = tid; e < osz; e += BT) {
+      for (int e = tid; e < osz; e += NT) {
         const int q = e % rest2, P = e / rest2;
         double v = 0.0;
         for (int pp = 0; pp < np; pp++) v += cur[pp + np * q] * sG[pp + np * P];
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
     const double sign = (td.meta & 1) ? -1.0 : 1.0;
     if (sFast) {
       int r = r0, q = q0;
-      for (int k = 0; k < KPT; k++) {
+      for (int k = 0; k < EPT; k++) {
         if (q >= dk) break;
         const int2 R = sRowT[r], C = sColT[q];
         if (R.y == C.y) {
@@ -295,7 +297,7 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
       }
     } else {
       int r = r0, q = q0;
-      for (int k = 0; k < KPT; k++) {
+      for (int k = 0; k < EPT; k++) {
         if (q >= dk) break;
         const uint8_t *dB = sDigB + r * N, *dK = sDigK + q * N;
         double val = sign;
@@ -332,15 +334,15 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
     double *sY = sX[0], *sC = sX[1];
     const int fR = a.fR, nY = db * fR;
     const int64_t kv0 = a.kvar[K];
-    for (int e = tid; e < nY; e += BT) sY[e] = 0.0;
-    for (int e = tid; e < dk * fR; e += BT)
+    for (int e = tid; e < nY; e += NT) sY[e] = 0.0;
+    for (int e = tid; e < dk * fR; e += NT)
       sC[e] = a.fc[(int64_t)(e / dk) * a.nk_var + kv0 + e % dk];
     __syncthreads();
     for (int rr = 0; rr < fR; rr++) {
       const double *cq = sC + rr * dk;
       int r = r0, q = q0, rc = r0;
       double run = 0.0;
-      for (int k = 0; k < KPT; k++) {
+      for (int k = 0; k < EPT; k++) {
         if (q >= dk) break;
         if (r != rc) {
           if (run != 0.0) atomicAdd(&sY[rc * fR + rr], run);
@@ -356,15 +358,15 @@ __global__ __launch_bounds__(BT, MINB) void k_build_blocks(BuildArgs a) {
     }
     __syncthreads();
     const int64_t bv0 = a.bvar[B];
-    for (int e = tid; e < nY; e += BT)
+    for (int e = tid; e < nY; e += NT)
       atomicAdd(&a.fsig[(int64_t)(e % a.fR) * a.nb_var + bv0 + e / a.fR], sY[e]);
     return;
   }
   double *Kout = a.K + a.pair_koff[p];
   const int64_t col0 = a.pair_col[p];
 #pragma unroll
-  for (int k = 0; k < KPT; k++) {
-    const int e = tid + BT * k;
+  for (int k = 0; k < EPT; k++) {
+    const int e = tid + NT * k;
     if (e < db * dk) Kout[panel_index(db, e % db, col0 + e / db)] = acc[k];
   }
 }
@@ -796,23 +798,27 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.nk_var = kp->nvar;
   a.bvar = sp->d_var;
   const int64_t chunk = 1 << 20;
-  // the list holds the small LDS class's pairs [0, n_small), then the middle
-  // class's (n_mid), then the full class's
-  const int64_t mid0 = pl->n_small, full0 = pl->n_small + pl->n_mid;
+  // the list holds the tiny class's pairs [0, n_tiny), then the small LDS
+  // class's (n_small), the middle class's (n_mid) and the full class's
+  const int64_t small0 = pl->n_tiny, mid0 = small0 + pl->n_small, full0 = mid0 + pl->n_mid;
   for (int64_t base = 0; base < pl->n_pair;) {
     a.pair_base = base;
-    const int64_t lim = base < mid0 ? mid0 : base < full0 ? full0 : pl->n_pair;
+    const int64_t lim = base < small0 ? small0 : base < mid0 ? mid0 : base < full0 ? full0 : pl->n_pair;
     const int64_t end = std::min(base + chunk, lim);
     const int64_t n = end - base;
-    if (base < mid0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL, 3>),
+    if (base < small0)
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY, 3, BT_TINY,
+                                         KPT_TINY>),
+                         dim3((unsigned)n), dim3(BT_TINY), 0, s, a);
+    else if (base < mid0)
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL, 3, BT, KPT>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else if (base < full0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID, 2>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID, 2, BT, KPT>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S, 1>), dim3((unsigned)n),
-                         dim3(BT), 0, s, a);
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S, 1, BT, KPT>),
+                         dim3((unsigned)n), dim3(BT), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     base = end;

@@ -413,7 +413,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   int64_t elems = 0;
   std::atomic<int> lim_err{FCG_OK};
   const char *lim_msg = nullptr;
-  RawVec<uint8_t> lds_class(np);  // 2 small, 1 middle, 0 full
+  RawVec<uint8_t> lds_class(np);  // 3 tiny, 2 small, 1 middle, 0 full
 #pragma omp parallel reduction(+ : flops, elems)
   {
     // the previous task's chain order, reused while its shape repeats (the
@@ -513,39 +513,44 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? m_best[s2] : 0) << (8 + 4 * s2);
         flops += m_fl + 2.0 * db * dk;
       }
-      lds_class[p] = need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
-                     ? 2
-                     : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
+      // the tiny class's LDS cannot stage a fused sigma's c and y
+      lds_class[p] = layout && db * dk <= FCG_TINY_ELEMS && need_x <= FCG_LDS_X_TINY &&
+                             need_t <= FCG_LDS_T_TINY && ssum <= FCG_LDS_S_TINY
+                         ? 3
+                     : need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
+                         ? 2
+                         : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
     }
   }
   if (lim_err.load()) {
     *err = lim_msg;
     return lim_err.load();
   }
-  // LDS classes, each in pair order: 2 small, 1 middle, 0 full (a stable
-  // partition over per-thread chunks)
+  // LDS classes, each in pair order: 3 tiny, 2 small, 1 middle, 0 full (a
+  // stable partition over per-thread chunks)
   {
     const int nth = std::max(1, omp_get_max_threads());
     const int64_t chunk = (np + nth - 1) / nth;
-    std::vector<int64_t> cnt(3 * (size_t)nth, 0);
+    std::vector<int64_t> cnt(4 * (size_t)nth, 0);
 #pragma omp parallel for schedule(static, 1)
     for (int t = 0; t < nth; t++)
       for (int64_t p = t * chunk; p < std::min(np, (t + 1) * chunk); p++) cnt[lds_class[p] * nth + t]++;
     int64_t at = 0;
-    for (int cls = 2; cls >= 0; cls--) {
+    for (int cls = 3; cls >= 0; cls--) {
       const int64_t start = at;
       for (int t = 0; t < nth; t++) {
         const int64_t n = cnt[cls * nth + t];
         cnt[cls * nth + t] = at;
         at += n;
       }
+      if (cls == 3) out->n_tiny = at - start;
       if (cls == 2) out->n_small = at - start;
       if (cls == 1) out->n_mid = at - start;
     }
     out->pair_list.resize(np);
 #pragma omp parallel for schedule(static, 1)
     for (int t = 0; t < nth; t++) {
-      int64_t pos[3] = {cnt[t], cnt[nth + t], cnt[2 * nth + t]};
+      int64_t pos[4] = {cnt[t], cnt[nth + t], cnt[2 * nth + t], cnt[3 * nth + t]};
       for (int64_t p = t * chunk; p < std::min(np, (t + 1) * chunk); p++)
         out->pair_list[pos[lds_class[p]]++] = p;
     }
