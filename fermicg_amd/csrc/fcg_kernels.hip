@@ -63,14 +63,15 @@ static_assert(FCG_MAX_BOX <= 256, "state digits are bytes");
 static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL,
               "fused sigma stages c and y in LDS (fused plans use no tiny class)");
 
-// LX / LT / LS: LDS capacities (doubles) of the chain intermediates and Γ
-// slices, the factor-compression rows and the overlaps; the host puts each
+// LX / LT / LS: LDS capacities (doubles) of the staged Γ slices (the chain's
+// arena holds 2 LX: a stage's input at one end, its output at the other),
+// the factor-compression rows and the overlaps; the host puts each
 // pair in the smallest class that holds it.  MINW: waves per SIMD that the
 // class's LDS allows (sets the register budget).  NT threads hold EPT block
 // elements each (NT * EPT >= the class's largest db * dk).
 template <int LX, int LT, int LS, int MINW, int NT, int EPT>
 __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
-  __shared__ double sX[2][LX];
+  __shared__ double sA[2 * LX];
   __shared__ double sG[LX];
   __shared__ double sT[LT];
   __shared__ double sS[LS];
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
     const TaskDev td = a.tasks[t];
     const TermDev tm = a.terms[td.term];
     const int nb = tm.nbody;
-    double *cur = sX[0], *nxt = sX[1];
+    double *cur = sA;
     int ord[4];
     for (int s = 0; s < 4; s++) ord[s] = task_order(td.meta, s);
     int rest = 1;
@@ -268,8 +269,10 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
           __syncthreads();
         }
       }
-      // contract the leading index: nxt[q, P] = sum_p cur[p, q] g[p, P]
+      // contract the leading index: nxt[q, P] = sum_p cur[p, q] g[p, P],
+      // nxt at the arena's other end
       const int rest2 = rest / np, PP = rb * rk, osz = rest2 * PP;
+      double *nxt = cur == sA ? sA + 2 * LX - osz : sA;
       for (int e = tid; e < osz; e += NT) {
         const int q = e % rest2, P = e / rest2;
         double v = 0.0;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
         nxt[e] = v;
       }
       __syncthreads();
-      double *tmp = cur; cur = nxt; nxt = tmp;
+      cur = nxt;
       rest = osz;
     }
     // Kronecker accumulate into this thread's block elements
@@ -331,7 +334,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
     // buffers are free now; the ket box's c staged beside them), a thread's
     // run of elements in one row added once, then one global add per
     // (row, root)
-    double *sY = sX[0], *sC = sX[1];
+    double *sY = sA, *sC = sA + LX;
     const int fR = a.fR, nY = db * fR;
     const int64_t kv0 = a.kvar[K];
     for (int e = tid; e < nY; e += NT) sY[e] = 0.0;

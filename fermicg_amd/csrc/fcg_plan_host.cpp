@@ -419,7 +419,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
     // the previous task's chain order, reused while its shape repeats (the
     // tasks of one pair share their clusters, so mostly it does)
     int m_nb = -1;
-    int64_t m_np[4] = {0, 0, 0, 0}, m_rr[4] = {0, 0, 0, 0}, m_peak = 0;
+    int64_t m_np[4] = {0, 0, 0, 0}, m_rr[4] = {0, 0, 0, 0}, m_arena = 0;
     int m_best[4] = {0, 1, 2, 3};
     double m_fl = 0.0;
 #pragma omp for schedule(dynamic, 1024)
@@ -447,9 +447,10 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         fail("inactive-cluster overlaps exceed LDS budget");
         continue;
       }
-      // largest LDS use of the pair's tasks: chain intermediates and staged
-      // Γ slices (X), factor-compression rows (T), overlaps (S)
-      int64_t need_x = 0, need_t = 0;
+      // largest LDS use of the pair's tasks: staged Γ slices (X), the chain
+      // arena (A: a stage's input and output side by side, and the staged
+      // integrals), factor-compression rows (T), overlaps (S)
+      int64_t need_x = 0, need_a = 0, need_t = 0;
       for (int64_t t = r.t0; t < r.t1; t++) {
         TaskDev &d = out->tasks[t];
         const int tm = d.term, nb = h->nbody[tm];
@@ -474,27 +475,28 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
           fail(lds_err);
           break;
         }
-        if (rest0 > FCG_LDS_X) {
+        if (rest0 > 2 * FCG_LDS_X) {
           fail("term integrals exceed LDS budget");
           break;
         }
-        // chain order: the permutation with the smallest peak intermediate
-        // (ties: fewest flops)
+        // chain order: the permutation whose stages need the smallest arena
+        // (input and output of a stage side by side; ties: fewest flops)
         if (nb != m_nb || !std::equal(np_, np_ + 4, m_np) || !std::equal(rr_, rr_ + 4, m_rr)) {
           int perm[4] = {0, 1, 2, 3};
-          int64_t best_peak = INT64_MAX;
+          int64_t best_arena = INT64_MAX;
           double best_fl = 0.0;
           do {
-            int64_t rest = rest0, peak = rest0;
+            int64_t rest = rest0, arena = rest0;
             double fl = 0.0;
             for (int s2 = 0; s2 < nb; s2++) {
               const int a = perm[s2];
-              rest = rest / np_[a] * rr_[a];
-              fl += 2.0 * np_[a] * (double)rest;
-              peak = std::max(peak, rest);
+              const int64_t next = rest / np_[a] * rr_[a];
+              fl += 2.0 * np_[a] * (double)next;
+              arena = std::max(arena, rest + next);
+              rest = next;
             }
-            if (peak < best_peak || (peak == best_peak && fl < best_fl)) {
-              best_peak = peak;
+            if (arena < best_arena || (arena == best_arena && fl < best_fl)) {
+              best_arena = arena;
               best_fl = fl;
               std::copy(perm, perm + 4, m_best);
             }
@@ -502,24 +504,26 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
           m_nb = nb;
           std::copy(np_, np_ + 4, m_np);
           std::copy(rr_, rr_ + 4, m_rr);
-          m_peak = best_peak;
+          m_arena = best_arena;
           m_fl = best_fl;
         }
-        if (m_peak > FCG_LDS_X) {
+        if (m_arena > 2 * FCG_LDS_X) {
           fail("chain intermediate exceeds LDS budget");
           break;
         }
-        need_x = std::max(need_x, std::max(m_peak, rest0));
+        need_a = std::max(need_a, m_arena);
         for (int s2 = 0; s2 < 4; s2++) d.meta |= (s2 < nb ? m_best[s2] : 0) << (8 + 4 * s2);
         flops += m_fl + 2.0 * db * dk;
       }
       // the tiny class's LDS cannot stage a fused sigma's c and y
-      lds_class[p] = layout && db * dk <= FCG_TINY_ELEMS && need_x <= FCG_LDS_X_TINY &&
-                             need_t <= FCG_LDS_T_TINY && ssum <= FCG_LDS_S_TINY
+      auto fits = [&](int lx, int lt, int ls) {
+        return need_x <= lx && need_a <= 2 * lx && need_t <= lt && ssum <= ls;
+      };
+      lds_class[p] = layout && db * dk <= FCG_TINY_ELEMS && fits(FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY)
                          ? 3
-                     : need_x <= FCG_LDS_X_SMALL && need_t <= FCG_LDS_T_SMALL && ssum <= FCG_LDS_S_SMALL
+                     : fits(FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL)
                          ? 2
-                         : need_x <= FCG_LDS_X_MID && need_t <= FCG_LDS_T_MID && ssum <= FCG_LDS_S_MID;
+                         : fits(FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID);
     }
   }
   if (lim_err.load()) {
