@@ -31,6 +31,7 @@ EXPORTS = [
     "fcg_plan_apply_variant",
     "fcg_plan_set_apply_variant",
     "fcg_apply_variant_counts",
+    "fcg_set_build_class_cap",
 ]
 
 
@@ -107,6 +108,7 @@ def load():
                                              ctypes.c_int]
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_apply_variant_counts.argtypes = [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
+    L.fcg_set_build_class_cap.argtypes = [VP, ctypes.c_int]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -145,6 +147,12 @@ class Context:
 
     def event(self, slot):
         self.check(self.L.fcg_event_record(self.h, slot), "fcg_event_record")
+
+    def set_build_class_cap(self, cap):
+        """Largest LDS class (3 tiny, 2 small, 1 middle, 0 full) of the block
+        build for plans created on this context from now on; every class
+        builds the same blocks bit for bit."""
+        self.check(self.L.fcg_set_build_class_cap(self.h, int(cap)), "fcg_set_build_class_cap")
 
     def elapsed_ms(self, a, b):
         ms = c_f64()

@@ -399,7 +399,7 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   }
   HostPlan hp;
   const bool store = fused == nullptr;
-  int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, store, &hp, &ctx->err);
+  int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, store, ctx->build_class_cap, &hp, &ctx->err);
   if (rc) return rc;
   int64_t reserved = 0;
   {
@@ -569,6 +569,12 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   if (pl->d_ci) fcg_dfree(ctx, pl->d_ci);
   fcg_plan_release(pl->reserved_bytes);
   delete pl;
+}
+
+int fcg_set_build_class_cap(fcg_ctx_t ctx, int cap) {
+  if (!ctx || cap < 0 || cap > 3) return FCG_EINVAL;
+  ctx->build_class_cap = cap;
+  return FCG_OK;
 }
 
 int fcg_plan_stats(fcg_plan_t pl, double *o6) {

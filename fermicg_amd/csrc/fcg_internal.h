@@ -63,6 +63,7 @@ struct fcg_ctx {
   // scratch for host-pointer sigma calls
   double *d_c = nullptr, *d_s = nullptr;
   int64_t scratch_elems = 0;
+  int build_class_cap = 3;  // largest LDS class plans on this context may use
 };
 
 struct TermDev {
@@ -218,9 +219,10 @@ void fcg_apply_order(const std::vector<int64_t> &var_off, const std::vector<int6
 // layout = false: pairs, tasks and LDS classes only (a plan whose blocks are
 // used once during the build); pair_col, pair_koff, the row panels, colidx,
 // the apply order and the diagonal tables are left empty
+// class_cap: largest LDS class index a pair may take (3 tiny .. 0 full)
 int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
-                        const fcg_space *ket, int nbody, bool layout, HostPlan *out,
-                        std::string *err);
+                        const fcg_space *ket, int nbody, bool layout, int class_cap,
+                        HostPlan *out, std::string *err);
 int fcg_tpsci_boxes(const fcg_ops *o, int N, int64_t n_fock, const int16_t *fock,
                     const int64_t *fock_start, int64_t n_cfg, const int16_t *cfg,
                     fcg_space *sp, std::string *err);

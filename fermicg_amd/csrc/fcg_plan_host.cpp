@@ -219,8 +219,8 @@ static int check_space(const fcg_ops *o, const fcg_space *sp, std::string *err) 
 }
 
 int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra,
-                        const fcg_space *ket, int nbody, bool layout, HostPlan *out,
-                        std::string *err) {
+                        const fcg_space *ket, int nbody, bool layout, int class_cap,
+                        HostPlan *out, std::string *err) {
   const int N = bra->N, S = o->S;
   if (h->N != N || o->N != N || ket->N != N) {
     *err = "cluster counts of ham/ops/spaces differ";
@@ -519,11 +519,12 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       auto fits = [&](int lx, int lt, int ls) {
         return need_x <= lx && need_a <= 2 * lx && need_t <= lt && ssum <= ls;
       };
-      lds_class[p] = layout && db * dk <= FCG_TINY_ELEMS && fits(FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY)
-                         ? 3
-                     : fits(FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL)
-                         ? 2
-                         : fits(FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID);
+      const int cls = layout && db * dk <= FCG_TINY_ELEMS && fits(FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY)
+                          ? 3
+                      : fits(FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL)
+                          ? 2
+                          : fits(FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID);
+      lds_class[p] = (uint8_t)std::min(cls, class_cap);
     }
   }
   if (lim_err.load()) {

@@ -242,6 +242,15 @@ int fcg_plan_set_apply_variant(fcg_plan_t plan, int precision, int R, int multi,
  * count that has all of them.  No counterpart in the reference. */
 int fcg_apply_variant_counts(int *n_multi, int *n_rows, int *max_r);
 
+/* LDS classes of the block build: each (bra box, ket box) pair of a plan is
+ * built by the smallest class that holds it (3 tiny, 2 small, 1 middle,
+ * 0 full).  fcg_set_build_class_cap(ctx, cap), cap in [0, 3], makes plans
+ * created on ctx afterwards use at most class `cap` (0 builds every pair in
+ * the full class); 3 is the default.  Every class builds the same blocks
+ * bit for bit, so this is for testing and measurement.  No counterpart in
+ * the reference. */
+int fcg_set_build_class_cap(fcg_ctx_t ctx, int cap);
+
 /* Device time of the last fcg_sigma* call's apply kernel (milliseconds,
  * HIP events on the context's stream) and its launch count. */
 int fcg_last_sigma_time(fcg_ctx_t ctx, double *ms, int *launches);
