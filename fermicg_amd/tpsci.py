@@ -4,6 +4,7 @@
     compute_diagonal(ci_vector, cluster_ops, clustered_ham)   tpsci_outer.jl:694
     tps_ci_davidson(ci_vector, cluster_ops, clustered_ham; …) tpsci_outer.jl:364
     compute_pt2_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:10
+    open_matvec_thread(ci_vector, cluster_ops, clustered_ham; …) tpsci_matvec_thread.jl:11
 
 Same arguments, same meaning, same output layout (rows in the TPSCIstate's
 flattening order, one column per root).  All arithmetic runs in
@@ -183,6 +184,88 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
             print(" %5d %12.8f %12.8f" % (r + 1, evar[r], evar[r] + e2[r]))
     compute_pt2_energy.last_stats = stats
     return e2
+
+
+def first_order_focks(ci_vector, cluster_ops, clustered_ham):
+    """Target Fock blocks of the open matvec, F_ket + TransferConfig for every
+    Fock block of the state and every TransferConfig of the operator, kept
+    when each cluster's electron counts lie in [0, n_orbitals] and its sector
+    has an "H" block (open_matvec_thread job list, src/tpsci_matvec_thread.jl:
+    25-52), in first-appearance order."""
+    sizes = cluster_sizes_of(cluster_ops)
+    N = len(cluster_ops)
+    seen, out = set(), []
+    for fk in ci_vector.focks():
+        for tc in clustered_ham.trans:
+            fb = tuple((fk[c][0] + tc[c][0], fk[c][1] + tc[c][1]) for c in range(N))
+            if fb in seen:
+                continue
+            if not all(0 <= a <= sizes[c] and 0 <= b <= sizes[c] for c, (a, b) in enumerate(fb)):
+                continue
+            if not all((fb[c], fb[c]) in cluster_ops[c]["H"] for c in range(N)):
+                continue
+            seen.add(fb)
+            out.append(fb)
+    return out
+
+
+def _full_space(cluster_ops, focks, R):
+    """Every configuration of each Fock block, first cluster fastest
+    (expand_each_fock_space!, tpsci_outer.jl:796-819)."""
+    from .state import TPSCIstate
+    N = len(cluster_ops)
+    s = TPSCIstate(N, R)
+    cfgs, starts = [], [0]
+    for f in focks:
+        dims = [cluster_ops[c]["H"][(f[c], f[c])].shape[0] for c in range(N)]
+        grids = np.meshgrid(*[np.arange(1, d + 1, dtype=np.int16) for d in dims], indexing="ij")
+        cfgs.append(np.stack([g.ravel(order="F") for g in grids], axis=1))
+        starts.append(starts[-1] + cfgs[-1].shape[0])
+    s.fock = np.array(focks, np.int16).reshape(len(focks), N, 2)
+    s.fock_start = np.array(starts, np.int64)
+    s.cfg = np.concatenate(cfgs).astype(np.int16) if cfgs else np.zeros((0, N), np.int16)
+    s.coeffs = np.zeros((s.cfg.shape[0], R))
+    return s
+
+
+def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, prescreen=True,
+                       nbody=4, engine=None):
+    """sigma = H c on the first-order space of a TPSCI state
+    (open_matvec_thread, src/tpsci_matvec_thread.jl:11-129), on the GPU.
+
+    The target Fock blocks are first_order_focks; sigma over every
+    configuration of them is one screened pair plan (a term's element from
+    ket configuration j enters only when its magnitude exceeds
+    thresh / max_r |c[j, r]|, the test of _collect_significant_thread!,
+    :830-895).  Returns a TPSCIstate holding every target Fock block (the
+    reference adds each job's block even when nothing lands in it) and in it
+    the configurations whose sigma is nonzero, i.e. those some element
+    reached, with sigma as coefficients.  Configurations follow each block's
+    enumeration order (first cluster fastest) and blocks their first
+    appearance; the reference's order is its Dicts', so compare by key.
+    `prescreen` is accepted for call compatibility: the reference's bounds
+    only skip elements that fail the same test.  Blocks of the target space
+    are stored on the device while sigma is formed (its footprint is that of
+    a pair plan over every configuration of those blocks)."""
+    from .state import TPSCIstate
+    eng = engine or default_engine()
+    R = ci_vector.R
+    focks = first_order_focks(ci_vector, cluster_ops, clustered_ham)
+    target = _full_space(cluster_ops, focks, R)
+    bra = _lib.DeviceSpace.tpsci(eng.ctx, target)
+    ket = _lib.DeviceSpace.tpsci(eng.ctx, ci_vector)
+    pl = _lib.Plan.create_pair(eng.ctx, eng.ham(clustered_ham, nbody), eng.ops(cluster_ops), bra,
+                               ket, nbody=nbody, screen=(ci_vector.coeffs, thresh))
+    sig = pl.sigma(ci_vector.coeffs)
+    del pl, bra, ket
+    keep = np.any(sig != 0.0, axis=1)
+    out = TPSCIstate(ci_vector.N, R)
+    out.fock = target.fock.copy()
+    kept_before = np.concatenate([[0], np.cumsum(keep)]).astype(np.int64)
+    out.fock_start = kept_before[target.fock_start]
+    out.cfg = target.cfg[keep].copy()
+    out.coeffs = sig[keep].copy()
+    return out
 
 
 def _copy_state(s):

@@ -327,8 +327,9 @@ static int64_t fock_intern(int64_t *slot, int64_t cap, int16_t *keys, int64_t *n
  * (their `haskey(...) || return`). */
 static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx,
                     const int16_t *fk, const int16_t *ket, const double *cket /* R */, int R,
-                    double thresh, const int64_t *stride, int64_t nx, double *sig, double **w1,
-                    double **w2, int64_t *wcap) {
+                    double thresh, const int64_t *stride, int64_t nx, double *sig,
+                    unsigned char *hit /* nullable: marks inserted configurations */,
+                    double **w1, double **w2, int64_t *wcap) {
   const int N = o->n_clusters;
   const int k = h->nbody[t];
   const int32_t *cl = h->clusters + 4 * t;
@@ -409,6 +410,7 @@ static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx
       r0 /= mb[a];
     }
     for (int r = 0; r < R; r++) sig[lin + nx * r] += sgn * v * cket[r];
+    if (hit) hit[lin] = 1;
   }
   return 0;
 }
@@ -477,7 +479,7 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
           for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
             for (int r = 0; r < R; r++) cket[r] = c[(int64_t)r * sp->n_cfg + kj];
             rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, sig,
-                          &w1, &w2, &wcap);
+                          NULL, &w1, &w2, &wcap);
           }
         }
       }
@@ -535,5 +537,80 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
   free(cand_job);
   free(jfock);
   free(slot);
+  return err;
+}
+
+/* Open matvec (open_matvec_thread, src/tpsci_matvec_thread.jl:11-129, with
+ * prescreen=false) on caller-listed target Fock blocks: every (TransferConfig
+ * group, ket Fock block) with F_ket + TransferConfig = F_x feeds block x
+ * (the job list of :25-52); each term (nbody or fewer clusters) on each ket
+ * configuration adds its elements above thresh / max_r |c_ket| (pt2_term).
+ * Block j covers configurations off[j] .. off[j+1]-1 (first cluster fastest;
+ * the sizes must match the "H" blocks, else -2).  sig is [R][off[n]]
+ * (root-major), hit[x] = 1 for the configurations the reference inserts
+ * into its output (some element passed the threshold). */
+int orc_tpsci_open_sigma(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
+                         const double *c, int nbody, double thresh, int64_t n_target,
+                         const int16_t *target_fock, const int64_t *off, double *sig,
+                         unsigned char *hit, int nthreads) {
+  const int N = o->n_clusters, S = o->S;
+  if (N > MAXN || R < 1 || R > 64) return -1;
+  const int64_t total = off[n_target];
+  for (int64_t x = 0; x < total * R; x++) sig[x] = 0.0;
+  for (int64_t x = 0; x < total; x++) hit[x] = 0;
+  fock_map tm;
+  fock_map_build(&tm, target_fock, n_target, N);
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t j = 0; j < n_target; j++) {
+    const int16_t *fx = target_fock + j * 2 * N;
+    int64_t stride[MAXN], nx = 1;
+    int rc = 0;
+    for (int ci = 0; ci < N; ci++) {
+      const int32_t *d = NULL;
+      if (!gamma_block(o, ci, 0, fx, fx, &d)) { rc = -2; break; }
+      stride[ci] = nx;
+      nx *= d[1];
+    }
+    if (!rc && nx != off[j + 1] - off[j]) rc = -2;
+    double *js = rc ? NULL : (double *)calloc((size_t)nx * R, sizeof(double));
+    double *w1 = NULL, *w2 = NULL;
+    int64_t wcap = 0;
+    double cket[64];
+    for (int64_t f = 0; f < sp->n_fock && !rc; f++) {
+      const int16_t *fk = sp->fock + f * 2 * N;
+      for (int g = 0; g < h->n_tc && !rc; g++) {
+        int16_t fy[2 * MAXN];
+        int ok = 1;
+        for (int i = 0; i < 2 * N && ok; i++) {
+          fy[i] = fk[i] + h->tc[(int64_t)g * 2 * N + i];
+          ok = fy[i] >= 0 && fy[i] < S;
+        }
+        if (!ok || fock_map_find(&tm, fy) != j) continue;
+        for (int t = h->tc_start[g]; t < h->tc_start[g + 1] && !rc; t++) {
+          if (h->nbody[t] > nbody) continue;
+          for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
+            for (int r = 0; r < R; r++) cket[r] = c[(int64_t)r * sp->n_cfg + kj];
+            rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, js,
+                          hit + off[j], &w1, &w2, &wcap);
+          }
+        }
+      }
+    }
+    if (!rc)
+      for (int r = 0; r < R; r++)
+        for (int64_t x = 0; x < nx; x++) sig[(int64_t)r * total + off[j] + x] = js[x + nx * r];
+    if (rc) {
+#pragma omp critical
+      err = rc;
+    }
+    free(js);
+    free(w1);
+    free(w2);
+  }
+  free(tm.slot);
   return err;
 }

@@ -76,6 +76,17 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
                   const double *c, const double *e0, int nbody, int h0_op, double thresh,
                   double *e2, int64_t *n_jobs, int nthreads);
 
+/* Reference open_matvec_thread (src/tpsci_matvec_thread.jl:11-129,
+ * prescreen=false) on the target Fock blocks the caller lists (block j =
+ * configurations off[j] .. off[j+1]-1, first cluster fastest): sig [R][n]
+ * root-major, hit[x] = 1 where some term's element passed the threshold
+ * (the configurations the reference inserts).  Returns 0, -2 when a block's
+ * size disagrees with its "H" blocks, -3 on a missing 1-body block. */
+int orc_tpsci_open_sigma(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
+                         const double *c, int nbody, double thresh, int64_t n_target,
+                         const int16_t *target_fock, const int64_t *off, double *sig,
+                         unsigned char *hit, int nthreads);
+
 #endif
 
 #ifndef FCG_ORACLE_BST

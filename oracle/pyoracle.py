@@ -49,6 +49,9 @@ def lib():
                                            P(i64), i64, P(f64), ctypes.c_int]
         L.orc_tpsci_pt2.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64), P(f64),
                                     ctypes.c_int, ctypes.c_int, f64, P(f64), P(i64), ctypes.c_int]
+        L.orc_tpsci_open_sigma.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64),
+                                           ctypes.c_int, f64, i64, P(ctypes.c_int16), P(i64), P(f64),
+                                           P(ctypes.c_uint8), ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -140,6 +143,29 @@ class Oracle:
         if rc:
             raise RuntimeError("oracle pt2 failed: %d" % rc)
         return e2, nj.value
+
+    def open_sigma(self, c, target_focks, sizes, nbody=4, thresh=0.0, nthreads=0):
+        """open_matvec_thread restated (prescreen=false) on the listed target
+        Fock blocks of `sizes` configurations each (first cluster fastest):
+        returns (sigma [n x R], hit [n] bool: configurations the reference
+        inserts)."""
+        c = np.asarray(c, dtype=np.float64)
+        if c.ndim == 1:
+            c = c[:, None]
+        R = c.shape[1]
+        cc = np.ascontiguousarray(c.T)
+        tf = np.ascontiguousarray(np.asarray(target_focks, np.int16).reshape(len(target_focks), -1))
+        off = np.zeros(len(sizes) + 1, np.int64)
+        off[1:] = np.cumsum(sizes)
+        sig = np.zeros((R, int(off[-1])))
+        hit = np.zeros(int(off[-1]), np.uint8)
+        rc = lib().orc_tpsci_open_sigma(ctypes.byref(self.ops), ctypes.byref(self.ham),
+                                        ctypes.byref(self.space), R, _p(cc, f64), nbody, thresh,
+                                        len(sizes), _p(tf, ctypes.c_int16), _p(off, i64),
+                                        _p(sig, f64), _p(hit, ctypes.c_uint8), nthreads)
+        if rc:
+            raise RuntimeError("oracle open_sigma failed: %d" % rc)
+        return sig.T.copy(), hit.astype(bool)
 
     def diagonal(self):
         hd = np.zeros(self.n_cfg)
