@@ -3,6 +3,7 @@
     tps_ci_matvec(ci_vector, cluster_ops, clustered_ham)      tpsci_outer.jl:462
     compute_diagonal(ci_vector, cluster_ops, clustered_ham)   tpsci_outer.jl:694
     tps_ci_davidson(ci_vector, cluster_ops, clustered_ham; …) tpsci_outer.jl:364
+    tps_ci_direct(ci_vector, cluster_ops, clustered_ham; …)   tpsci_outer.jl:165
     compute_pt2_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:10
     open_matvec_thread(ci_vector, cluster_ops, clustered_ham; …) tpsci_matvec_thread.jl:11
 
@@ -125,6 +126,52 @@ def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lin
     out.set_vector(v)
     out.davidson_stats = stats
     return e, out
+
+
+def build_full_H(ci_vector, cluster_ops, clustered_ham, engine=None):
+    """Dense H over the configurations of `ci_vector` (n x n, the order of
+    its flattening), as build_full_H_parallel (src/tpsci_outer.jl:59-149)
+    returns it: the GPU plan's FP64 sigma applied to columns of the identity,
+    up to 16 at a time."""
+    eng = engine or default_engine()
+    pl = eng.tpsci_plan(ci_vector, cluster_ops, clustered_ham)
+    n = len(ci_vector)
+    H = np.zeros((n, n))
+    for j0 in range(0, n, 16):
+        j1 = min(n, j0 + 16)
+        E = np.zeros((n, j1 - j0))
+        E[np.arange(j0, j1), np.arange(j1 - j0)] = 1.0
+        H[:, j0:j1] = pl.sigma(E)
+    return H
+
+
+def tps_ci_direct(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lindep_thresh=1e-12,
+                  max_ss_vecs=12, max_iter=40, shift=None, precond=True, H_old=None, v_old=None,
+                  verbose=0, solver="davidson", engine=None):
+    """Lowest R eigenpairs through the dense H of the space
+    (tps_ci_direct, src/tpsci_outer.jl:165-305).  Returns (e0, TPSCIstate
+    with the eigenvectors, H).
+
+    H is build_full_H (GPU).  As in the reference, a space of at most 500
+    configurations is diagonalised whole (the R lowest eigenpairs of H);
+    a larger one iteratively from the state's coefficients: the GPU Davidson
+    on the same plan, with conv_thresh, lindep_thresh, max_ss_vecs, max_iter
+    and precond as given (the reference's default solver="davidson"; the
+    other solvers and `shift` are accepted for call compatibility).  H_old and
+    v_old, the reference's incremental rebuild, are accepted and not needed:
+    the whole H is rebuilt from the cached plan."""
+    eng = engine or default_engine()
+    H = build_full_H(ci_vector, cluster_ops, clustered_ham, engine=eng)
+    R = ci_vector.R
+    if len(ci_vector) > 500:
+        e0, out = tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=conv_thresh,
+                                  lindep_thresh=lindep_thresh, max_ss_vecs=max_ss_vecs,
+                                  max_iter=max_iter, precond=precond, engine=eng)
+        return np.asarray(e0), out, H
+    w, v = np.linalg.eigh(H)
+    out = _copy_state(ci_vector)
+    out.set_vector(v[:, :R])
+    return w[:R].copy(), out, H
 
 
 def orthonormalize(v):
