@@ -147,3 +147,105 @@ class TPSCIstate:
     def blocks(self):
         for fi in range(self.n_fock):
             yield fi, self.fock_start[fi], self.fock_start[fi + 1]
+
+
+# ---- keyed algebra on TPSCIstates (the reference's OrderedDict operations;
+# a configuration is the key (Fock block, ClusterConfig), blocks and
+# configurations keep insertion order) ----
+
+def _state_blocks(s):
+    """{fock: {config: coefficient row}} in insertion order."""
+    out = {}
+    focks = s.focks()
+    for fi, a, b in s.blocks():
+        d = out.setdefault(focks[fi], {})
+        for i in range(a, b):
+            d[tuple(int(x) for x in s.cfg[i])] = s.coeffs[i].copy()
+    return out
+
+
+def _from_dicts(N, R, blocks):
+    s = TPSCIstate(N, R)
+    focks, starts, cfgs, rows = [], [0], [], []
+    for f, d in blocks.items():
+        focks.append(f)
+        for k, v in d.items():
+            cfgs.append(k)
+            rows.append(v)
+        starts.append(len(cfgs))
+    s.fock = np.array(focks, np.int16).reshape(len(focks), N, 2)
+    s.fock_start = np.array(starts, np.int64)
+    s.cfg = np.array(cfgs, np.int16).reshape(len(cfgs), N)
+    s.coeffs = np.array(rows, np.float64).reshape(len(cfgs), R)
+    return s
+
+
+def copy_state(s):
+    o = TPSCIstate(s.N, s.R)
+    o.fock, o.fock_start, o.cfg, o.coeffs = s.fock.copy(), s.fock_start.copy(), s.cfg.copy(), s.coeffs.copy()
+    return o
+
+
+def add_states(s1, s2):
+    """s1 + s2 by key (add!, src/type_TPSCIstate.jl:617-633): s2's new
+    configurations follow s1's in their block, its new blocks follow."""
+    b = _state_blocks(s1)
+    for f, d in _state_blocks(s2).items():
+        t = b.setdefault(f, {})
+        for k, v in d.items():
+            t[k] = t[k] + v if k in t else v
+    return _from_dicts(s1.N, s1.R, b)
+
+
+def scale_state(s, M):
+    """Coefficients times an R x R' matrix (the reference's `v * S`)."""
+    o = copy_state(s)
+    o.coeffs = s.coeffs @ np.asarray(M, np.float64)
+    o.R = o.coeffs.shape[1]
+    return o
+
+
+def zero_state(s):
+    o = copy_state(s)
+    o.coeffs[:] = 0.0
+    return o
+
+
+def clip_state(s, thresh=1e-5):
+    """Drop configurations whose every coefficient is below thresh in
+    magnitude, then empty blocks (clip!, src/type_TPSCIstate.jl:589-600)."""
+    b = _state_blocks(s)
+    out = {}
+    for f, d in b.items():
+        kept = {k: v for k, v in d.items() if not np.all(np.abs(v) < thresh)}
+        if kept:
+            out[f] = kept
+    return _from_dicts(s.N, s.R, out)
+
+
+def project_out_state(v, w):
+    """Remove from v the configurations of w; a block of v that this empties
+    is removed (project_out!, src/tpsci_outer.jl:869-889)."""
+    b = _state_blocks(v)
+    for f, d in _state_blocks(w).items():
+        if f not in b:
+            continue
+        for k in d:
+            b[f].pop(k, None)
+        if not b[f]:
+            del b[f]
+    return _from_dicts(v.N, v.R, b)
+
+
+def overlap_states(v1, v2):
+    """R x R overlap over common configurations (overlap,
+    src/type_TPSCIstate.jl:416-436)."""
+    b1 = _state_blocks(v1)
+    S = np.zeros((v1.R, v2.R))
+    for f, d in _state_blocks(v2).items():
+        if f not in b1:
+            continue
+        for k, c in d.items():
+            if k in b1[f]:
+                S += np.outer(b1[f][k], c)
+    return S

@@ -6,6 +6,8 @@
     tps_ci_direct(ci_vector, cluster_ops, clustered_ham; …)   tpsci_outer.jl:165
     compute_pt2_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:10
     open_matvec_thread(ci_vector, cluster_ops, clustered_ham; …) tpsci_matvec_thread.jl:11
+    compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt1_wavefunction.jl:11
+    tpsci_ci(ci_vector, cluster_ops, clustered_ham; …)        tpsci.jl:39
 
 Same arguments, same meaning, same output layout (rows in the TPSCIstate's
 flattening order, one column per root).  All arithmetic runs in
@@ -80,6 +82,11 @@ class Engine:
             hit = (clustered_ham, extract_1body_operator(clustered_ham, op_string=op_string))
             self._local[key] = hit
         return hit[1]
+
+    def drop_plans(self):
+        """Free the cached plans (their device blocks), keeping the uploaded
+        operators: for loops whose spaces change every iteration."""
+        self._plans.clear()
 
     def clear(self):
         self._local.clear()
@@ -313,6 +320,124 @@ def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, presc
     out.cfg = target.cfg[keep].copy()
     out.coeffs = sig[keep].copy()
     return out
+
+
+def compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
+                             thresh_foi=1e-8, prescreen=False, verbose=1, threaded=True,
+                             engine=None):
+    """First-order wavefunction and e2 (compute_pt1_wavefunction,
+    src/tpsci_pt1_wavefunction.jl:11-95): sigma = open_matvec_thread on the
+    GPU, the state's own configurations projected out, and per root
+    v_pt = sigma / (E0 / |c|^2 - Hd) with Hd the H0 diagonal (GPU) and
+    e2 = sigma . v_pt.  Returns (e2, TPSCIstate holding v_pt)."""
+    from .state import project_out_state
+    eng = engine or default_engine()
+    norms = np.linalg.norm(ci_vector.coeffs, axis=0)
+    sig = open_matvec_thread(ci_vector, cluster_ops, clustered_ham, nbody=nbody, thresh=thresh_foi,
+                             prescreen=prescreen, engine=eng)
+    sig = project_out_state(sig, ci_vector)
+    h0 = eng.local_operator(clustered_ham, H0)
+    if E0 is None:
+        E0 = compute_expectation_value(ci_vector, cluster_ops, h0, engine=eng)
+    E0 = np.asarray(E0, np.float64).reshape(-1)
+    hd = compute_diagonal(sig, cluster_ops, h0, engine=eng) if len(sig) else np.zeros(0)
+    sv = sig.coeffs
+    v_pt = sv / (E0[None, :] / (norms[None, :] ** 2) - hd[:, None])
+    e2 = np.einsum("ir,ir->r", sv, v_pt)
+    sig.coeffs = v_pt
+    if verbose:
+        print(" PT1: %d configurations, e2 = %s" % (len(sig), np.array2string(e2, precision=8)))
+    return e2, sig
+
+
+def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_foi=1e-6,
+             thresh_asci=None, thresh_var=None, thresh_spin=None, max_iter=10, conv_thresh=1e-4,
+             nbody=4, incremental=True, ci_conv=1e-5, ci_max_iter=50, ci_max_ss_vecs=12,
+             ci_lindep_thresh=1e-12, davidson=False, max_mem_ci=20.0, threaded=True, verbose=1,
+             engine=None):
+    """Selected CI in the TPS basis (tpsci_ci, src/tpsci.jl:39-363): each
+    iteration adds the configurations of the clipped first-order
+    wavefunction to the variational space, solves for R roots there
+    (tps_ci_direct, or tps_ci_davidson when davidson=True or the dense H
+    would exceed max_mem_ci GB), forms E0 = <H_cmf>, and builds the next
+    first-order wavefunction from the variational vector (clipped at
+    thresh_asci when given) -- incrementally as the reference does (the
+    open matvec of the change of the vector since the last iteration, added
+    to the last sigma after rotating it by the overlap) or, with
+    incremental=False, by compute_pt1_wavefunction.  Stops when no root's
+    energy changed by conv_thresh.  Every sigma, diagonal and solve runs on
+    the GPU; the bookkeeping between them is keyed state algebra on the host.
+    thresh_spin (the S^2 extension) is not built here and must be None.
+    Returns (e0, TPSCIstate of the variational vector); the last PT2 estimate
+    is tpsci_ci.last_e2."""
+    from .state import (add_states, clip_state, copy_state, overlap_states, project_out_state,
+                        scale_state, zero_state, TPSCIstate)
+    if thresh_spin is not None:
+        raise ValueError("tpsci_ci: the S^2 extension (thresh_spin) is not available")
+    if len(ci_vector) == 0:
+        raise ValueError("tpsci_ci: input vector has zero length")
+    eng = engine or default_engine()
+    R, N = ci_vector.R, ci_vector.N
+    vec_var = copy_state(ci_vector)
+    vec_pt = zero_state(ci_vector)
+    e0 = np.zeros(R)
+    e2 = np.zeros(R)
+    e0_last = np.zeros(R)
+    vec_asci_old = TPSCIstate(N, R)
+    sig_old = TPSCIstate(N, R)
+    h0 = eng.local_operator(clustered_ham, "Hcmf")
+    for it in range(1, max_iter + 1):
+        if it > 1:
+            if thresh_var is not None:
+                vec_var = clip_state(vec_var, thresh_var)
+            vec_var = add_states(vec_var, zero_state(vec_pt))
+        if 8.0 * len(vec_var) ** 2 * 1e-9 > max_mem_ci or davidson:
+            vec_var.set_vector(orthonormalize(vec_var.coeffs))
+            e0, vec_var = tps_ci_davidson(vec_var, cluster_ops, clustered_ham, conv_thresh=ci_conv,
+                                          max_iter=ci_max_iter, max_ss_vecs=ci_max_ss_vecs,
+                                          lindep_thresh=ci_lindep_thresh, engine=eng)
+        else:
+            e0, vec_var, _ = tps_ci_direct(vec_var, cluster_ops, clustered_ham, conv_thresh=ci_conv,
+                                           max_ss_vecs=ci_max_ss_vecs, max_iter=ci_max_iter,
+                                           lindep_thresh=ci_lindep_thresh, engine=eng)
+        e0 = np.asarray(e0)
+        efock = compute_expectation_value(vec_var, cluster_ops, h0, engine=eng)
+        vec_asci = copy_state(vec_var)
+        if thresh_asci is not None:
+            vec_asci = clip_state(vec_asci, thresh_asci)
+        if incremental:
+            S = overlap_states(vec_asci_old, vec_asci)
+            sig = scale_state(sig_old, S)
+            del_v0 = add_states(vec_asci, scale_state(vec_asci_old, -S))
+            vec_asci_old = copy_state(vec_asci)
+            del_sig = open_matvec_thread(del_v0, cluster_ops, clustered_ham, nbody=nbody,
+                                         thresh=thresh_foi, engine=eng)
+            sig = add_states(sig, del_sig)
+            sig_old = copy_state(sig)
+            sig = project_out_state(sig, vec_asci)
+            hd = compute_diagonal(sig, cluster_ops, h0, engine=eng) if len(sig) else np.zeros(0)
+            norms = np.linalg.norm(vec_asci.coeffs, axis=0)
+            sv = sig.coeffs
+            v_pt = sv / (efock[None, :] / (norms[None, :] ** 2) - hd[:, None])
+            e2 = np.einsum("ir,ir->r", sv, v_pt)
+            vec_pt = copy_state(sig)
+            vec_pt.coeffs = v_pt
+        else:
+            e2, vec_pt = compute_pt1_wavefunction(vec_asci, cluster_ops, clustered_ham, E0=efock,
+                                                  thresh_foi=thresh_foi, nbody=nbody, verbose=0,
+                                                  engine=eng)
+        vec_pt = clip_state(vec_pt, thresh_cipsi)
+        done = np.abs(e0_last - e0).max() < conv_thresh
+        if verbose:
+            print(" TPSCI iter %2d: dim %8d  E0 %s  E0+E2 %s%s"
+                  % (it, len(vec_var), np.array2string(e0, precision=10),
+                     np.array2string(e0 + e2, precision=10), "  *converged" if done else ""))
+        eng.drop_plans()  # this iteration's spaces are not used again
+        if done:
+            break
+        e0_last = e0.copy()
+    tpsci_ci.last_e2 = e2
+    return e0, vec_var
 
 
 def _copy_state(s):

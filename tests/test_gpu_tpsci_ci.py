@@ -1,0 +1,87 @@
+"""tpsci_ci (src/tpsci.jl:39-363) and compute_pt1_wavefunction
+(src/tpsci_pt1_wavefunction.jl:11-95) on the GPU.
+
+Pins:
+  * with thresh_cipsi and thresh_foi at ~0 every iteration adds the whole
+    first-order space, so the selection grows to every configuration the
+    Hamiltonian connects and e0 must equal the lowest eigenvalues of the
+    dense H over all Fock blocks (oracle build_full_H; with complete cluster
+    bases this is FCI) within 1e-8 Eh, incremental and not;
+  * at the reference test's thresholds (thresh_cipsi 1e-2, thresh_foi 1e-5)
+    e0 is variational (>= exact) and e0 + e2 is no farther from exact than
+    e0 (or within 1 uEh of it once e0 is exact);
+  * compute_pt1_wavefunction's e2 equals compute_pt2_energy's (the same
+    sum over the same first-order space) within 1e-10 relative at
+    thresh_foi 0, and its vector is sigma / (E0 - Hd) of the oracle's open
+    sigma.
+"""
+import numpy as np
+import pytest
+
+from fermicg_amd.marshal import flatten_ham, flatten_ops
+from fermicg_amd.state import TPSCIstate
+from fermicg_amd.tpsci import (Engine, compute_pt1_wavefunction, compute_pt2_energy, orthonormalize,
+                               tpsci_ci)
+from oracle.pyoracle import Oracle
+from tests.pt2_cases import four_clusters
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    return Engine(0)
+
+
+@pytest.fixture(scope="module")
+def case():
+    c = four_clusters(R=2)  # complete cluster bases: the full space is FCI (784)
+    full = TPSCIstate.full_blocks(c.bases, c.all_focks, R=1)
+    fo = flatten_ops(c.ops, [len(x) for x in c.clusters])
+    c.exact = np.linalg.eigvalsh(Oracle(fo, flatten_ham(c.H), full).full_H())[:2]
+    return c
+
+
+def _start(case):
+    # the reference configuration block only: a small start space
+    st = case.state
+    f0 = st.focks().index(tuple(tuple(x) for x in case.ref))
+    a, b = st.fock_start[f0], st.fock_start[f0 + 1]
+    s = TPSCIstate.from_blocks(st.N, [(st.focks()[f0], [tuple(int(x) for x in st.cfg[i])
+                                                          for i in range(a, b)])], R=2)
+    v = np.zeros((len(s), 2))
+    v[0, 0] = 1.0
+    v[1, 1] = 1.0
+    s.set_vector(v)
+    return s
+
+
+@pytest.mark.parametrize("incremental", [True, False])
+def test_tpsci_ci_reaches_exact(eng, case, incremental):
+    s = _start(case)
+    e0, v = tpsci_ci(s, case.ops, case.H, thresh_cipsi=1e-14, thresh_foi=0.0, max_iter=12,
+                     conv_thresh=1e-10, incremental=incremental, ci_conv=1e-10, engine=eng)
+    print("incremental", incremental, "dim", len(v), "e0", e0, "exact", case.exact, flush=True)
+    assert np.abs(e0 - case.exact).max() < 1e-8
+
+
+@pytest.mark.parametrize("incremental", [True, False])
+def test_tpsci_ci_selected(eng, case, incremental):
+    s = _start(case)
+    e0, v = tpsci_ci(s, case.ops, case.H, thresh_cipsi=1e-2, thresh_foi=1e-5, thresh_asci=-1,
+                     conv_thresh=1e-7, ci_conv=1e-8, incremental=incremental, engine=eng)
+    e2 = tpsci_ci.last_e2
+    print("incremental", incremental, "dim", len(v), "e0", e0, "e0+e2", e0 + e2, "exact",
+          case.exact, flush=True)
+    assert np.all(e0 >= case.exact - 1e-9)
+    # PT2 is no bound: it may overshoot once e0 is exact, by its clipped tail
+    assert np.all(np.abs(e0 + e2 - case.exact) <= np.maximum(np.abs(e0 - case.exact), 1e-6))
+
+
+def test_pt1_matches_pt2(eng, case):
+    st = case.state
+    st.set_vector(orthonormalize(st.coeffs))
+    e2_pt2 = compute_pt2_energy(st, case.ops, case.H, thresh_foi=0.0, verbose=0, engine=eng)
+    e2, vpt = compute_pt1_wavefunction(st, case.ops, case.H, thresh_foi=0.0, verbose=0, engine=eng)
+    print("pt1 e2", e2, "pt2", e2_pt2, "n", len(vpt), flush=True)
+    assert np.abs(e2 - e2_pt2).max() < 1e-10 * np.abs(e2_pt2).max()
