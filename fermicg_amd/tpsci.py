@@ -83,10 +83,12 @@ class Engine:
             self._local[key] = hit
         return hit[1]
 
-    def drop_plans(self):
-        """Free the cached plans (their device blocks), keeping the uploaded
-        operators: for loops whose spaces change every iteration."""
-        self._plans.clear()
+    def drop_plans(self, keep=()):
+        """Free the cached plans (their device blocks) except those whose
+        keys are in `keep`, keeping the uploaded operators: for loops whose
+        spaces change every iteration."""
+        for k in [k for k in self._plans if k not in keep]:
+            del self._plans[k]
 
     def clear(self):
         self._local.clear()
@@ -386,6 +388,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
     vec_asci_old = TPSCIstate(N, R)
     sig_old = TPSCIstate(N, R)
     h0 = eng.local_operator(clustered_ham, "Hcmf")
+    callers_plans = set(eng._plans)
     for it in range(1, max_iter + 1):
         if it > 1:
             if thresh_var is not None:
@@ -432,7 +435,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
             print(" TPSCI iter %2d: dim %8d  E0 %s  E0+E2 %s%s"
                   % (it, len(vec_var), np.array2string(e0, precision=10),
                      np.array2string(e0 + e2, precision=10), "  *converged" if done else ""))
-        eng.drop_plans()  # this iteration's spaces are not used again
+        eng.drop_plans(keep=callers_plans)  # this iteration's spaces are not used again
         if done:
             break
         e0_last = e0.copy()
