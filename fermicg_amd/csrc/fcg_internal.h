@@ -27,7 +27,9 @@ constexpr int FCG_LDS_S = 1024;       // inactive-cluster overlaps (doubles)
 constexpr int FCG_LDS_X_TINY = 256, FCG_LDS_T_TINY = 128, FCG_LDS_S_TINY = 64;
 constexpr int FCG_TINY_ELEMS = 512;
 constexpr int FCG_LDS_X_SMALL = 1024, FCG_LDS_T_SMALL = 512, FCG_LDS_S_SMALL = 256;
-constexpr int FCG_LDS_X_MID = 2048, FCG_LDS_T_MID = 1024, FCG_LDS_S_MID = 512;
+// the middle class stages Γ slices as large as the full class's (its arena
+// and factor rows are smaller), so that two workgroups still share a CU
+constexpr int FCG_LDS_X_MID = 2048, FCG_LDS_G_MID = 4096, FCG_LDS_T_MID = 768, FCG_LDS_S_MID = 512;
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
 constexpr int FCG_N_PREC = 4;  // FCG_FP64 .. FCG_BF16
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad

@@ -63,16 +63,17 @@ static_assert(FCG_MAX_BOX <= 256, "state digits are bytes");
 static_assert(FCG_MAX_BOX * FCG_MAX_R <= FCG_LDS_X_SMALL,
               "fused sigma stages c and y in LDS (fused plans use no tiny class)");
 
-// LX / LT / LS: LDS capacities (doubles) of the staged Γ slices (the chain's
-// arena holds 2 LX: a stage's input at one end, its output at the other),
-// the factor-compression rows and the overlaps; the host puts each
+// LX / LG / LT / LS: LDS capacities (doubles): the chain's arena holds 2 LX
+// (a stage's input at one end, its output at the other), the staged Γ
+// slices LG, the factor-compression rows LT and the overlaps LS; the host
+// puts each
 // pair in the smallest class that holds it.  MINW: waves per SIMD that the
 // class's LDS allows (sets the register budget).  NT threads hold EPT block
 // elements each (NT * EPT >= the class's largest db * dk).
-template <int LX, int LT, int LS, int MINW, int NT, int EPT>
+template <int LX, int LG, int LT, int LS, int MINW, int NT, int EPT>
 __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
   __shared__ double sA[2 * LX];
-  __shared__ double sG[LX];
+  __shared__ double sG[LG];
   __shared__ double sT[LT];
   __shared__ double sS[LS];
   __shared__ int sLoB[FCG_MAXN], sLoK[FCG_MAXN], sNB[FCG_MAXN], sNK[FCG_MAXN];
@@ -810,17 +811,18 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
     const int64_t end = std::min(base + chunk, lim);
     const int64_t n = end - base;
     if (base < small0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY, 3, BT_TINY,
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_TINY, FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY, 3, BT_TINY,
                                          KPT_TINY>),
                          dim3((unsigned)n), dim3(BT_TINY), 0, s, a);
     else if (base < mid0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL, 3, BT, KPT>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_SMALL, FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL, 3, BT,
+                                         KPT>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else if (base < full0)
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID, 2, BT, KPT>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X_MID, FCG_LDS_G_MID, FCG_LDS_T_MID, FCG_LDS_S_MID, 2, BT, KPT>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     else
-      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_T, FCG_LDS_S, 1, BT, KPT>),
+      hipLaunchKernelGGL((k_build_blocks<FCG_LDS_X, FCG_LDS_X, FCG_LDS_T, FCG_LDS_S, 1, BT, KPT>),
                          dim3((unsigned)n), dim3(BT), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -516,14 +516,17 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         flops += m_fl + 2.0 * db * dk;
       }
       // the tiny class's LDS cannot stage a fused sigma's c and y
-      auto fits = [&](int lx, int lt, int ls) {
-        return need_x <= lx && need_a <= 2 * lx && need_t <= lt && ssum <= ls;
+      // need_x: staged Γ slice (LG), need_a: chain arena (2 LX)
+      auto fits = [&](int lx, int lg, int lt, int ls) {
+        return need_x <= lg && need_a <= 2 * lx && need_t <= lt && ssum <= ls;
       };
-      const int cls = layout && db * dk <= FCG_TINY_ELEMS && fits(FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY)
-                          ? 3
-                      : fits(FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL)
-                          ? 2
-                          : fits(FCG_LDS_X_MID, FCG_LDS_T_MID, FCG_LDS_S_MID);
+      const int cls =
+          layout && db * dk <= FCG_TINY_ELEMS &&
+                  fits(FCG_LDS_X_TINY, FCG_LDS_X_TINY, FCG_LDS_T_TINY, FCG_LDS_S_TINY)
+              ? 3
+          : fits(FCG_LDS_X_SMALL, FCG_LDS_X_SMALL, FCG_LDS_T_SMALL, FCG_LDS_S_SMALL)
+              ? 2
+              : fits(FCG_LDS_X_MID, FCG_LDS_G_MID, FCG_LDS_T_MID, FCG_LDS_S_MID);
       lds_class[p] = (uint8_t)std::min(cls, class_cap);
     }
   }
