@@ -16,6 +16,7 @@ per-cluster (Δnα, Δnβ)) they cause.
 """
 from dataclasses import dataclass, field
 from itertools import permutations
+from math import prod
 
 import numpy as np
 
@@ -129,7 +130,7 @@ def _emit(H, clus, sorted_cl, opers_sorted, focks_sorted, v, reshape):
         axes_of[k].append(pos)
     opers, v = _canonicalize(opers, axes_of, v)
     if reshape:
-        shape = [int(np.prod([v.shape[a] for a in axes_of[k]])) for k in range(len(clus))]
+        shape = [prod(v.shape[a] for a in axes_of[k]) for k in range(len(clus))]
         v = np.asfortranarray(v).reshape(shape, order="F")
     for o in opers:
         if o not in OP_ID:
