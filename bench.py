@@ -1,17 +1,20 @@
 """Benchmark: sigma builds/s (FP64 vs mixed) and Davidson solves on one MI355X.
 
-Workload (default): the Tucker/BST sigma build of fermicg_amd.workloads.
-bst_medium -- 24 orbitals, 24 electrons, 6 clusters of 4, 16 states per Fock
-sector, Tucker blocks with ranks 1..8 over the 1,441 Fock configurations with
-<= 2 electrons transferred, 2 roots (464,878 variables).  Synthetic seeded
-integrals and random-init Tucker cores/factors; nothing is read from disk.
+Workload (default, --workload sparse): the batched TPSCI sigma build of
+fermicg_amd.workloads.tpsci_sparse -- 32 orbitals, 32 electrons, 8 clusters
+of 4, 16 states per Fock sector, the 104,777 Fock configurations within 3
+transferred electrons, 2,000,089 rank <= 1 configurations selected by
+splitmix64 hash, 4 roots.  Synthetic seeded integrals and random start
+vectors; nothing is read from disk.  --workload bst measures the Tucker/BST
+sigma build (24 orbitals, 6 clusters, ranks 1..8, 464,878 variables).
 
 One step = one sigma build H c over the whole space for all roots, inputs
 resident in HBM (device pointers, no PCIe in the timed region).  The headline
 `value` is the mixed-precision sigma of --precision (default FP32 Hamiltonian
 blocks, FP32 accumulation); the FP64, FP32, split-BF16 and BF16 rates,
-Davidson solves (FP64 vs mixed with FP64 residual refresh at restarts) and
-their |dE| against FP64 are reported beside it.
+Davidson solves (FP64 vs mixed with FP64 residual refresh at restarts, with
+and without the one-time plan build) and their |dE| against FP64 are
+reported beside it.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
 """
@@ -33,12 +36,37 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Threads the CPU baseline uses -- every core this job is allotted
+    (OMP_NUM_THREADS when the host sets it, else all of nproc) -- and what
+    the host reports about its processors."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = nproc
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = max(1, min(int(env) if env and env.isdigit() else allowed, allowed))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": nproc, "affinity": allowed, "omp_num_threads": env,
+                     "model": model}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bst", choices=["bst", "tpsci", "sparse", "serve", "pt2"])
+    ap.add_argument("--workload", default="sparse",
+                    choices=["bst", "tpsci", "sparse", "serve", "pt2"])
     ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve workload: arrivals per s")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
@@ -49,7 +77,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
     ap.add_argument("--ss-vecs", type=int, default=12,
                     help="Davidson subspace vectors per root before a restart")
-    ap.add_argument("--cpu-rows", type=int, default=2000)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows (sparse) or bra blocks (bst) the CPU baseline times; 0: about "
+                         "10 s of work (100,000 sparse rows, 2,000 BST blocks)")
     ap.add_argument("--clusters", type=int, default=6, help="bst workload: number of clusters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (known bytes) to scale PMC counters")
@@ -60,6 +90,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_rows <= 0:
+        args.cpu_rows = 100000 if args.workload == "sparse" else 2000
     if args.workload == "serve":
         return serve(args)
     if args.workload == "pt2":
@@ -145,7 +177,7 @@ def main():
         sig[prec] = s_dev.cpu().numpy().T.copy()
     if not args.no_cpu:
         from oracle.pyoracle import BstOracle, Oracle
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, host = host_cpu()
         fo = flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops))
         fh = flatten_ham(p.clustered_ham)
         rng = np.random.Generator(np.random.PCG64(12345))
@@ -186,6 +218,7 @@ def main():
             wall = time.perf_counter() - ta
             cpu = {"value": 1.0 / wall, "unit": "sigma builds/s", "cores": threads, "kind": "port",
                    "sample": "C restatement of tps_ci_matvec, full sigma, %d threads" % threads}
+        cpu["host"] = host
         scale = np.abs(ref).max()
         for prec in precs:
             parity[prec] = float(np.abs(sig[prec][idx] - ref).max() / scale)
@@ -250,6 +283,10 @@ def main():
         "kernel_ms": kernel_ms,
         "solves": solves,
         "solves_per_s": {k: 1.0 / v["seconds"] for k, v in solves.items()},
+        # a solve from scratch also builds the plan once (the reference's
+        # ci_solve includes cache_hamiltonian, src/tucker_outer.jl:131-137)
+        "solves_per_s_with_plan": {k: 1.0 / (v["seconds"] + ps["build_s"])
+                                   for k, v in solves.items()},
         "abs_dE_mixed_vs_fp64": dE,
         "abs_dE_vs_fp64": dEs,
         "parity_rel_vs_cpu_oracle": parity,
@@ -295,7 +332,7 @@ def pt2(args):
     cpu, parity = None, None
     if not args.no_cpu:
         from oracle.pyoracle import Oracle
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, host = host_cpu()
         orc = Oracle(flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
                      flatten_ham(p.clustered_ham), st)
         ta = time.perf_counter()
@@ -307,6 +344,7 @@ def pt2(args):
                "sample": "C restatement of compute_pt2_energy (src/tpsci_pt2_energy.jl:10) on the "
                          "whole workload (%d target Fock jobs), %d OpenMP threads, %.1f s wall"
                          % (n_jobs, threads, wall)}
+        cpu["host"] = host
         log("parity", parity, "cpu", cpu)
     bf, bs = stats["build_flops"], stats["build_seconds"]
     out = {

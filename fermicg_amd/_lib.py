@@ -32,6 +32,7 @@ EXPORTS = [
     "fcg_plan_set_apply_variant",
     "fcg_apply_variant_counts",
     "fcg_set_build_class_cap",
+    "fcg_open_sigma",
 ]
 
 
@@ -109,6 +110,8 @@ def load():
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_apply_variant_counts.argtypes = [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
     L.fcg_set_build_class_cap.argtypes = [VP, ctypes.c_int]
+    L.fcg_open_sigma.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(c_f64), c_f64,
+                                 P(c_f64), P(ctypes.c_uint8)]
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -406,6 +409,26 @@ def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
                                   nbody, h0_op, float(thresh), _p(e2, c_f64), ctypes.byref(st)),
               "fcg_tpsci_pt2")
     return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}
+
+
+def open_sigma(ctx, ham, ops, bra, ket, c, nbody=4, thresh=0.0):
+    """Screened open matvec onto the configurations of `bra` (fcg_open_sigma):
+    returns (sigma [n_bra x R], hit [n_bra] bool, the configurations some
+    element reached)."""
+    c = np.asarray(c, dtype=np.float64)
+    if c.ndim == 1:
+        c = c[:, None]
+    R = c.shape[1]
+    if c.shape[0] != ket.nvar():
+        raise ValueError("ket vector length %d != space dimension %d" % (c.shape[0], ket.nvar()))
+    cc = np.ascontiguousarray(c.T)
+    nb = bra.nvar()
+    out = np.zeros((R, nb))
+    hit = np.zeros(max(nb, 1), np.uint8)
+    ctx.check(ctx.L.fcg_open_sigma(ctx.h, ham.h, ops.h, bra.h, ket.h, nbody, R, _p(cc, c_f64),
+                                   float(thresh), _p(out, c_f64), _p(hit, ctypes.c_uint8)),
+              "fcg_open_sigma")
+    return out.T.copy(), hit[:nb].astype(bool)
 
 
 def set_plan_budget(nbytes):

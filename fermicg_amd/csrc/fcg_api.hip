@@ -27,6 +27,33 @@ void fcg_plan_note_user(fcg_plan *pl, fcg_ctx *ctx) {
 }
 
 namespace {
+std::mutex g_plans_mu;
+std::vector<fcg_plan *> g_plans;  // live plans
+}  // namespace
+
+void fcg_plan_register(fcg_plan *pl) {
+  std::lock_guard<std::mutex> g(g_plans_mu);
+  g_plans.push_back(pl);
+}
+
+void fcg_plan_unregister(fcg_plan *pl) {
+  std::lock_guard<std::mutex> g(g_plans_mu);
+  auto it = std::find(g_plans.begin(), g_plans.end(), pl);
+  if (it != g_plans.end()) g_plans.erase(it);
+}
+
+// a context being destroyed stops being a user of every live plan (its
+// stream is synchronised by fcg_destroy itself)
+static void forget_user(fcg_ctx *ctx) {
+  std::lock_guard<std::mutex> g(g_plans_mu);
+  for (fcg_plan *pl : g_plans) {
+    std::lock_guard<std::mutex> g2(pl->users_mu);
+    auto it = std::find(pl->users.begin(), pl->users.end(), ctx);
+    if (it != pl->users.end()) pl->users.erase(it);
+  }
+}
+
+namespace {
 
 template <typename T>
 int dev_alloc(fcg_ctx *ctx, T **dst, int64_t n) {
@@ -106,7 +133,9 @@ void fcg_destroy(fcg_ctx_t ctx) {
   (void)hipSetDevice(ctx->device);
   dev_free(ctx, ctx->d_c);
   dev_free(ctx, ctx->d_s);
+  if (ctx->d_ci) fcg_dfree(ctx, ctx->d_ci);
   (void)hipStreamSynchronize(ctx->stream);
+  forget_user(ctx);
   for (auto &e : ctx->slots)
     if (e) (void)hipEventDestroy(e);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -393,9 +422,13 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   auto t0 = std::chrono::steady_clock::now();
   TRY(fcg_form_boxes(ctx, o, sp));
   if (ksp != sp) TRY(fcg_form_boxes(ctx, o, ksp));
-  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box && !thr && !fused) {
-    // every box one configuration: join and build on the device
-    return fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
+  if (ksp == sp && sp->tpsci && sp->n_box > 0 && sp->nvar == sp->n_box && !thr && !fused &&
+      fcg_sparse_plan_fits(sp)) {
+    // every box one configuration: join and build on the device (spaces
+    // beyond the device join's limits take the host join below)
+    const int rc = fcg_create_sparse_plan(ctx, h, o, sp, nbody, precisions, out, max_bytes, need);
+    if (rc == FCG_OK) fcg_plan_register(*out);
+    return rc;
   }
   HostPlan hp;
   const bool store = fused == nullptr;
@@ -478,14 +511,29 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
       return FCG_EDEVICE;
     }
   }
-  // device copies of the task bookkeeping are only needed for the build
+  // device copies of the pair and task bookkeeping are only needed for the
+  // build: free them and release their share of the reservation, so that
+  // the plan holds exactly fcg_plan_device_bytes
   dev_free(ctx, pl->d_tasks);
   dev_free(ctx, pl->d_pair_list);
   dev_free(ctx, pl->d_pair_t0);
   dev_free(ctx, pl->d_pair_col);
+  dev_free(ctx, pl->d_pair_B);
+  dev_free(ctx, pl->d_pair_K);
+  dev_free(ctx, pl->d_pair_koff);
+  dev_free(ctx, pl->d_row_ptr);
+  {
+    const int64_t keep =
+        store ? fcg_plan_bytes(pl->n_kel, pl->n_cols, sp->n_box, pl->precisions) : 0;
+    if (pl->reserved_bytes > keep) {
+      fcg_plan_release(pl->reserved_bytes - keep);
+      pl->reserved_bytes = keep;
+    }
+  }
   pl->d_thr = nullptr;
   pl->build_us =
       std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  fcg_plan_register(pl);
   *out = pl;
   return FCG_OK;
 #undef PTRY
@@ -538,15 +586,20 @@ int fcg_plan_budget(int64_t *budget, int64_t *held) {
 
 int fcg_plan_device_bytes(fcg_plan_t pl, int64_t *bytes) {
   if (!pl || !bytes) return FCG_EINVAL;
-  *bytes = fcg_plan_bytes(pl->n_kel, pl->n_cols, pl->sp->n_box, pl->precisions);
+  *bytes = pl->holds_blocks ? fcg_plan_bytes(pl->n_kel, pl->n_cols, pl->sp->n_box, pl->precisions)
+                            : 0;
   return FCG_OK;
 }
 
 void fcg_plan_destroy(fcg_plan_t pl) {
   if (!pl) return;
+  fcg_plan_unregister(pl);
   // other contexts that read the plan finish first; the plan's own stream
   // orders the frees after its work
-  for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
+  {
+    std::lock_guard<std::mutex> g(pl->users_mu);
+    for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
+  }
   fcg_ctx *ctx = pl->ctx;
   dev_free(ctx, pl->d_row_ptr);
   dev_free(ctx, pl->d_pair_B);
@@ -566,7 +619,6 @@ void fcg_plan_destroy(fcg_plan_t pl) {
   dev_free(ctx, pl->d_K32);
   dev_free(ctx, pl->d_Kbf);
   dev_free(ctx, pl->d_Kb16);
-  if (pl->d_ci) fcg_dfree(ctx, pl->d_ci);
   fcg_plan_release(pl->reserved_bytes);
   delete pl;
 }
@@ -611,6 +663,7 @@ int fcg_plan_set_apply_variant(fcg_plan_t pl, int precision, int R, int multi, i
   // above the tuned root count each kernel has one compiled form
   if (R > max_r && (multi != fcg_apply_default_variant(precision) || rows != 0))
     return FCG_EINVAL;
+  std::lock_guard<std::mutex> g(pl->tune_mu);
   pl->tuned_vm[precision][R] = (int8_t)multi;
   pl->tuned_vr[precision][R] = (int8_t)rows;
   pl->tuned[precision][R] = true;
@@ -625,14 +678,15 @@ int fcg_plan_set_apply_variant(fcg_plan_t pl, int precision, int R, int multi, i
 // variants compute identical results, so the outputs of the timing runs are
 // simply overwritten.
 static int tune_apply(fcg_ctx *ctx, fcg_plan *pl, int R, const double *c, double *s,
-                      int precision) {
+                      int precision, void *ci) {
   int nm = 1, nr = 1, max_r = 0;
   fcg_apply_n_variants(&nm, &nr, &max_r);
   int best_m = fcg_apply_default_variant(precision), best_r = 0;
   if (R <= max_r) {
     auto time_it = [&](int vm, int vr, int parts, float *ms) -> int {
       for (int rep = 0; rep < 2; rep++) {
-        FCG_HIP(fcg_launch_apply(pl, R, c, s, precision, ctx->stream, ctx->ev0, vm, vr, parts),
+        FCG_HIP(fcg_launch_apply(pl, R, c, s, precision, ctx->stream, ctx->ev0, vm, vr, ci,
+                                 parts),
                 ctx);
         FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
         FCG_HIP(hipEventSynchronize(ctx->ev1), ctx);
@@ -675,23 +729,23 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
   (void)hipSetDevice(ctx->device);
   fcg_plan_note_user(pl, ctx);
   const int64_t n = (int64_t)R * pl->ksp->nvar;
-  if (pl->ci_bytes < n * 8) {  // interleaved copy of c, sized for the FP64 form
-    if (pl->d_ci) {
-      // the old copy may still be read on the streams that used the plan
-      (void)hipStreamSynchronize(pl->ctx->stream);
-      for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
-      fcg_dfree(ctx, pl->d_ci);
-    }
-    pl->d_ci = nullptr;
-    pl->ci_bytes = 0;
+  if (ctx->ci_bytes < n * 8) {  // interleaved copy of c, sized for the FP64 form
+    // stream-ordered on this context: the old copy's readers ran on it
+    if (ctx->d_ci) fcg_dfree(ctx, ctx->d_ci);
+    ctx->d_ci = nullptr;
+    ctx->ci_bytes = 0;
     double *p = nullptr;
     TRY(dev_alloc(ctx, &p, n));
-    pl->d_ci = p;
-    pl->ci_bytes = n * 8;
+    ctx->d_ci = p;
+    ctx->ci_bytes = n * 8;
   }
-  if (!pl->tuned[precision][R]) TRY(tune_apply(ctx, pl, R, c_dev, s_dev, precision));
+  {
+    std::lock_guard<std::mutex> g(pl->tune_mu);
+    if (!pl->tuned[precision][R])
+      TRY(tune_apply(ctx, pl, R, c_dev, s_dev, precision, ctx->d_ci));
+  }
   FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0,
-                           pl->tuned_vm[precision][R], pl->tuned_vr[precision][R]),
+                           pl->tuned_vm[precision][R], pl->tuned_vr[precision][R], ctx->d_ci),
           ctx);
   FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
   ctx->last_launches = 2;

@@ -65,6 +65,10 @@ struct fcg_ctx {
   // scratch for host-pointer sigma calls
   double *d_c = nullptr, *d_s = nullptr;
   int64_t scratch_elems = 0;
+  // c interleaved ([n_var][R], FP64 or FP32) for the apply: owned by the
+  // context, so plans shared between contexts never share it
+  void *d_ci = nullptr;
+  int64_t ci_bytes = 0;
   int build_class_cap = 3;  // largest LDS class plans on this context may use
 };
 
@@ -164,11 +168,11 @@ struct fcg_plan {
   float *d_K32 = nullptr;
   uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
   uint16_t *d_Kb16 = nullptr;     // bf16 (the hi part alone)
-  void *d_ci = nullptr;           // c interleaved ([n_var][R], FP64 or FP32) for the apply
-  int64_t ci_bytes = 0;
   int64_t reserved_bytes = 0;     // held against the process-wide plan budget
   const double *d_thr = nullptr;  // build only: per-ket-variable screening threshold
   // apply variants chosen by timing on the first sigma of each precision and R
+  // (tune_mu: one context times them while others wait)
+  std::mutex tune_mu;
   bool tuned[FCG_N_PREC][FCG_MAX_R + 1] = {};
   int8_t tuned_vm[FCG_N_PREC][FCG_MAX_R + 1] = {}, tuned_vr[FCG_N_PREC][FCG_MAX_R + 1] = {};
 };
@@ -272,7 +276,13 @@ void fcg_dfree(fcg_ctx *ctx, void *p);
 // note that ctx launches work reading pl (no-op for the owning context)
 void fcg_plan_note_user(fcg_plan *pl, fcg_ctx *ctx);
 void fcg_plan_release(int64_t bytes);
+// live plans, so that a destroyed context can be removed from their users
+void fcg_plan_register(fcg_plan *pl);
+void fcg_plan_unregister(fcg_plan *pl);
 constexpr int FCG_SPARSE_MAX_BLOCK = 256;  // configurations per Fock block (device join)
+// the device join's limits: at most FCG_SPARSE_MAX_BLOCK configurations per
+// Fock block (consecutive), state indices below 256
+bool fcg_sparse_plan_fits(const fcg_space *sp);
 
 // per-ket-variable screening thresholds on the device (fcg_pt2.hip)
 int fcg_screen_thresholds(fcg_ctx *ctx, int64_t n, int R, const double *c, double thresh,
@@ -290,6 +300,10 @@ struct FusedSigma {
   int R;
   const double *c;  // device, ket n_var x R, column-major
   double *sigma;    // device, bra n_var x R, column-major, accumulated into
+  // device, bra n_var bytes, or nullptr: set to 1 for every bra variable
+  // that some term's nonzero element reached after screening (the keys the
+  // reference's open matvec inserts)
+  uint8_t *hit = nullptr;
 };
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
@@ -302,9 +316,10 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
 // vm / vr: apply variants for multi-row and one-row boxes; parts: bit 0
 // multi-row boxes, bit 1 one-row boxes
+// ci: the interleave buffer (at least R * ket n_var * 8 bytes)
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
                             int precision, hipStream_t s, hipEvent_t before_apply, int vm, int vr,
-                            int parts = 3);
+                            void *ci, int parts = 3);
 int fcg_apply_default_variant(int precision);
 int fcg_apply_n_variants(int *n_multi, int *n_rows, int *max_r);
 hipError_t fcg_launch_diag(const fcg_plan *pl, double *hd, hipStream_t s);

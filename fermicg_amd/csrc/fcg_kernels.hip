@@ -50,6 +50,8 @@ struct BuildArgs {
   int fR;
   int64_t nb_var, nk_var;
   const int64_t *bvar;
+  // bra variables reached by a nonzero element that passed the screen
+  uint8_t *hit;
 };
 
 __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int x, int i) {
@@ -147,6 +149,8 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
   double acc[EPT];
 #pragma unroll
   for (int k = 0; k < EPT; k++) acc[k] = 0.0;
+  uint32_t hm = 0;  // bit k: element k received a nonzero screened contribution
+  static_assert(EPT <= 32, "hit mask holds one bit per element of a thread");
   __syncthreads();
 
   for (int64_t t = a.pair_t0[p]; t < a.pair_t0[p + 1]; t++) {
@@ -293,7 +297,10 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
         const int2 R = sRowT[r], C = sColT[q];
         if (R.y == C.y) {
           const double x = cur[R.x + C.x];
-          if (!a.thr || fabs(x) > sThr[q]) acc[k] += sign * x;
+          if (!a.thr || fabs(x) > sThr[q]) {
+            acc[k] += sign * x;
+            if (x != 0.0) hm |= 1u << k;
+          }
         }
         r += dr;
         q += dq;
@@ -321,7 +328,10 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
         }
         if (val != 0.0) {
           const double x = cur[opidx];
-          if (!a.thr || fabs(x) > sThr[q]) acc[k] += val * x;
+          if (!a.thr || fabs(x) > sThr[q]) {
+            acc[k] += val * x;
+            if (x != 0.0) hm |= 1u << k;
+          }
         }
         r += dr;
         q += dq;
@@ -329,6 +339,12 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       }
     }
     __syncthreads();
+  }
+  if (a.hit && hm) {
+    const int64_t bv0 = a.bvar[B];
+#pragma unroll
+    for (int k = 0; k < EPT; k++)
+      if ((hm >> k) & 1u) a.hit[bv0 + (tid + NT * k) % db] = 1;
   }
   if (a.fsig) {
     // fused sigma: the block's products summed per bra row in LDS (the chain
@@ -798,6 +814,7 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.fc = fused ? fused->c : nullptr;
   a.fsig = fused ? fused->sigma : nullptr;
   a.fR = fused ? fused->R : 0;
+  a.hit = fused ? fused->hit : nullptr;
   a.nb_var = sp->nvar;
   a.nk_var = kp->nvar;
   a.bvar = sp->d_var;
@@ -909,7 +926,7 @@ int fcg_apply_n_variants(int *n_multi, int *n_rows, int *max_r) {
 
 hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *sigma,
                             int precision, hipStream_t s, hipEvent_t before_apply, int vm, int vr,
-                            int parts) {
+                            void *ci_buf, int parts) {
   const fcg_space *sp = pl->sp;
   if (sp->n_box == 0) {
     if (before_apply) (void)hipEventRecord(before_apply, s);
@@ -917,12 +934,12 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
   }
   const int64_t nk = pl->ksp->nvar;
   if (precision == FCG_FP64) {
-    double *ci = static_cast<double *>(pl->d_ci);
+    double *ci = static_cast<double *>(ci_buf);
     hipLaunchKernelGGL(k_interleave<double>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
     if (before_apply) (void)hipEventRecord(before_apply, s);
     return launch_by_r<double, double, double>(pl, R, vm, vr, parts, pl->d_K64, ci, sigma, s);
   }
-  float *ci = static_cast<float *>(pl->d_ci);
+  float *ci = static_cast<float *>(ci_buf);
   hipLaunchKernelGGL(k_interleave<float>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
   if (before_apply) (void)hipEventRecord(before_apply, s);
   if (precision == FCG_FP32)

@@ -210,3 +210,58 @@ extern "C" int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_
   }
   return FCG_OK;
 }
+
+// Open matvec with the reference's screening (open_matvec_thread,
+// src/tpsci_matvec_thread.jl:11-129, its element test
+// _collect_significant_thread! :830-895): sigma over the configurations of
+// `bra` from the state (ket, c), built and applied in one pass (no block is
+// stored), and hit[x] = 1 for every bra configuration that some term's
+// element reached with |element| > thresh / max_r |c[j, r]| -- the keys the
+// reference inserts, whether or not their summed sigma cancels to zero.
+extern "C" int fcg_open_sigma(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t bra,
+                              fcg_space_t ket, int nbody, int R, const double *c, double thresh,
+                              double *sigma, uint8_t *hit) {
+  if (!ctx || !h || !o || !bra || !ket || !c || !sigma || R < 1 || R > FCG_MAX_R || nbody < 1 ||
+      nbody > 4 || !(thresh >= 0.0))
+    return FCG_EINVAL;
+  if (!bra->tpsci || !ket->tpsci) {
+    ctx->err = "fcg_open_sigma needs TPSCI spaces";
+    return FCG_EINVAL;
+  }
+  (void)hipSetDevice(ctx->device);
+  if (int rc = fcg_form_boxes(ctx, o, bra)) return rc;
+  if (int rc = fcg_form_boxes(ctx, o, ket)) return rc;
+  const int64_t nb = bra->nvar, nk = ket->nvar;
+  DevBuf<double> dthr(ctx), dc(ctx), ds(ctx);
+  DevBuf<uint8_t> dhit(ctx);
+  // thresholds always: a ket variable with vanishing coefficients gets an
+  // infinite threshold and reaches nothing, as in the reference
+  if (int rc = fcg_screen_thresholds(ctx, nk, R, c, thresh, &dthr.p)) return rc;
+  if (!dc.alloc((int64_t)R * nk) || !ds.alloc((int64_t)R * nb) || !dhit.alloc(nb)) {
+    ctx->err = "open sigma workspace allocation failed";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemcpyAsync(dc.p, c, sizeof(double) * R * nk, hipMemcpyHostToDevice, ctx->stream),
+          ctx);
+  FCG_HIP(hipMemsetAsync(ds.p, 0, sizeof(double) * R * std::max<int64_t>(nb, 1), ctx->stream),
+          ctx);
+  FCG_HIP(hipMemsetAsync(dhit.p, 0, std::max<int64_t>(nb, 1), ctx->stream), ctx);
+  FusedSigma fused{R, dc.p, ds.p};
+  fused.hit = dhit.p;
+  fcg_plan *pl = nullptr;
+  if (nb > 0 && nk > 0) {
+    if (int rc = fcg_create_plan(ctx, h, o, bra, ket, nbody, 1 << FCG_FP64, &pl, INT64_MAX,
+                                 nullptr, dthr.p, &fused))
+      return rc;
+  }
+  std::unique_ptr<fcg_plan, void (*)(fcg_plan *)> pguard(pl, fcg_plan_destroy);
+  if (nb > 0) {
+    FCG_HIP(hipMemcpyAsync(sigma, ds.p, sizeof(double) * R * nb, hipMemcpyDeviceToHost,
+                           ctx->stream),
+            ctx);
+    if (hit)
+      FCG_HIP(hipMemcpyAsync(hit, dhit.p, nb, hipMemcpyDeviceToHost, ctx->stream), ctx);
+  }
+  FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);
+  return FCG_OK;
+}

@@ -644,6 +644,19 @@ int device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int n
 
 }  // namespace
 
+bool fcg_sparse_plan_fits(const fcg_space *sp) {
+  const int N = sp->N;
+  int64_t run = 0;
+  for (int64_t b = 0; b < sp->n_box; b++) {
+    if (b > 0 && sp->box_fock[b] < sp->box_fock[b - 1]) return false;
+    run = (b > 0 && sp->box_fock[b] == sp->box_fock[b - 1]) ? run + 1 : 1;
+    if (run > FCG_SPARSE_MAX_BLOCK) return false;
+    for (int c = 0; c < N; c++)
+      if (sp->lo[b * N + c] > 255) return false;
+  }
+  return true;
+}
+
 int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
                            int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need) {
   const auto t0 = std::chrono::steady_clock::now();

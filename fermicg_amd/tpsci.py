@@ -16,6 +16,8 @@ and the Hamiltonian blocks of each space are cached on an Engine and reused
 across calls (the reference rebuilds nothing between Davidson iterations
 either; its op cache is term.cache, tucker_inner.jl:90-159).
 """
+import warnings
+
 import numpy as np
 
 from . import _lib
@@ -83,6 +85,25 @@ class Engine:
             self._local[key] = hit
         return hit[1]
 
+    def open_sigma(self, ket_state, target_state, cluster_ops, clustered_ham, nbody=4,
+                   thresh=0.0):
+        """Screened open matvec of `ket_state` onto every configuration of
+        `target_state` (fcg_open_sigma): (sigma [n_target x R], hit [n_target]
+        bool, the configurations some term's element reached)."""
+        bra = _lib.DeviceSpace.tpsci(self.ctx, target_state)
+        ket = _lib.DeviceSpace.tpsci(self.ctx, ket_state)
+        return _lib.open_sigma(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
+                               bra, ket, ket_state.coeffs, nbody=nbody, thresh=thresh)
+
+    def pt2(self, ci_vector, cluster_ops, clustered_ham, E0, nbody=4, H0="Hcmf", thresh=0.0):
+        """e2 of an orthonormal TPSCI state over its first-order space
+        (fcg_tpsci_pt2): (e2 [R], statistics dict)."""
+        from .cluster_ops import OP_NAMES
+        space = _lib.DeviceSpace.tpsci(self.ctx, ci_vector)
+        return _lib.tpsci_pt2(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
+                              space, ci_vector.coeffs, E0, nbody=nbody,
+                              h0_op=OP_NAMES.index(H0), thresh=thresh)
+
     def drop_plans(self, keep=()):
         """Free the cached plans (their device blocks) except those whose
         keys are in `keep`, keeping the uploaded operators: for loops whose
@@ -131,6 +152,13 @@ def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lin
     e, v, stats = pl.davidson(ci_vector.coeffs, conv_thresh=conv_thresh, max_iter=max_iter,
                               max_ss_vecs=max_ss_vecs, lindep_thresh=lindep_thresh,
                               precision=precision, refresh_k=refresh_k, precond=precond)
+    if not stats["converged"]:
+        # the reference prints its residuals and returns; so does this, loudly
+        warnings.warn("tps_ci_davidson: not converged after %d iterations (residuals %s, "
+                      "conv_thresh %g)" % (stats["iterations"],
+                                           np.array2string(np.asarray(stats["residual"]),
+                                                           precision=3), conv_thresh),
+                      RuntimeWarning, stacklevel=2)
     out = _copy_state(ci_vector)
     out.set_vector(v)
     out.davidson_stats = stats
@@ -219,7 +247,6 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     thresh_foi / max_r |c[j, r]|); `prescreen` is accepted for call
     compatibility: the reference's bounds only skip contributions that fail
     that test.  Returns e2 (length R)."""
-    from .cluster_ops import OP_NAMES
     eng = engine or default_engine()
     ci = _copy_state(ci_vector)
     ci.set_vector(orthonormalize(ci.coeffs))
@@ -227,10 +254,8 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
         E0 = compute_expectation_value(ci, cluster_ops, eng.local_operator(clustered_ham, H0),
                                        engine=eng)
     E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
-    space = _lib.DeviceSpace.tpsci(eng.ctx, ci)
-    e2, stats = _lib.tpsci_pt2(eng.ctx, eng.ham(clustered_ham, nbody), eng.ops(cluster_ops), space,
-                               ci.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0),
-                               thresh=thresh_foi)
+    e2, stats = eng.pt2(ci, cluster_ops, clustered_ham, E0, nbody=nbody, H0=H0,
+                        thresh=thresh_foi)
     if verbose:
         evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
         print(" first-order space: %d Fock blocks, %d configurations; %.3f s"
@@ -290,31 +315,26 @@ def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, presc
     (open_matvec_thread, src/tpsci_matvec_thread.jl:11-129), on the GPU.
 
     The target Fock blocks are first_order_focks; sigma over every
-    configuration of them is one screened pair plan (a term's element from
-    ket configuration j enters only when its magnitude exceeds
-    thresh / max_r |c[j, r]|, the test of _collect_significant_thread!,
-    :830-895).  Returns a TPSCIstate holding every target Fock block (the
-    reference adds each job's block even when nothing lands in it) and in it
-    the configurations whose sigma is nonzero, i.e. those some element
-    reached, with sigma as coefficients.  Configurations follow each block's
-    enumeration order (first cluster fastest) and blocks their first
-    appearance; the reference's order is its Dicts', so compare by key.
-    `prescreen` is accepted for call compatibility: the reference's bounds
-    only skip elements that fail the same test.  Blocks of the target space
-    are stored on the device while sigma is formed (its footprint is that of
-    a pair plan over every configuration of those blocks)."""
+    configuration of them is one screened open matvec (fcg_open_sigma: a
+    term's element from ket configuration j enters only when its magnitude
+    exceeds thresh / max_r |c[j, r]|, the test of
+    _collect_significant_thread!, :830-895, built and applied in one pass,
+    nothing stored).  Returns a TPSCIstate holding every target Fock block
+    (the reference adds each job's block even when nothing lands in it) and
+    in it the configurations some term's element reached -- the keys the
+    reference inserts, kept even where the contributions cancel -- with
+    sigma as coefficients.  Configurations follow each block's enumeration
+    order (first cluster fastest) and blocks their first appearance; the
+    reference's order is its Dicts', so compare by key.  `prescreen` is
+    accepted for call compatibility: the reference's bounds only skip
+    elements that fail the same test."""
     from .state import TPSCIstate
     eng = engine or default_engine()
     R = ci_vector.R
     focks = first_order_focks(ci_vector, cluster_ops, clustered_ham)
     target = _full_space(cluster_ops, focks, R)
-    bra = _lib.DeviceSpace.tpsci(eng.ctx, target)
-    ket = _lib.DeviceSpace.tpsci(eng.ctx, ci_vector)
-    pl = _lib.Plan.create_pair(eng.ctx, eng.ham(clustered_ham, nbody), eng.ops(cluster_ops), bra,
-                               ket, nbody=nbody, screen=(ci_vector.coeffs, thresh))
-    sig = pl.sigma(ci_vector.coeffs)
-    del pl, bra, ket
-    keep = np.any(sig != 0.0, axis=1)
+    sig, keep = eng.open_sigma(ci_vector, target, cluster_ops, clustered_ham, nbody=nbody,
+                               thresh=thresh)
     out = TPSCIstate(ci_vector.N, R)
     out.fock = target.fock.copy()
     kept_before = np.concatenate([[0], np.cumsum(keep)]).astype(np.int64)

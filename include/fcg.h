@@ -15,8 +15,9 @@
  *   - one fcg_ctx_t per host thread; calls on one ctx are serialised on its
  *     HIP stream.  Device memory is stream-ordered on the context that made
  *     the handle, so destroying a handle waits for no other context's work;
- *     a plan may serve several contexts, one call at a time, and
- *     fcg_plan_destroy first synchronises each of them.
+ *     a plan may serve several contexts at once (each context keeps its own
+ *     interleaved copy of c), fcg_plan_destroy first synchronises each of
+ *     them, and a context destroyed earlier stops being one of them.
  *   - layouts are column-major: Γ blocks (np, m_bra, m_ket), integrals as
  *     the ClusteredTerm ints array, vectors (n_var x R) with element (i,r)
  *     at r*n_var + i.  State indices and ranges are 1-based (Julia).
@@ -223,6 +224,19 @@ typedef struct {
 int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
                   const double *c, const double *e0, int nbody, int h0_op, double thresh,
                   double *e2, fcg_pt2_stats *stats);
+
+/* Open matvec of a TPSCI state onto the configurations of `bra` (replaces
+ * open_matvec_thread, src/tpsci_matvec_thread.jl:11-129, for the target
+ * configurations the caller lists; prescreen does not change the result).
+ * c: the state (n_ket x R, host); a term's element from ket configuration j
+ * enters when |element| > thresh / max_r |c[j, r]|
+ * (_collect_significant_thread!, :830-895).  sigma (n_bra x R) and hit
+ * (n_bra bytes, may be NULL) are host outputs: hit[x] = 1 when some element
+ * reached x (the keys the reference inserts, kept even when their sum
+ * cancels).  Nothing is stored on the device after the call. */
+int fcg_open_sigma(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t bra, fcg_space_t ket,
+                   int nbody, int R, const double *c, double thresh, double *sigma,
+                   uint8_t *hit);
 
 /* Apply variants the plan chose for a precision and root count (-1 before
  * its first sigma there).  The first sigma of each (precision, R) times each

@@ -54,7 +54,30 @@ def _rank1(n_clusters, M, transfer, frac, seed, R):
     return p, st
 
 
+def _thinned_large_blocks():
+    """3 clusters, M=8: Fock blocks of up to 512 configurations, thinned so
+    no block stays whole but several keep more than 256 -- beyond the device
+    join's limit, so the plan takes the host join with one-configuration
+    boxes."""
+    p = build_problem(12, 3, 8, 24, seed=0, max_transfer=1)
+    st = TPSCIstate.full_blocks(p.bases, p.focks, R=2)
+    rng = np.random.Generator(np.random.PCG64(5))
+    blocks = []
+    for fi, a, b in st.blocks():
+        idx = [i for i in range(a, b) if rng.random() > 0.25]
+        if b - a > 1 and len(idx) == b - a:
+            idx = idx[1:]
+        if not idx:
+            idx = [a]
+        blocks.append((st.focks()[fi], [tuple(int(x) for x in st.cfg[i]) for i in idx]))
+    sp = TPSCIstate.from_blocks(3, blocks, R=2)
+    sp.set_vector(rng.normal(size=(len(sp), 2)))
+    assert np.diff(sp.fock_start).max() > 256
+    return p, sp
+
+
 CASES = {
+    "thinned_3c_M8_over256": _thinned_large_blocks,
     "thinned_3c": _thinned_small,
     "rank1_4c_M8": lambda: _rank1(4, 8, 2, 0.5, 5, 3),
     "rank1_5c_M16_full": lambda: _rank1(5, 16, 1, 1.0, 9, 2),   # blocks of up to 76 rows
