@@ -33,6 +33,7 @@ EXPORTS = [
     "fcg_apply_variant_counts",
     "fcg_set_build_class_cap",
     "fcg_open_sigma",
+    "fcg_sigma_device_noalloc",
 ]
 
 
@@ -117,6 +118,7 @@ def load():
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
     L.fcg_sigma.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(c_f64), ctypes.c_int]
     L.fcg_sigma_device.argtypes = [VP, VP, ctypes.c_int, VP, VP, ctypes.c_int]
+    L.fcg_sigma_device_noalloc.argtypes = [VP, VP, ctypes.c_int, VP, VP, ctypes.c_int]
     L.fcg_diagonal.argtypes = [VP, VP, P(c_f64)]
     L.fcg_davidson.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(DavidsonOpts), P(c_f64),
                                P(c_f64), P(DavidsonStats)]
@@ -359,6 +361,14 @@ class Plan(_Handle):
     def sigma_device(self, c_ptr, s_ptr, R, precision="fp64"):
         self.ctx.check(self.ctx.L.fcg_sigma_device(self.ctx.h, self.h, R, VP(c_ptr), VP(s_ptr),
                                                    PRECISION[precision]), "fcg_sigma_device")
+
+    def sigma_device_noalloc(self, c_ptr, s_ptr, R, precision="fp64", ctx=None):
+        """fcg_sigma_device without any allocation (fixed device scratch;
+        R x n_var <= 2^23); `ctx` defaults to the plan's context."""
+        ctx = ctx or self.ctx
+        ctx.check(ctx.L.fcg_sigma_device_noalloc(ctx.h, self.h, R, VP(c_ptr), VP(s_ptr),
+                                                 PRECISION[precision]),
+                  "fcg_sigma_device_noalloc")
 
     def last_sigma_ms(self):
         ms = c_f64()

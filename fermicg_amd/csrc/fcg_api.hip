@@ -32,6 +32,11 @@ std::vector<fcg_plan *> g_plans;  // live plans
 }  // namespace
 
 void fcg_plan_register(fcg_plan *pl) {
+  {
+    // room for the user contexts, so that noting one allocates nothing
+    std::lock_guard<std::mutex> g2(pl->users_mu);
+    pl->users.reserve(16);
+  }
   std::lock_guard<std::mutex> g(g_plans_mu);
   g_plans.push_back(pl);
 }
@@ -110,17 +115,54 @@ int fcg_space_to_device(fcg_space *sp) {
   return FCG_OK;
 }
 
+// Scratch of fcg_sigma_device_noalloc: the interleaved copy of c ([n_var][R]
+// FP64 or FP32) in a file-scope device array, which each GPU holds until the
+// process ends.  Host code never uses the array's name (a host-side
+// stand-in, not a device address): fcg_create looks up its address on each
+// device once, with that device current, and the entry point passes that
+// address to the kernels.  Every call on one GPU shares the array.
+constexpr int64_t FCG_NOALLOC_SCRATCH_DOUBLES = int64_t(1) << 23;  // 64 MiB
+static __device__ __align__(256) double scratch[FCG_NOALLOC_SCRATCH_DOUBLES];
+constexpr int FCG_MAX_DEVICES = 64;
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_dev[FCG_MAX_DEVICES] = {};  // address of `scratch` on each device
+
+// the device must be current
+int scratch_address(fcg_ctx *ctx) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_dev[ctx->device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return FCG_EDEVICE;
+    }
+    g_scratch_dev[ctx->device] = ptr;
+  }
+  ctx->noalloc_scratch = g_scratch_dev[ctx->device];
+  return FCG_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int fcg_create(int device, fcg_ctx_t *out) {
   if (!out) return FCG_EINVAL;
   *out = nullptr;
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return FCG_EDEVICE;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n ||
+      device >= FCG_MAX_DEVICES)
+    return FCG_EDEVICE;
   fcg_ctx *ctx = new fcg_ctx();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
-      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+      scratch_address(ctx) != FCG_OK) {
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return FCG_EDEVICE;
   }
@@ -746,6 +788,44 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
   }
   FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0,
                            pl->tuned_vm[precision][R], pl->tuned_vr[precision][R], ctx->d_ci),
+          ctx);
+  FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
+  ctx->last_launches = 2;
+  return FCG_OK;
+}
+
+int fcg_sigma_device_noalloc(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev,
+                             double *s_dev, int precision) {
+  if (!ctx || !pl || R < 1 || R > FCG_MAX_R || !c_dev || !s_dev) return FCG_EINVAL;
+  if (precision < FCG_FP64 || precision >= FCG_N_PREC || !(pl->precisions & (1 << precision))) {
+    ctx->err = "precision not built into this plan";
+    return FCG_EINVAL;
+  }
+  if (!pl->holds_blocks) {
+    ctx->err = "plan holds no blocks";
+    return FCG_EINVAL;
+  }
+  const int64_t n = (int64_t)R * pl->ksp->nvar;
+  if (n > FCG_NOALLOC_SCRATCH_DOUBLES) {
+    ctx->err = "fcg_sigma_device_noalloc: R x n_var = " + std::to_string(n) +
+               " exceeds the fixed scratch of " + std::to_string(FCG_NOALLOC_SCRATCH_DOUBLES) +
+               " values (64 MiB); use fcg_sigma_device";
+    return FCG_EOVERFLOW;
+  }
+  if (!ctx->noalloc_scratch) {
+    ctx->err = "fcg_sigma_device_noalloc: scratch address unknown";
+    return FCG_EDEVICE;
+  }
+  (void)hipSetDevice(ctx->device);
+  fcg_plan_note_user(pl, ctx);  // capacity reserved at plan creation
+  {
+    std::lock_guard<std::mutex> g(pl->tune_mu);
+    if (!pl->tuned[precision][R])
+      TRY(tune_apply(ctx, pl, R, c_dev, s_dev, precision, ctx->noalloc_scratch));
+  }
+  FCG_HIP(fcg_launch_apply(pl, R, c_dev, s_dev, precision, ctx->stream, ctx->ev0,
+                           pl->tuned_vm[precision][R], pl->tuned_vr[precision][R],
+                           ctx->noalloc_scratch),
           ctx);
   FCG_HIP(hipEventRecord(ctx->ev1, ctx->stream), ctx);
   ctx->last_launches = 2;

@@ -69,6 +69,9 @@ struct fcg_ctx {
   // context, so plans shared between contexts never share it
   void *d_ci = nullptr;
   int64_t ci_bytes = 0;
+  // device address of the fixed scratch array of fcg_sigma_device_noalloc
+  // on this context's device
+  void *noalloc_scratch = nullptr;
   int build_class_cap = 3;  // largest LDS class plans on this context may use
 };
 

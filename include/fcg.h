@@ -163,6 +163,19 @@ int fcg_sigma(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *c, double *si
 int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *c_dev,
                      double *sigma_dev, int precision);
 
+/* fcg_sigma_device that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The interleaved copy of c that
+ * fcg_sigma_device keeps in a per-context allocation lives here in a fixed
+ * device array of 2^23 values (64 MiB) that each GPU holds until the process
+ * ends; R x (ket n_var) above 2^23 returns FCG_EOVERFLOW.  Every call on one
+ * GPU shares that array (it is written in full before it is read), so calls
+ * that overlap on one GPU -- from any contexts -- race; serialise them.  The
+ * first call of a plan for a (precision, R) also times the apply variants
+ * (no allocation).  Results equal fcg_sigma_device's bit for bit. */
+int fcg_sigma_device_noalloc(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *c_dev,
+                             double *sigma_dev, int precision);
+
 /* Diagonal of H in the space's basis (replaces compute_diagonal,
  * src/tpsci_outer.jl:694-715). */
 int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t plan, double *hd);
