@@ -32,9 +32,13 @@ OP_NAMES = [
     "A", "a", "B", "b",
     "AA", "aa", "BB", "bb", "Aa", "Bb", "Ab", "Ba", "AB", "ba", "BA", "ab",
     "AAa", "Aaa", "BBb", "Bbb", "ABa", "Aba", "ABb", "Bba",
+    # spin and one-particle operators (build_local_quantities.jl:81-124,
+    # 168-172): S+_p = a'_pα a_pβ and S-_p = a'_pβ a_pα per orbital, Sz and
+    # the local S^2 (both on the sector's states), E1 = Aa + Bb
+    "S+", "S-", "Sz", "S2", "E1",
 ]
 OP_ID = {s: i for i, s in enumerate(OP_NAMES)}
-STRING_OPS = OP_NAMES[2:]
+STRING_OPS = OP_NAMES[2:26]
 
 
 def op_delta(opstring):
@@ -209,6 +213,73 @@ def compute_cluster_eigenbasis(ints: InCoreInts, clusters, max_roots=10, sectors
     return bases
 
 
+def _spin_ladder(space, V, sec, lower):
+    """S- (lower) or S+ applied to the columns of V in sector `sec`: the
+    images in the neighbouring sector, columns that vanish (states whose S
+    is below the new |Ms|) dropped and the rest normalised; returns
+    (vectors, sector) or (empty, sector) when the sector does not exist."""
+    n = space.n
+    na, nb = sec
+    tgt = (na - 1, nb + 1) if lower else (na + 1, nb - 1)
+    if not (0 <= tgt[0] <= n and 0 <= tgt[1] <= n):
+        return np.zeros((0, 0)), tgt
+    W = np.zeros((space.dim(tgt), V.shape[1]))
+    for p in range(n):
+        if lower:    # a'_pβ a_pα
+            r1 = space.single("a", p, sec)
+            r2 = space.single("B", p, r1[0]) if r1 is not None else None
+        else:        # a'_pα a_pβ
+            r1 = space.single("b", p, sec)
+            r2 = space.single("A", p, r1[0]) if r1 is not None else None
+        if r1 is None or r2 is None:
+            continue
+        W += r2[1] @ (r1[1] @ V)
+    nrm = np.linalg.norm(W, axis=0)
+    keep = nrm > 1e-10 * max(1.0, float(np.abs(V).max()) if V.size else 1.0)
+    return W[:, keep] / nrm[keep], tgt
+
+
+def compute_cluster_eigenbasis_spin(ints: InCoreInts, clusters, rdm1a, rdm1b, delta_elec,
+                                    ref_fock, max_roots=10):
+    """Spin-adapted local bases (compute_cluster_eigenbasis_spin,
+    build_local_quantities.jl:899-1068).  For each cluster and each electron
+    count within delta_elec[c] of its count in ref_fock, the local FCI of
+    the embedded Hamiltonian is solved in the highest-Ms sector
+    (n/2 + n%2, n/2), its lowest max_roots states kept; then S- is applied
+    repeatedly (min(nα, n_orb - nβ) times) and S+ likewise
+    (min(nβ, n_orb - nα) times), each image sector holding the normalised
+    images of the states that survive (the spin multiplet partners, so every
+    Ms component of a multiplet carries the same phase convention), with
+    energies <v|H|v>."""
+    bases = []
+    for ci, orbs in enumerate(clusters):
+        n = len(orbs)
+        space = LocalFockSpace(n)
+        li = subset(ints, orbs, rdm1a, rdm1b)
+        ni = ref_fock[ci][0] + ref_fock[ci][1]
+        d = delta_elec[ci]
+        sectors = [(nj // 2 + nj % 2, nj // 2) for nj in range(ni - d, ni + d + 1)
+                   if 0 <= nj <= 2 * n]
+        vecs, ens = {}, {}
+        for sec in sectors:
+            H = space.hamiltonian(li.h1, li.h2, sec)
+            w, v = np.linalg.eigh(H)
+            m = min(max_roots, len(w))
+            vecs[sec] = _fix_phase(np.ascontiguousarray(v[:, :m]))
+            ens[sec] = w[:m]
+            for lower, steps in ((True, min(sec[0], n - sec[1])), (False, min(sec[1], n - sec[0]))):
+                vi, si = vecs[sec], sec
+                for _ in range(steps):
+                    vi, si = _spin_ladder(space, vi, si, lower)
+                    if vi.shape[1] == 0:
+                        break
+                    Hi = space.hamiltonian(li.h1, li.h2, si)
+                    vecs[si] = np.ascontiguousarray(vi)
+                    ens[si] = np.einsum("ik,ij,jk->k", vi, Hi, vi)
+        bases.append(ClusterBasis(ci, orbs, space, vecs, ens))
+    return bases
+
+
 def compute_cluster_ops(bases, ints: InCoreInts):
     """ClusterOps for every cluster: list of dict op -> {(sec_bra, sec_ket): Γ}
     (reference compute_cluster_ops, build_local_quantities.jl:40-191).  The
@@ -240,8 +311,42 @@ def compute_cluster_ops(bases, ints: InCoreInts):
                 # (p1..pk, I, J) -> column-major (n^k, Mb, Mk), p1 fastest
                 Gf = np.asfortranarray(G).reshape((n ** k, Vb.shape[1], cb.n_states(sec_ket)), order="F")
                 ops[name][(sec_bra, sec_ket)] = np.asfortranarray(Gf)
+        _add_spin_ops(ops, cb)
         all_ops.append(ops)
     return all_ops
+
+
+def _add_spin_ops(ops, cb):
+    """S+, S-, Sz, S2 and E1 of one cluster (build_local_quantities.jl:81-124,
+    168-172): S+[p, I, J] = Ab[p, p, I, J], S-[p, I, J] = Ba[p, p, I, J],
+    Sz = (nα - nβ)/2 on each sector's states (shape (1, M, M)), S2 = the
+    sector's local S^2 = S- S+ + Sz (Sz + 1) between its states (2-D like
+    H, tdm_S2 :233-257), E1 = Aa + Bb."""
+    n = cb.space.n
+    for name, src in (("S+", "Ab"), ("S-", "Ba")):
+        d = {}
+        for (sb, sk), G in ops[src].items():
+            G4 = G.reshape(n, n, G.shape[1], G.shape[2], order="F")
+            d[(sb, sk)] = np.asfortranarray(np.stack([G4[p, p] for p in range(n)], axis=0))
+        ops[name] = d
+    sz, s2, e1 = {}, {}, {}
+    for sec in cb.sectors():
+        V = cb[sec]
+        M = V.shape[1]
+        m = 0.5 * (sec[0] - sec[1])
+        sz[(sec, sec)] = np.asfortranarray((m * np.eye(M)).reshape(1, M, M))
+        S2 = (m * (m + 1.0)) * np.eye(V.shape[0])
+        up = (sec[0] + 1, sec[1] - 1)
+        if sec[1] > 0 and sec[0] < n:
+            Sp = np.zeros((len(cb.space.dets[up]), V.shape[0]))
+            for p in range(n):
+                mid, Bq = cb.space.single("b", p, sec)
+                _, Ap = cb.space.single("A", p, mid)
+                Sp += Ap @ Bq
+            S2 += Sp.T @ Sp  # S- = (S+)'
+        s2[(sec, sec)] = np.asfortranarray(V.T @ S2 @ V)
+        e1[(sec, sec)] = np.asfortranarray(ops["Aa"][(sec, sec)] + ops["Bb"][(sec, sec)])
+    ops["Sz"], ops["S2"], ops["E1"] = sz, s2, e1
 
 
 def add_cmf_operators(all_ops, bases, ints: InCoreInts, rdm1a, rdm1b):

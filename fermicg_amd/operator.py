@@ -241,3 +241,26 @@ def extract_1body_operator(H, op_string="H"):
             if t.nbody == 1:
                 out.add(tc, ClusteredTerm((op_string,), t.delta, t.parity, t.clusters, t.ints))
     return out
+
+
+def extract_S2(n_clusters, cluster_sizes):
+    """The total S^2 as a ClusteredOperator (extract_S2,
+    type_ClusteredOperator.jl:644-715): per cluster the local "S2"
+    (1-body), per cluster pair i < j the terms S+_i S-_j and S-_i S+_j
+    (integrals of ones over the orbitals of both clusters, so each sums its
+    per-orbital operators) and 2 Sz_i Sz_j, in the reference's order."""
+    N = n_clusters
+    out = ClusteredOperator(N)
+    out.trans[_zero_tc(N)] = []
+    for i in range(N):
+        out.add(_zero_tc(N), ClusteredTerm(("S2",), ((0, 0),), (0,), (i,), np.ones(1)))
+        for j in range(i + 1, N):
+            ni, nj = cluster_sizes[i], cluster_sizes[j]
+            for (oi, oj), (fi, fj) in ((("S+", "S-"), ((1, -1), (-1, 1))),
+                                       (("S-", "S+"), ((-1, 1), (1, -1)))):
+                out.add(_tc_replace(N, (i, j), (fi, fj)),
+                        ClusteredTerm((oi, oj), (fi, fj), (0, 0), (i, j),
+                                      np.ones((ni, nj), order="F")))
+            out.add(_zero_tc(N), ClusteredTerm(("Sz", "Sz"), ((0, 0), (0, 0)), (0, 0), (i, j),
+                                               2.0 * np.ones((1, 1), order="F")))
+    return out

@@ -249,3 +249,42 @@ def overlap_states(v1, v2):
             if k in b1[f]:
                 S += np.outer(b1[f][k], c)
     return S
+
+
+def possible_spin_focksectors(ref_fock):
+    """Fock configurations that spin-complete `ref_fock`
+    (possible_spin_focksectors, type_FockConfig.jl:83-121): per cluster the
+    sectors (nα, nβ), (nα-1, nβ+1), ... down to the spin-flipped one (or
+    upwards when nα < nβ), combined by Iterators.product (first cluster
+    fastest) and kept when the total Nα and Nβ equal the reference's."""
+    spaces = []
+    for na, nb in ref_fock:
+        sp = [(na, nb)]
+        a, b = na, nb
+        for _ in range(abs(na - nb)):
+            a, b = (a - 1, b + 1) if na > nb else (a + 1, b - 1)
+            sp.append((a, b))
+        spaces.append(sp)
+    Na = sum(f[0] for f in ref_fock)
+    Nb = sum(f[1] for f in ref_fock)
+    out = []
+    for t in product(*reversed(spaces)):
+        f = tuple(reversed(t))
+        if sum(x[0] for x in f) == Na and sum(x[1] for x in f) == Nb:
+            out.append(f)
+    return out
+
+
+def add_spin_focksectors(state):
+    """add_spin_focksectors (type_TPSCIstate.jl:660-671): for every Fock
+    block of `state`, each spin-completing Fock block (the block itself
+    included) is set to hold only the all-ground configuration with zero
+    coefficients -- the reference re-creates each such block empty
+    (add_fockconfig!, :92-95) before inserting that configuration, so a
+    block that existed keeps its position and loses its other entries."""
+    b = _state_blocks(state)
+    gs = tuple([1] * state.N)
+    for f in list(b.keys()):
+        for g in possible_spin_focksectors(f):
+            b[g] = {gs: np.zeros(state.R)}
+    return _from_dicts(state.N, state.R, b)

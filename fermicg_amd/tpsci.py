@@ -85,6 +85,18 @@ class Engine:
             self._local[key] = hit
         return hit[1]
 
+    def s2_operator(self, cluster_ops):
+        """extract_S2 for these clusters, cached (one object per cluster
+        layout, so its device plans are reused)."""
+        from .operator import extract_S2
+        sizes = tuple(cluster_sizes_of(cluster_ops))
+        key = ("S2", sizes)
+        hit = self._local.get(key)
+        if hit is None:
+            hit = (None, extract_S2(len(sizes), sizes))
+            self._local[key] = hit
+        return hit[1]
+
     def open_sigma(self, ket_state, target_state, cluster_ops, clustered_ham, nbody=4,
                    thresh=0.0):
         """Screened open matvec of `ket_state` onto every configuration of
@@ -141,11 +153,30 @@ def compute_diagonal(ci_vector, cluster_ops, clustered_ham, engine=None):
     return eng.tpsci_plan(ci_vector, cluster_ops, clustered_ham).diagonal()
 
 
+def compute_s2(ci_vector, cluster_ops, engine=None):
+    """<r|S^2|r> per root with the S^2 of extract_S2 (the report of
+    tps_ci_direct / tps_ci_davidson, src/tpsci_outer.jl:287-294,436-442)."""
+    eng = engine or default_engine()
+    return compute_expectation_value(ci_vector, cluster_ops, eng.s2_operator(cluster_ops),
+                                     engine=eng)
+
+
+def _report_s2(out, e, cluster_ops, eng, verbose):
+    s2 = compute_s2(out, cluster_ops, engine=eng)
+    out.s2 = s2
+    if verbose:
+        print(" %5s %12s %12s" % ("Root", "Energy", "S2"))
+        for r in range(len(e)):
+            print(" %5d %12.8f %12.8f" % (r + 1, e[r], abs(s2[r])))
+
+
 def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lindep_thresh=1e-12,
                     max_ss_vecs=12, max_iter=40, precond=True, precision="fp64", refresh_k=0,
-                    engine=None):
+                    verbose=0, engine=None):
     """Lowest R eigenpairs in the space of `ci_vector`, starting from its
-    coefficients.  Returns (energies, TPSCIstate with the eigenvectors)."""
+    coefficients.  Returns (energies, TPSCIstate with the eigenvectors);
+    the state carries <S^2> per root as `.s2` (printed with verbose, as the
+    reference prints it)."""
     eng = engine or default_engine()
     pl = eng.tpsci_plan(ci_vector, cluster_ops, clustered_ham,
                         precisions=tuple(sorted({"fp64", precision})))
@@ -162,6 +193,7 @@ def tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, lin
     out = _copy_state(ci_vector)
     out.set_vector(v)
     out.davidson_stats = stats
+    _report_s2(out, e, cluster_ops, eng, verbose)
     return e, out
 
 
@@ -203,11 +235,13 @@ def tps_ci_direct(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, linde
     if len(ci_vector) > 500:
         e0, out = tps_ci_davidson(ci_vector, cluster_ops, clustered_ham, conv_thresh=conv_thresh,
                                   lindep_thresh=lindep_thresh, max_ss_vecs=max_ss_vecs,
-                                  max_iter=max_iter, precond=precond, engine=eng)
+                                  max_iter=max_iter, precond=precond, verbose=verbose,
+                                  engine=eng)
         return np.asarray(e0), out, H
     w, v = np.linalg.eigh(H)
     out = _copy_state(ci_vector)
     out.set_vector(v[:, :R])
+    _report_s2(out, w[:R], cluster_ops, eng, verbose)
     return w[:R].copy(), out, H
 
 
@@ -389,13 +423,15 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
     incremental=False, by compute_pt1_wavefunction.  Stops when no root's
     energy changed by conv_thresh.  Every sigma, diagonal and solve runs on
     the GPU; the bookkeeping between them is keyed state algebra on the host.
-    thresh_spin (the S^2 extension) is not built here and must be None.
+    With thresh_spin, each iteration first adds the spin-completing
+    configurations: the open matvec of the extract_S2 operator on the
+    variational vector at threshold thresh_spin, its residual
+    S^2|v> - |v><v|S^2|v> reported per root, and its configurations added
+    with zero coefficients (src/tpsci.jl:126-161).
     Returns (e0, TPSCIstate of the variational vector); the last PT2 estimate
     is tpsci_ci.last_e2."""
     from .state import (add_states, clip_state, copy_state, overlap_states, project_out_state,
                         scale_state, zero_state, TPSCIstate)
-    if thresh_spin is not None:
-        raise ValueError("tpsci_ci: the S^2 extension (thresh_spin) is not available")
     if len(ci_vector) == 0:
         raise ValueError("tpsci_ci: input vector has zero length")
     eng = engine or default_engine()
@@ -414,6 +450,20 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
             if thresh_var is not None:
                 vec_var = clip_state(vec_var, thresh_var)
             vec_var = add_states(vec_var, zero_state(vec_pt))
+        if thresh_spin is not None:
+            # S^2|v> - |v><v|S^2|v>: its configurations spin-complete the space
+            s2op = eng.s2_operator(cluster_ops)
+            res = open_matvec_thread(vec_var, cluster_ops, s2op, nbody=nbody, thresh=thresh_spin,
+                                     engine=eng)
+            expval = overlap_states(vec_var, res)
+            res = add_states(res, scale_state(vec_var, -expval))
+            if verbose:
+                for r in range(R):
+                    print(" S^2 Residual %12.8f" % float(res.coeffs[:, r] @ res.coeffs[:, r]))
+            n1 = len(vec_var)
+            vec_var = add_states(vec_var, zero_state(res))
+            if verbose:
+                print(" Add spin completing states %8d -> %8d" % (n1, len(vec_var)))
         if 8.0 * len(vec_var) ** 2 * 1e-9 > max_mem_ci or davidson:
             vec_var.set_vector(orthonormalize(vec_var.coeffs))
             e0, vec_var = tps_ci_davidson(vec_var, cluster_ops, clustered_ham, conv_thresh=ci_conv,

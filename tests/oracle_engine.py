@@ -82,6 +82,16 @@ class OracleEngine:
             self._local[key] = hit
         return hit[1]
 
+    def s2_operator(self, cluster_ops):
+        from fermicg_amd.operator import extract_S2
+        sizes = tuple(cluster_sizes_of(cluster_ops))
+        key = ("S2", sizes)
+        hit = self._local.get(key)
+        if hit is None:
+            hit = (None, extract_S2(len(sizes), sizes))
+            self._local[key] = hit
+        return hit[1]
+
     def open_sigma(self, ket_state, target_state, cluster_ops, clustered_ham, nbody=4,
                    thresh=0.0):
         orc = self.oracle(ket_state, cluster_ops, clustered_ham, nbody)
