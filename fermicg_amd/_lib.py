@@ -40,7 +40,7 @@ EXPORTS = [
 class DavidsonOpts(ctypes.Structure):
     _fields_ = [("conv_thresh", c_f64), ("max_iter", ctypes.c_int), ("max_ss_vecs", ctypes.c_int),
                 ("lindep_thresh", c_f64), ("precision", ctypes.c_int), ("refresh_k", ctypes.c_int),
-                ("precond", ctypes.c_int)]
+                ("precond", ctypes.c_int), ("precond_diag", P(c_f64))]
 
 
 class DavidsonStats(ctypes.Structure):
@@ -383,7 +383,9 @@ class Plan(_Handle):
         return out
 
     def davidson(self, v0, conv_thresh=1e-8, max_iter=200, max_ss_vecs=12, lindep_thresh=1e-12,
-                 precision="fp64", refresh_k=0, precond=True):
+                 precision="fp64", refresh_k=0, precond=True, precond_diag=None):
+        """Lowest R eigenpairs from start vectors v0 (n x R); precond_diag
+        (n) replaces H's diagonal in the preconditioner."""
         v0 = np.asarray(v0, dtype=np.float64)
         if v0.ndim == 1:
             v0 = v0[:, None]
@@ -391,8 +393,13 @@ class Plan(_Handle):
         vv = np.ascontiguousarray(v0.T)
         ev = np.zeros(R)
         vec = np.zeros((R, self.nvar))
+        pd = None
+        if precond_diag is not None:
+            pd = np.ascontiguousarray(precond_diag, dtype=np.float64)
+            if pd.shape != (self.nvar,):
+                raise ValueError("precond_diag must have the space's length")
         o = DavidsonOpts(conv_thresh, max_iter, max_ss_vecs, lindep_thresh, PRECISION[precision],
-                         refresh_k, 1 if precond else 0)
+                         refresh_k, 1 if precond else 0, _p(pd, c_f64) if pd is not None else None)
         st = DavidsonStats()
         rc = self.ctx.L.fcg_davidson(self.ctx.h, self.h, R, _p(vv, c_f64), ctypes.byref(o),
                                      _p(ev, c_f64), _p(vec, c_f64), ctypes.byref(st))

@@ -240,17 +240,105 @@ def build_sigma(sigma_vector, ci_vector, cluster_ops, clustered_ham, nbody=4, ca
     return sigma_vector
 
 
+def compute_s2(ci_vector, cluster_ops, engine=None):
+    """<r|S^2|r> per root of a BST state (the report of ci_solve,
+    src/tucker_outer.jl:199-211: build_sigma! with extract_S2, then the
+    overlap); the S^2 plan is built for the call and freed."""
+    eng = _engine(engine)
+    sp = _lib.DeviceSpace.bst(eng.ctx, ci_vector.flat())
+    pl = _lib.Plan.create(eng.ctx, eng.ham(eng.s2_operator(cluster_ops), 2),
+                          eng.ops(cluster_ops), sp, nbody=2, precisions=("fp64",))
+    s = pl.sigma(ci_vector.coeffs)
+    del pl, sp
+    return np.einsum("ir,ir->r", ci_vector.coeffs, s)
+
+
+def _mode_product(core, ranks, mats):
+    """core (prod(ranks) x R, each column a column-major tensor of shape
+    ranks) with mode c multiplied by mats[c]' (core' = core x_c mats[c]')."""
+    R = core.shape[1]
+    out = np.empty((int(np.prod([m.shape[1] for m in mats])), R))
+    for r in range(R):
+        T = core[:, r].reshape(ranks, order="F")
+        for c, F in enumerate(mats):
+            T = np.moveaxis(np.tensordot(T, F, axes=([c], [0])), -1, c)
+        out[:, r] = T.ravel(order="F")
+    return out
+
+
+def form_1body_operator_diagonal(ci_vector, cluster_ops, H0="Hcmf", pseudo_canon=True):
+    """Zeroth-order diagonal for the BST preconditioner
+    (form_1body_operator_diagonal!, src/tucker_pt.jl:284-355).  Per block
+    and cluster, the local operator H0 on the block's TuckerConfig range is
+    projected on the Tucker factor, U' H0 U.  With pseudo_canon it is
+    diagonalised and the factor and core rotated by its eigenvectors
+    (transform_basis!, src/hosvd.jl:621-631: U <- U F, core <- core x F',
+    so the state is unchanged), the energies being its eigenvalues;
+    without, they are its diagonal.  Returns (state in the rotated basis,
+    diagonal of sum_c H0_c over its variables)."""
+    out = BSTstate(ci_vector.N, ci_vector.R)
+    out.focks = list(ci_vector.focks)
+    off = ci_vector.var_offsets()
+    hd = np.zeros(len(ci_vector))
+    cores = []
+    for b, (fi, rgs, facs) in enumerate(ci_vector.blocks):
+        f = ci_vector.focks[fi]
+        newf, rots, ens = [], [], []
+        for c in range(ci_vector.N):
+            lo, hi = rgs[c]
+            Hi = np.asarray(cluster_ops[c][H0][(f[c], f[c])])[lo - 1:hi, lo - 1:hi]
+            U = facs[c] if facs[c] is not None else np.eye(hi - lo + 1)
+            Hu = U.T @ Hi @ U
+            if pseudo_canon and U.shape[1] > 1:
+                w, F = np.linalg.eigh(0.5 * (Hu + Hu.T))
+                newf.append(np.asfortranarray(U @ F))
+                rots.append(F)
+                ens.append(w)
+            else:
+                newf.append(facs[c])
+                rots.append(np.eye(U.shape[1]))
+                ens.append(np.diag(Hu).copy())
+        ranks = [len(e) for e in ens]
+        D = ens[0]
+        for c in range(1, ci_vector.N):
+            D = np.add.outer(ens[c], D)  # first cluster fastest
+        hd[off[b]:off[b + 1]] = D.ravel()
+        core = ci_vector.coeffs[off[b]:off[b + 1]]
+        cores.append(_mode_product(core, ranks, rots) if pseudo_canon else core.copy())
+        out.blocks.append((fi, list(rgs), newf if pseudo_canon else list(facs)))
+    out.coeffs = np.concatenate(cores, axis=0) if cores else np.zeros((0, ci_vector.R))
+    return out, hd
+
+
 def ci_solve(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, max_ss_vecs=12,
              max_iter=40, lindep_thresh=1e-10, precond=True, nbody=4, precision="fp64",
-             refresh_k=0, engine=None):
-    """Davidson over the concatenated cores (reference ci_solve).  Returns
-    (energies, BSTstate with the eigenvectors)."""
-    pl = cache_hamiltonian(ci_vector, ci_vector, cluster_ops, clustered_ham, nbody=nbody,
+             refresh_k=0, verbose=0, engine=None):
+    """Davidson over the concatenated cores (reference ci_solve,
+    src/tucker_outer.jl:68-215).  As there, with precond the Tucker factors
+    are first rotated to the pseudo-canonical basis of the clusters' Hcmf
+    and the Hcmf diagonal there preconditions the solve
+    (form_1body_operator_diagonal!, :91-96); the cached Hamiltonian is built
+    in that basis, and the eigenvectors are returned in it.  Returns
+    (energies, BSTstate with the eigenvectors; .s2 holds <S^2> per root)."""
+    vec = ci_vector
+    hd = None
+    if precond:
+        if any("Hcmf" not in ops for ops in cluster_ops):
+            raise ValueError("ci_solve(precond=True) needs the clusters' Hcmf (add_cmf_operators)")
+        vec, hd = form_1body_operator_diagonal(ci_vector, cluster_ops, H0="Hcmf",
+                                               pseudo_canon=True)
+    pl = cache_hamiltonian(vec, vec, cluster_ops, clustered_ham, nbody=nbody,
                            precisions=("fp64", precision), engine=engine)
-    e, v, stats = pl.davidson(ci_vector.coeffs, conv_thresh=conv_thresh, max_iter=max_iter,
+    e, v, stats = pl.davidson(vec.coeffs, conv_thresh=conv_thresh, max_iter=max_iter,
                               max_ss_vecs=max_ss_vecs, lindep_thresh=lindep_thresh,
-                              precision=precision, refresh_k=refresh_k, precond=precond)
-    out = ci_vector.copy_structure()
+                              precision=precision, refresh_k=refresh_k, precond=precond,
+                              precond_diag=hd)
+    out = vec.copy_structure()
     out.set_vector(v)
     out.davidson_stats = stats
+    out.s2 = compute_s2(out, cluster_ops, engine=engine)
+    if verbose:
+        print(" %5s %12s %12s" % ("Root", "Energy", "S2"))
+        for r in range(len(e)):
+            print(" %5d %12.8f %12.8f" % (r + 1, e[r], abs(out.s2[r])))
     return e, out

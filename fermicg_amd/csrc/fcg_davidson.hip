@@ -285,7 +285,12 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
     ctx->err = "Davidson workspace allocation failed";
     return FCG_ENOMEM;
   }
-  FCG_HIP(fcg_launch_diag(pl, d.hd, ctx->stream), ctx);
+  if (o->precond_diag)
+    FCG_HIP(hipMemcpyAsync(d.hd, o->precond_diag, sizeof(double) * n, hipMemcpyHostToDevice,
+                           ctx->stream),
+            ctx);
+  else
+    FCG_HIP(fcg_launch_diag(pl, d.hd, ctx->stream), ctx);
   FCG_HIP(hipMemcpyAsync(d.T, v0, sizeof(double) * n * R, hipMemcpyHostToDevice, ctx->stream),
           ctx);
   int kept = 0;

@@ -197,6 +197,10 @@ typedef struct {
   int precision;        /* FCG_FP64 / FCG_FP32 / FCG_BF16X2 / FCG_BF16 */
   int refresh_k;        /* extra FP64 refresh period (0: restarts only) */
   int precond;          /* 1: diagonal (Davidson) preconditioner */
+  /* host (n_var) diagonal for the preconditioner, or NULL for the diagonal
+   * of H; ci_solve passes the H0 (Hcmf) diagonal of the pseudo-canonical
+   * Tucker basis (form_1body_operator_diagonal!, src/tucker_pt.jl:284-355) */
+  const double *precond_diag;
 } fcg_davidson_opts;
 
 typedef struct {

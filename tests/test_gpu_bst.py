@@ -76,6 +76,32 @@ def test_ci_solve_vs_dense(eng):
         assert np.abs(e2 - w).max() < 1e-6
 
 
+def test_ci_solve_pseudo_canonical_precond(eng):
+    # the reference's preconditioner (Hcmf diagonal in the pseudo-canonical
+    # Tucker basis) and the plain diagonal of H reach the same energies; the
+    # preconditioned result is returned in the rotated basis, where it is an
+    # eigenvector of the projected H (checked through the embedding)
+    from fermicg_amd.bst import form_1body_operator_diagonal
+    p, bst, Hb = _small(12, 3, 8, 3, 4)
+    w = np.linalg.eigvalsh(0.5 * (Hb + Hb.T))[:2]
+    e1, v1 = ci_solve(bst, p.cluster_ops, p.clustered_ham, conv_thresh=1e-9, max_iter=300,
+                      precond=True, engine=eng)
+    e0, v0 = ci_solve(bst, p.cluster_ops, p.clustered_ham, conv_thresh=1e-9, max_iter=300,
+                      precond=False, engine=eng)
+    print("pseudo-canonical", v1.davidson_stats["iterations"], "plain",
+          v0.davidson_stats["iterations"], "s2", v1.s2, flush=True)
+    assert np.abs(e1 - w).max() < 1e-10 and np.abs(e0 - w).max() < 1e-10
+    rot, _ = form_1body_operator_diagonal(bst, p.cluster_ops)
+    assert rot.signature() == v1.signature()
+    tps = TPSCIstate.full_blocks(p.bases, p.focks)
+    U = embedding(rot, p.bases, tps)
+    H = U.T @ Oracle(flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
+                     flatten_ham(p.clustered_ham), tps).full_H() @ U
+    r = H @ v1.coeffs - v1.coeffs * e1[None, :]
+    assert np.abs(r).max() < 1e-8
+    assert np.all(np.isfinite(v1.s2))
+
+
 def test_bst_sigma_sampled_rows_four_clusters(eng):
     # 4 clusters of 4 orbitals, M=8, Tucker ranks <= 4: whole-space GPU sigma
     # against the C restatement of build_sigma! on 300 sampled bra blocks
