@@ -34,6 +34,7 @@ EXPORTS = [
     "fcg_set_build_class_cap",
     "fcg_open_sigma",
     "fcg_sigma_device_noalloc",
+    "fcg_tpsci_pt2_sparse",
 ]
 
 
@@ -53,6 +54,12 @@ class PT2Stats(ctypes.Structure):
     _fields_ = [("n_target_fock", c_i64), ("n_target_var", c_i64), ("n_first_order", c_i64),
                 ("block_pairs", c_i64), ("block_elems", c_i64), ("seconds", c_f64),
                 ("build_flops", c_f64), ("build_seconds", c_f64)]
+
+
+class PT2SparseStats(ctypes.Structure):
+    _fields_ = [("n_items", c_i64), ("n_groups", c_i64), ("n_contributions", c_i64),
+                ("n_configs", c_i64), ("n_first_order", c_i64), ("chain_flops", c_f64),
+                ("seconds", c_f64)]
 
 
 class FCGError(RuntimeError):
@@ -111,6 +118,9 @@ def load():
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_apply_variant_counts.argtypes = [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
     L.fcg_set_build_class_cap.argtypes = [VP, ctypes.c_int]
+    L.fcg_tpsci_pt2_sparse.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64),
+                                       ctypes.c_int, ctypes.c_int, c_f64, P(c_f64),
+                                       P(PT2SparseStats)]
     L.fcg_open_sigma.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(c_f64), c_f64,
                                  P(c_f64), P(ctypes.c_uint8)]
     L.fcg_plan_destroy.argtypes = [VP]
@@ -426,6 +436,24 @@ def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
                                   nbody, h0_op, float(thresh), _p(e2, c_f64), ctypes.byref(st)),
               "fcg_tpsci_pt2")
     return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}
+
+
+def tpsci_pt2_sparse(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
+    """Second-order energies over the sparse first-order space
+    (fcg_tpsci_pt2_sparse); returns (e2, stats dict)."""
+    c = np.asarray(c, dtype=np.float64)
+    if c.ndim == 1:
+        c = c[:, None]
+    R = c.shape[1]
+    cc = np.ascontiguousarray(c.T)
+    e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (R,)))
+    e2 = np.zeros(R)
+    st = PT2SparseStats()
+    ctx.check(ctx.L.fcg_tpsci_pt2_sparse(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64),
+                                         _p(e0, c_f64), nbody, h0_op, float(thresh),
+                                         _p(e2, c_f64), ctypes.byref(st)),
+              "fcg_tpsci_pt2_sparse")
+    return e2, {k: getattr(st, k) for k, _ in PT2SparseStats._fields_}
 
 
 def open_sigma(ctx, ham, ops, bra, ket, c, nbody=4, thresh=0.0):

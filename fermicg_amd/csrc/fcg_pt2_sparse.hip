@@ -1,0 +1,1230 @@
+// Second-order energy over a sparse first-order space, on the GPU.
+//
+// The dense-target PT2 of fcg_pt2.hip forms sigma on every configuration of
+// each target Fock block; at many clusters those blocks are astronomically
+// large (32^12 configurations at 12 clusters of M=32), and only the
+// configurations that some screened contribution reaches matter.  This
+// engine follows the reference's own formulation instead
+// (compute_pt2_energy, src/tpsci_pt2_energy.jl:10-175, with _pt2_job
+// :178-266 and contract_matvec_thread, src/tpsci_matvec_thread.jl:279-818):
+// contributions are generated per (ket configuration J, term t) as every
+// bra state combination I_A of the term's active clusters,
+//
+//     x = (I_A, J on the other clusters),   H_t[x, J] = s_t sum_p V[p] prod_a Γ_a[p_a, I_a, J_a],
+//
+// kept when |H_t[x, J]| > thresh / max_r |c[J, r]| (_collect_significant_thread!
+// :830-895), and summed per x before sigma_x^2 / (E0 - Hd_x) is summed over
+// the x outside the variational space (project_out!, _sum_pt2 :269-282).
+//
+// Work and pruning, all bounds being upper bounds of |H_t[x, J]| so that
+// nothing that passes the element test is skipped (the reference's
+// upper_bound_thread / upper_bound2_thread, :897-1283):
+//   items   (ket Fock block F, term t) pairs whose target F + Δ_t has every
+//           cluster sector in ClusterOps (the job filter, :49-79) and whose
+//           Γ blocks exist (missing blocks contribute nothing, as in
+//           contract_matvec_thread); kept when
+//           sum_p |V[p]| prod_a max|Γ_a[p_a, :, :]| * max_{J in F} |c_J| > thresh;
+//   groups  inside an item, the kets with equal states on the active
+//           clusters share every element; a group is kept when
+//           sum_p |V[p]| prod_a max_I |Γ_a[p_a, I, J_a]| * max_J |c_J| > thresh;
+//   indices bra states I_a whose per-index bound cannot pass are dropped
+//           before the contraction (upper_bound2_thread's survivor lists).
+// The kept contraction is the reference's chain (contract the last active
+// cluster first), tiled over the last cluster's surviving states so the
+// intermediates stay in LDS.
+//
+// Accumulation is deterministic: a count pass and a fill pass over the same
+// work write every kept contribution (key = target Fock sectors | states of
+// x, value = H_t[x,J] c[J,:]) at positions fixed by prefix sums; a stable
+// radix sort on the 128-bit keys keeps equal keys in that order, and each
+// x's contributions are summed in it.  So e2 does not depend on scheduling.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fcg_internal.h"
+
+namespace {
+
+constexpr int PT = 256;          // threads per workgroup
+constexpr int KCH = 256;         // kets per chunk of a ket Fock block
+constexpr int ARENA = 8192;      // chain intermediates (doubles)
+constexpr int VMAX = 1024;       // staged integrals (doubles)
+constexpr int SMAX = 64;         // bra states per cluster the survivor lists hold
+constexpr double BOUND_SLACK = 1.0 + 1e-10;  // bounds are inflated against rounding
+
+struct PTerm {
+  int32_t nbody;
+  int32_t cl[4];
+  int32_t op[4];
+  int32_t dims[4];
+  int32_t odd;    // bit q: operator string q has odd fermion parity
+  int32_t group;  // TransferConfig group
+  int64_t ioff;
+};
+
+struct Key128 {
+  uint64_t hi, lo;  // hi: target Fock sector codes, lo: states
+};
+
+struct KeyDecomposer {
+  __host__ __device__ rocprim::tuple<uint64_t &, uint64_t &> operator()(Key128 &k) const {
+    return rocprim::tuple<uint64_t &, uint64_t &>(k.hi, k.lo);
+  }
+};
+
+struct Pt2Args {
+  int N, S, R, nbody, fbits, sbits;
+  double thresh;
+  // ket space
+  int64_t n_fock, n_cfg;
+  const int64_t *fstart;  // [n_fock+1]
+  const int16_t *fock;    // [n_fock][N][2]
+  const uint8_t *cst;     // [n_cfg][N] 0-based states
+  const double *cabs;     // [n_cfg] max_r |c|
+  const double *thr;      // [n_cfg] thresh / cabs (the reference's thresh_curr)
+  const double *cmaxF;    // [n_fock]
+  const double *c;        // [R][n_cfg]
+  // operator
+  const PTerm *terms;
+  const int32_t *tc_start;  // [n_tc+1]
+  const int16_t *tc;        // [n_tc][N][2]
+  int n_tc;
+  const double *ints;
+  const int32_t *gblock;  // [N][n_ops][S*S][S*S]
+  int n_ops;
+  const int64_t *gb_off;
+  const int32_t *gb_dims;
+  const double *gdata;
+  const int32_t *nstates;  // [N][S*S]
+  const int64_t *bmax_off;  // per Γ block: np values of max_{I,J} |Γ[p,I,J]|
+  const double *bmax;
+  const int64_t *cmx_off;   // per Γ block: np x m_ket values of max_I |Γ[p,I,J]|
+  const double *cmx;
+  // items
+  int64_t *item_count;      // count pass: items per ket Fock block
+  const int64_t *item_ptr;  // fill pass
+  int32_t *item_f, *item_t;
+  int64_t fb_base;
+  // emission
+  const int32_t *items_f, *items_t;
+  int64_t item_base, n_items;
+  int64_t *emit_count;      // count pass: kept contributions per item
+  const int64_t *emit_ptr;  // fill pass: first position of each item
+  Key128 *ekey;
+  double *eval;             // [E][R]
+  unsigned long long *stats;  // [0] groups kept, [1] chain flops, [2] overflow
+};
+
+__device__ __forceinline__ int sector_code(int na, int nb, int S) { return na * S + nb; }
+
+__device__ __forceinline__ int32_t gblk(const Pt2Args &a, int c, int op, int cb, int ck) {
+  const int SS = a.S * a.S;
+  return a.gblock[(((int64_t)c * a.n_ops + op) * SS + cb) * SS + ck];
+}
+
+// ---- items: (ket Fock block, term) pairs that can contribute ----
+template <bool FILL>
+__global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
+  __shared__ int16_t sF[2 * FCG_MAXN];
+  __shared__ int32_t sScan[PT];
+  __shared__ int64_t sBase;
+  const int tid = threadIdx.x;
+  const int64_t fj = a.fb_base + blockIdx.x;
+  const int N = a.N, S = a.S;
+  if (tid < 2 * N) sF[tid] = a.fock[fj * 2 * N + tid];
+  if (tid == 0) sBase = 0;
+  __syncthreads();
+  const double cm = a.cmaxF[fj];
+  int64_t total = 0;
+  for (int g0 = 0; g0 < a.n_tc; g0 += PT) {
+    const int g = g0 + tid;
+    int cnt = 0;
+    int32_t tlist[8];
+    int nl = 0;
+    bool more = false;
+    if (g < a.n_tc && cm > 0.0) {
+      const int16_t *d = a.tc + (int64_t)g * 2 * N;
+      bool ok = true;
+      for (int c = 0; c < N && ok; c++) {
+        const int na = sF[2 * c] + d[2 * c], nb = sF[2 * c + 1] + d[2 * c + 1];
+        ok = na >= 0 && nb >= 0 && na < S && nb < S &&
+             a.nstates[(int64_t)c * S * S + sector_code(na, nb, S)] > 0;
+      }
+      if (ok) {
+        for (int t = a.tc_start[g]; t < a.tc_start[g + 1]; t++) {
+          const PTerm T = a.terms[t];
+          if (T.nbody > a.nbody) continue;
+          int32_t blk[4];
+          bool have = true;
+          for (int q = 0; q < T.nbody; q++) {
+            const int c = T.cl[q];
+            const int ck = sector_code(sF[2 * c], sF[2 * c + 1], S);
+            const int cb = sector_code(sF[2 * c] + d[2 * c], sF[2 * c + 1] + d[2 * c + 1], S);
+            blk[q] = gblk(a, c, T.op[q], cb, ck);
+            have = have && blk[q] >= 0;
+          }
+          if (!have) continue;
+          double B = 0.0;
+          if (T.nbody == 1) {
+            B = a.bmax[a.bmax_off[blk[0]]];
+          } else {
+            const double *m0 = a.bmax + a.bmax_off[blk[0]];
+            const double *m1 = a.bmax + a.bmax_off[blk[1]];
+            const double *m2 = T.nbody > 2 ? a.bmax + a.bmax_off[blk[2]] : nullptr;
+            const double *m3 = T.nbody > 3 ? a.bmax + a.bmax_off[blk[3]] : nullptr;
+            const int d0 = T.dims[0], d1 = T.dims[1];
+            const int d2 = T.nbody > 2 ? T.dims[2] : 1, d3 = T.nbody > 3 ? T.dims[3] : 1;
+            const double *V = a.ints + T.ioff;
+            for (int p3 = 0; p3 < d3; p3++)
+              for (int p2 = 0; p2 < d2; p2++) {
+                const double f23 = (m2 ? m2[p2] : 1.0) * (m3 ? m3[p3] : 1.0);
+                if (f23 == 0.0) continue;
+                for (int p1 = 0; p1 < d1; p1++) {
+                  const double f = f23 * m1[p1];
+                  if (f == 0.0) continue;
+                  const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
+                  double s = 0.0;
+                  for (int p0 = 0; p0 < d0; p0++) s += fabs(Vp[p0]) * m0[p0];
+                  B += s * f;
+                }
+              }
+          }
+          if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) continue;
+          if (FILL) {
+            if (nl < 8) tlist[nl++] = t;
+            else more = true;
+          }
+          cnt++;
+        }
+      }
+    }
+    if (!FILL) {
+      total += cnt;
+    } else {
+      // ordered compaction: block-wide exclusive scan of the counts
+      sScan[tid] = cnt;
+      __syncthreads();
+      for (int off = 1; off < PT; off <<= 1) {
+        const int v = tid >= off ? sScan[tid - off] : 0;
+        __syncthreads();
+        sScan[tid] += v;
+        __syncthreads();
+      }
+      int64_t at = a.item_ptr[fj] + sBase + sScan[tid] - cnt;
+      if (cnt > 0) {
+        if (!more) {
+          for (int i = 0; i < nl; i++) {
+            a.item_f[at + i] = (int32_t)fj;
+            a.item_t[at + i] = tlist[i];
+          }
+        } else {
+          // more kept terms than the register list holds: recompute in order
+          const int16_t *d = a.tc + (int64_t)g * 2 * N;
+          for (int t = a.tc_start[g]; t < a.tc_start[g + 1]; t++) {
+            const PTerm T = a.terms[t];
+            if (T.nbody > a.nbody) continue;
+            int32_t blk[4];
+            bool have = true;
+            for (int q = 0; q < T.nbody; q++) {
+              const int c = T.cl[q];
+              const int ck = sector_code(sF[2 * c], sF[2 * c + 1], S);
+              const int cb = sector_code(sF[2 * c] + d[2 * c], sF[2 * c + 1] + d[2 * c + 1], S);
+              blk[q] = gblk(a, c, T.op[q], cb, ck);
+              have = have && blk[q] >= 0;
+            }
+            if (!have) continue;
+            double B = 0.0;
+            if (T.nbody == 1) {
+              B = a.bmax[a.bmax_off[blk[0]]];
+            } else {
+              const double *m0 = a.bmax + a.bmax_off[blk[0]];
+              const double *m1 = a.bmax + a.bmax_off[blk[1]];
+              const double *m2 = T.nbody > 2 ? a.bmax + a.bmax_off[blk[2]] : nullptr;
+              const double *m3 = T.nbody > 3 ? a.bmax + a.bmax_off[blk[3]] : nullptr;
+              const int d0 = T.dims[0], d1 = T.dims[1];
+              const int d2 = T.nbody > 2 ? T.dims[2] : 1, d3 = T.nbody > 3 ? T.dims[3] : 1;
+              const double *V = a.ints + T.ioff;
+              for (int p3 = 0; p3 < d3; p3++)
+                for (int p2 = 0; p2 < d2; p2++) {
+                  const double f23 = (m2 ? m2[p2] : 1.0) * (m3 ? m3[p3] : 1.0);
+                  if (f23 == 0.0) continue;
+                  for (int p1 = 0; p1 < d1; p1++) {
+                    const double f = f23 * m1[p1];
+                    if (f == 0.0) continue;
+                    const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
+                    double s = 0.0;
+                    for (int p0 = 0; p0 < d0; p0++) s += fabs(Vp[p0]) * m0[p0];
+                    B += s * f;
+                  }
+                }
+            }
+            if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) continue;
+            a.item_f[at] = (int32_t)fj;
+            a.item_t[at] = t;
+            at++;
+          }
+        }
+      }
+      __syncthreads();
+      if (tid == PT - 1) sBase += sScan[PT - 1];
+      __syncthreads();
+    }
+  }
+  if (!FILL) {
+    sScan[tid] = (int32_t)total;
+    __syncthreads();
+    if (tid == 0) {
+      int64_t s = 0;
+      for (int k = 0; k < PT; k++) s += sScan[k];
+      a.item_count[fj] = s;
+    }
+  }
+}
+
+// block-wide exclusive scan of one int per thread; returns the total
+__device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sScan, int64_t *out_excl) {
+  const int tid = threadIdx.x;
+  sScan[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < PT; off <<= 1) {
+    const int64_t w = tid >= off ? sScan[tid - off] : 0;
+    __syncthreads();
+    sScan[tid] += w;
+    __syncthreads();
+  }
+  *out_excl = sScan[tid] - v;
+  const int64_t tot = sScan[PT - 1];
+  __syncthreads();
+  return tot;
+}
+
+// ---- emission: every kept contribution of an item ----
+// One workgroup per item.  COUNT: emit_count[item] = number of kept
+// contributions; FILL: writes them from emit_ptr[item] on, in the order
+// (ket chunk, group, tile, output element, ket), the same in both passes.
+template <bool FILL>
+__global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
+  __shared__ double sV[VMAX];
+  __shared__ double sA[ARENA];
+  __shared__ double sW[4][64];        // per-cluster bound weights w_q[p]
+  __shared__ double sCM[4][64];       // column maxima of the group's ket states
+  __shared__ int16_t sSurv[4][SMAX];  // surviving bra states per active cluster
+  __shared__ int sNS[4];
+  __shared__ int32_t sKet[KCH];       // kets of the chunk, sorted by active-state key
+  __shared__ uint32_t sKey[KCH];
+  __shared__ int sGrp[KCH + 1];
+  __shared__ int sNG;
+  __shared__ double sRed[PT];
+  __shared__ int64_t sScan[PT];
+  __shared__ int sCodeX[FCG_MAXN];
+  __shared__ uint64_t sFockKey;
+  __shared__ int sBlk[4], sNp[4], sMb[4], sMk[4];
+  __shared__ double sSign;
+  __shared__ int64_t sEmit;  // kept contributions so far (count) / next position (fill)
+  __shared__ int sSkip;
+
+  const int tid = threadIdx.x;
+  const int64_t it = a.item_base + blockIdx.x;
+  if (it >= a.n_items) return;  // uniform
+  const int fj = a.items_f[it];
+  const PTerm T = a.terms[a.items_t[it]];
+  const int N = a.N, S = a.S, nb = T.nbody;
+  const int16_t *F = a.fock + (int64_t)fj * 2 * N;
+  const int16_t *D = a.tc + (int64_t)T.group * 2 * N;
+  if (tid == 0) {
+    uint64_t fk = 0;
+    for (int c = 0; c < N; c++) {
+      const int cx = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
+      sCodeX[c] = cx;
+      fk |= (uint64_t)cx << (a.fbits * c);
+    }
+    sFockKey = fk;
+    int par = 0;
+    for (int q = 0; q < nb; q++) {
+      const int c = T.cl[q];
+      const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
+      const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
+      const int blk = gblk(a, c, T.op[q], cb, ck);
+      sBlk[q] = blk;
+      sNp[q] = a.gb_dims[3 * blk];
+      sMb[q] = a.gb_dims[3 * blk + 1];
+      sMk[q] = a.gb_dims[3 * blk + 2];
+      if ((T.odd >> q) & 1) {
+        int ne = 0;
+        for (int c2 = 0; c2 < c; c2++) ne += F[2 * c2] + F[2 * c2 + 1];
+        par ^= ne & 1;
+      }
+    }
+    sSign = par ? -1.0 : 1.0;
+    sEmit = FILL ? a.emit_ptr[it] : 0;
+    // shapes the staged chain cannot hold are reported, not computed
+    int nv = 1;
+    for (int q = 0; q < nb; q++) nv *= (nb == 1 ? 1 : T.dims[q]);
+    int skip = nv > VMAX;
+    for (int q = 0; q < nb; q++) skip |= sMb[q] > SMAX || sNp[q] > 64 || sMk[q] > 255;
+    sSkip = skip;
+    if (skip) atomicAdd(&a.stats[2], 1ull);
+  }
+  __syncthreads();
+  if (sSkip) {
+    if (!FILL && tid == 0) a.emit_count[it] = 0;
+    return;
+  }
+  const double sign = sSign;
+  int nv = 1;
+  if (nb > 1)
+    for (int q = 0; q < nb; q++) nv *= T.dims[q];
+  for (int e = tid; e < nv; e += PT) sV[e] = nb > 1 ? a.ints[T.ioff + e] : 1.0;
+  const int64_t k0 = a.fstart[fj], k1 = a.fstart[fj + 1];
+  unsigned long long groups_kept = 0, flops = 0;
+  for (int64_t c0 = k0; c0 < k1; c0 += KCH) {
+    const int nk = (int)min((int64_t)KCH, k1 - c0);
+    __syncthreads();
+    // group the chunk's kets by their states on the active clusters
+    for (int j = tid; j < nk; j += PT) {
+      const uint8_t *st = a.cst + (c0 + j) * N;
+      uint32_t key = 0;  // states below 255 (checked on the host): never the sentinel
+      for (int q = 0; q < nb; q++) key |= (uint32_t)st[T.cl[q]] << (8 * q);
+      sKey[j] = key;
+      sKet[j] = j;
+    }
+    for (int j = nk + tid; j < KCH; j += PT) {
+      sKey[j] = 0xFFFFFFFFu;
+      sKet[j] = j;
+    }
+    __syncthreads();
+    // bitonic sort of (key, ket) over KCH entries; ties keep ket order since
+    // ket index is the secondary key
+    for (int k = 2; k <= KCH; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < KCH; i += PT) {
+          const int l = i ^ j;
+          if (l > i) {
+            const bool up = (i & k) == 0;
+            const uint32_t ki = sKey[i], kl = sKey[l];
+            const int vi = sKet[i], vl = sKet[l];
+            const bool gt = ki > kl || (ki == kl && vi > vl);
+            if (gt == up) {
+              sKey[i] = kl;
+              sKey[l] = ki;
+              sKet[i] = vl;
+              sKet[l] = vi;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    if (tid == 0) {
+      int ng = 0;
+      for (int j = 0; j < nk; j++)
+        if (j == 0 || sKey[j] != sKey[j - 1]) sGrp[ng++] = j;
+      sGrp[ng] = nk;
+      sNG = ng;
+    }
+    __syncthreads();
+    const int ng = sNG;
+    for (int gi = 0; gi < ng; gi++) {
+      const int j0 = sGrp[gi], j1 = sGrp[gi + 1];
+      // group's active ket states and its largest |c|
+      const uint8_t *st0 = a.cst + (c0 + sKet[j0]) * N;
+      double cm = 0.0;
+      for (int j = j0 + tid; j < j1; j += PT) cm = fmax(cm, a.cabs[c0 + sKet[j]]);
+      sRed[tid] = cm;
+      __syncthreads();
+      for (int s = PT / 2; s > 0; s >>= 1) {
+        if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
+        __syncthreads();
+      }
+      cm = sRed[0];
+      // column maxima max_I |Γ_q[p, I, J_q]| for the group's ket states
+      for (int q = 0; q < nb; q++) {
+        const int Jq = st0[T.cl[q]];
+        const double *cmx = a.cmx + a.cmx_off[sBlk[q]] + (int64_t)sNp[q] * Jq;
+        for (int p = tid; p < sNp[q]; p += PT) sCM[q][p] = cmx[p];
+      }
+      __syncthreads();
+      // group bound and per-cluster weights w_q[p_q] = sum over the other
+      // indices of |V| prod_{r != q} colmax_r
+      if (nb == 1) {
+        if (tid == 0) sW[0][0] = 1.0;
+      } else {
+        for (int e = tid; e < 4 * 64; e += PT) {
+          const int q = e / 64, pq = e % 64;
+          if (q >= nb || pq >= sNp[q]) continue;
+          int strides[4], st = 1;
+          for (int r = 0; r < nb; r++) {
+            strides[r] = st;
+            st *= T.dims[r];
+          }
+          double s = 0.0;
+          for (int v = 0; v < nv; v++) {
+            int rem = v;
+            double f = fabs(sV[v]);
+            bool mine = true;
+            for (int r = 0; r < nb; r++) {
+              const int pr = rem % T.dims[r];
+              rem /= T.dims[r];
+              if (r == q) mine = pr == pq;
+              else f *= sCM[r][pr];
+            }
+            if (mine) s += f;
+          }
+          sW[q][pq] = s;
+        }
+      }
+      __syncthreads();
+      double B;
+      if (nb == 1) {
+        B = sCM[0][0];
+      } else {
+        B = 0.0;
+        for (int p = 0; p < sNp[0]; p++) B += sW[0][p] * sCM[0][p];
+      }
+      if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) {
+        __syncthreads();
+        continue;  // uniform
+      }
+      // survivor lists: bra states I_q whose bound can pass
+      if (tid < 4) sNS[tid] = 0;
+      __syncthreads();
+      if (tid < nb) {
+        const int q = tid;
+        const int Jq = st0[T.cl[q]];
+        const double *G = a.gdata + a.gb_off[sBlk[q]];
+        const int np = sNp[q], mb = sMb[q];
+        int n = 0;
+        for (int I = 0; I < mb; I++) {
+          double s = 0.0;
+          const double *Gi = G + (int64_t)np * (I + (int64_t)mb * Jq);
+          for (int p = 0; p < np; p++) s += sW[q][p] * fabs(Gi[p]);
+          if (s * BOUND_SLACK * cm > a.thresh && s != 0.0) sSurv[q][n++] = (int16_t)I;
+        }
+        sNS[q] = n;
+      }
+      __syncthreads();
+      bool empty = false;
+      for (int q = 0; q < nb; q++) empty |= sNS[q] == 0;
+      if (empty) {
+        __syncthreads();
+        continue;  // uniform
+      }
+      groups_kept++;
+      // chain: contract the last active cluster first, tiled over its
+      // surviving states so every stored stage fits half the arena.  Stored
+      // stage sizes per tile state: after the last cluster prod_{q<last} d_q,
+      // after cluster q (last-1 .. 1) prod_{r<q} d_r * prod_{r=q}^{last-1} |S_r|;
+      // cluster 0 is contracted per output during the emission
+      const int last = nb - 1;
+      int need = 1;
+      if (nb > 1) {
+        int pre = 1;
+        for (int q = 0; q < last; q++) pre *= T.dims[q];
+        need = pre;
+        int ps = 1;
+        for (int q = last - 1; q >= 1; q--) {
+          pre /= T.dims[q];
+          ps *= sNS[q];
+          need = max(need, pre * ps);
+        }
+      }
+      if (need > ARENA / 2) {  // uniform: flags the call, computes nothing wrong
+        if (tid == 0) atomicAdd(&a.stats[2], 1ull);
+        __syncthreads();
+        continue;
+      }
+      const int ntile = nb == 1 ? 1 : sNS[last];
+      const int tk = nb == 1 ? 1 : min(ntile, (ARENA / 2) / need);
+      const int J0 = st0[T.cl[0]];
+      const double *G0 = a.gdata + a.gb_off[sBlk[0]];
+      for (int t0 = 0; t0 < ntile; t0 += tk) {
+        const int tn = nb == 1 ? 1 : min(tk, ntile - t0);
+        double *cur = sA, *nxt = sA + ARENA / 2;
+        int cols = 1;
+        if (nb > 1) {
+          int pre = 1;
+          for (int q = 0; q < last; q++) pre *= T.dims[q];
+          {  // the last cluster: cur[r + pre * i] = sum_p V[r + pre * p] Γ[p, S[t0+i], J]
+            const int q = last;
+            const int np = sNp[q], mb = sMb[q];
+            const int Jq = st0[T.cl[q]];
+            const double *G = a.gdata + a.gb_off[sBlk[q]];
+            for (int e = tid; e < pre * tn; e += PT) {
+              const int r = e % pre, i = e / pre;
+              const double *Gi = G + (int64_t)np * (sSurv[q][t0 + i] + (int64_t)mb * Jq);
+              double v = 0.0;
+              for (int p = 0; p < np; p++) v += sV[r + pre * p] * Gi[p];
+              cur[e] = v;
+            }
+            flops += 2ull * pre * tn * np;
+          }
+          cols = tn;
+          __syncthreads();
+          for (int q = last - 1; q >= 1; q--) {
+            // nxt[r2 + pre2 * (i + ns * col)] = sum_p cur[r2 + pre2 * p + pre * col] Γ_q[p, S_q[i], J]
+            const int np = sNp[q], mb = sMb[q];
+            const int Jq = st0[T.cl[q]];
+            const double *G = a.gdata + a.gb_off[sBlk[q]];
+            const int pre2 = pre / np, ns = sNS[q];
+            for (int e = tid; e < pre2 * ns * cols; e += PT) {
+              const int r2 = e % pre2, rem = e / pre2, i = rem % ns, col = rem / ns;
+              const double *Gi = G + (int64_t)np * (sSurv[q][i] + (int64_t)mb * Jq);
+              double v = 0.0;
+              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * col] * Gi[p];
+              nxt[e] = v;
+            }
+            flops += 2ull * pre2 * ns * cols * np;
+            __syncthreads();
+            double *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            pre = pre2;
+            cols *= ns;
+          }
+        }
+        const int ns0 = sNS[0];
+        const int nout = ns0 * cols;
+        flops += 2ull * nout * (nb == 1 ? 0 : sNp[0]);
+        // output e = i0 + ns0 * col, col = i_1 + |S_1| (i_2 + ... (tile index));
+        // kept contributions counted (and, filling, written in this order)
+        const int64_t base = sEmit;  // read before the scan's barriers
+        for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
+          int64_t pos = 0, cnt = 0;
+          if (pass == 1) pos = base + sScan[tid];  // exclusive prefix of pass 0
+          for (int e = tid; e < nout; e += PT) {
+            const int i0 = e % ns0, col = e / ns0;
+            double x;
+            if (nb == 1) {
+              x = G0[sSurv[0][i0] + (int64_t)sMb[0] * J0];  // 2-D block (np = 1)
+            } else {
+              const int np = sNp[0];
+              const double *Gi = G0 + (int64_t)np * (sSurv[0][i0] + (int64_t)sMb[0] * J0);
+              double v = 0.0;
+              for (int p = 0; p < np; p++) v += cur[p + np * col] * Gi[p];
+              x = v;
+            }
+            const double ax = fabs(x);
+            int bstate[4] = {sSurv[0][i0], 0, 0, 0};
+            if (nb > 1) {
+              int rem = col;
+              for (int q = 1; q < last; q++) {
+                bstate[q] = sSurv[q][rem % sNS[q]];
+                rem /= sNS[q];
+              }
+              bstate[last] = sSurv[last][t0 + rem];
+            }
+            for (int j = j0; j < j1; j++) {
+              const int64_t J = c0 + sKet[j];
+              if (!(ax > a.thr[J])) continue;  // _collect_significant_thread!'s test
+              if (pass == 1) {
+                // key: target Fock sectors | bra states on the active
+                // clusters, the ket's states elsewhere
+                const uint8_t *sj = a.cst + J * N;
+                uint64_t lo = 0;
+                for (int cc = 0; cc < N; cc++) {
+                  int sv = sj[cc];
+                  for (int q = 0; q < nb; q++)
+                    if (T.cl[q] == cc) sv = bstate[q];
+                  lo |= (uint64_t)sv << (a.sbits * cc);
+                }
+                a.ekey[pos] = Key128{sFockKey, lo};
+                const double v = sign * x;
+                for (int r = 0; r < a.R; r++)
+                  a.eval[pos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + J];
+                pos++;
+              }
+              cnt++;
+            }
+          }
+          if (pass == 0) {
+            int64_t excl;
+            const int64_t tot = block_scan(cnt, sScan, &excl);
+            // block_scan ends with a barrier: every thread has read sEmit
+            if (FILL) {
+              sScan[tid] = excl;  // own slot only; read back by this thread in pass 1
+            }
+            if (tid == 0) sEmit = base + tot;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  if (!FILL && tid == 0) a.emit_count[it] = sEmit;
+  if (tid == 0 && groups_kept) atomicAdd(&a.stats[0], groups_kept);
+  // every thread counted the same (uniform) chain flops
+  if (tid == 0 && flops) atomicAdd(&a.stats[1], flops);
+}
+
+// per Γ block: bmax[p] = max_{I,J} |Γ[p,I,J]|, cmx[p + np J] = max_I |Γ[p,I,J]|
+__global__ void k_gamma_max(int64_t n_blocks, const int64_t *gb_off, const int32_t *gb_dims,
+                            const double *gdata, const int64_t *bmax_off, double *bmax,
+                            const int64_t *cmx_off, double *cmx) {
+  const int64_t b = blockIdx.x;
+  if (b >= n_blocks) return;
+  const int np = gb_dims[3 * b], mb = gb_dims[3 * b + 1], mk = gb_dims[3 * b + 2];
+  const double *G = gdata + gb_off[b];
+  for (int e = threadIdx.x; e < np * mk; e += blockDim.x) {
+    const int p = e % np, J = e / np;
+    double m = 0.0;
+    for (int I = 0; I < mb; I++) m = fmax(m, fabs(G[p + (int64_t)np * (I + (int64_t)mb * J)]));
+    cmx[cmx_off[b] + e] = m;
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    double m = 0.0;
+    for (int J = 0; J < mk; J++) m = fmax(m, cmx[cmx_off[b] + p + (int64_t)np * J]);
+    bmax[bmax_off[b] + p] = m;
+  }
+}
+
+// segment heads of the sorted keys (x changes)
+__global__ void k_heads(int64_t n, const Key128 *k, uint32_t *head) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    head[i] = (i == 0 || k[i].hi != k[i - 1].hi || k[i].lo != k[i - 1].lo) ? 1u : 0u;
+}
+
+__global__ void k_seg_start(int64_t n, const uint32_t *head, const uint64_t *segid,
+                            int64_t *seg_start) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (head[i]) seg_start[segid[i]] = i;
+}
+
+__device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
+  const double t = sum + v;
+  comp += fabs(sum) >= fabs(v) ? (sum - t) + v : (v - t) + sum;
+  sum = t;
+}
+
+__device__ __forceinline__ bool key_less(const Key128 &a, const Key128 &b) {
+  return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+
+struct SumArgs {
+  int N, S, R, n_ops, h0_op, fbits, sbits;
+  int64_t n_seg;
+  const int64_t *seg_start;  // [n_seg + 1]
+  const uint64_t *perm;      // sorted position -> emission index
+  const Key128 *skey;        // sorted keys
+  const double *eval;        // [E][R] in emission order
+  const Key128 *vkey;        // variational configurations, sorted
+  int64_t n_v;
+  const int32_t *gblock;
+  const int64_t *gb_off;
+  const int32_t *gb_dims;
+  const double *gdata;
+  const double *e0;
+  double *part;              // [blocks][R][2]
+  unsigned long long *n_out; // configurations outside the variational space
+};
+
+// per x: sigma = sum of its contributions in sorted order; outside the
+// variational space, sigma^2 / (E0 - Hd) with Hd = sum_c H0_c[x_c, x_c]
+__global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
+  __shared__ double ss[256], sc[256];
+  __shared__ unsigned long long sOut;
+  if (threadIdx.x == 0) sOut = 0;
+  __syncthreads();
+  const int R = a.R;
+  double sum[FCG_MAX_R], comp[FCG_MAX_R];
+  for (int r = 0; r < FCG_MAX_R; r++) sum[r] = comp[r] = 0.0;
+  unsigned long long nout = 0;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n_seg;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = a.seg_start[s], e = a.seg_start[s + 1];
+    const Key128 k = a.skey[b];
+    // variational configurations are projected out
+    int64_t lo = 0, hi = a.n_v;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (key_less(a.vkey[mid], k)) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < a.n_v && a.vkey[lo].hi == k.hi && a.vkey[lo].lo == k.lo) continue;
+    nout++;
+    double hd = 0.0;
+    const int SS = a.S * a.S;
+    for (int c = 0; c < a.N; c++) {
+      const int code = (int)((k.hi >> (a.fbits * c)) & ((1ull << a.fbits) - 1));
+      const int xs = (int)((k.lo >> (a.sbits * c)) & ((1ull << a.sbits) - 1));
+      const int blk = a.gblock[(((int64_t)c * a.n_ops + a.h0_op) * SS + code) * SS + code];
+      const int mb = a.gb_dims[3 * blk + 1];
+      hd += a.gdata[a.gb_off[blk] + xs + (int64_t)mb * xs];
+    }
+    for (int r = 0; r < R; r++) {
+      double v = 0.0;
+      for (int64_t i = b; i < e; i++) v += a.eval[a.perm[i] * R + r];
+      neumaier(sum[r], comp[r], v * v / (a.e0[r] - hd));
+    }
+  }
+  if (nout) atomicAdd(&sOut, nout);
+  for (int r = 0; r < R; r++) {
+    ss[threadIdx.x] = sum[r];
+    sc[threadIdx.x] = comp[r];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0, c = 0.0;
+      for (int t = 0; t < (int)blockDim.x; t++) {
+        neumaier(s, c, ss[t]);
+        c += sc[t];
+      }
+      a.part[((int64_t)blockIdx.x * R + r) * 2] = s;
+      a.part[((int64_t)blockIdx.x * R + r) * 2 + 1] = c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sOut) atomicAdd(a.n_out, sOut);
+}
+
+__global__ void k_iota(int64_t n, uint64_t *p) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (uint64_t)i;
+}
+
+template <typename T>
+struct Buf {
+  fcg_ctx *ctx = nullptr;
+  T *p = nullptr;
+  explicit Buf(fcg_ctx *c) : ctx(c) {}
+  ~Buf() {
+    if (p) fcg_dfree(ctx, p);
+  }
+  bool alloc(int64_t n) {
+    if (p) fcg_dfree(ctx, p);
+    p = static_cast<T *>(fcg_dmalloc(ctx, sizeof(T) * std::max<int64_t>(n, 1)));
+    return p != nullptr;
+  }
+  int upload(const T *src, int64_t n) {
+    if (!alloc(n)) return FCG_ENOMEM;
+    if (n > 0)
+      FCG_HIP(hipMemcpyAsync(p, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream), ctx);
+    return FCG_OK;
+  }
+};
+
+int bits_for(int64_t n) {  // bits to hold values 0 .. n-1
+  int b = 0;
+  while ((int64_t(1) << b) < n) b++;
+  return std::max(b, 1);
+}
+
+constexpr int64_t FB_CHUNK = 65535;
+constexpr int64_t ITEM_CHUNK = 1 << 20;
+constexpr int SUM_BLOCKS = 1024;
+
+}  // namespace
+
+extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var,
+                                    int R, const double *c, const double *e0, int nbody,
+                                    int h0_op, double thresh, double *e2,
+                                    fcg_pt2_sparse_stats *st) {
+  if (!ctx || !h || !o || !var || !c || !e0 || !e2 || R < 1 || R > FCG_MAX_R || nbody < 1 ||
+      nbody > 4 || h0_op < 0 || h0_op >= o->n_ops || !(thresh >= 0.0))
+    return FCG_EINVAL;
+  if (!var->tpsci) {
+    ctx->err = "fcg_tpsci_pt2_sparse needs a TPSCI space";
+    return FCG_EINVAL;
+  }
+  (void)hipSetDevice(ctx->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  if (int rc = fcg_form_boxes(ctx, o, var)) return rc;
+  const int N = var->N, S = o->S;
+  hipStream_t s = ctx->stream;
+  // configurations of the space (the boxes enumerate them in the caller's
+  // order), grouped by Fock block
+  const int64_t n_cfg = var->nvar, n_fock = var->n_fock;
+  std::vector<uint8_t> cst((size_t)std::max<int64_t>(n_cfg, 1) * N);
+  std::vector<int64_t> fstart(n_fock + 1, 0);
+  int max_state = 1;
+  for (int64_t b = 0; b < var->n_box; b++) {
+    const int32_t f = var->box_fock[b];
+    if (b > 0 && f < var->box_fock[b - 1]) {
+      ctx->err = "fcg_tpsci_pt2_sparse: Fock blocks must be contiguous";
+      return FCG_EINVAL;
+    }
+    const int64_t v0 = var->var_off[b], d = var->var_off[b + 1] - v0;
+    fstart[f + 1] += d;
+    for (int64_t e = 0; e < d; e++) {
+      int64_t r = e;
+      for (int cc = 0; cc < N; cc++) {
+        const int rk = var->rank[b * N + cc];
+        const int st0 = (int)(r % rk) + var->lo[b * N + cc] - 1;
+        r /= rk;
+        if (st0 > 255) {
+          ctx->err = "fcg_tpsci_pt2_sparse: state index above 256";
+          return FCG_EOVERFLOW;
+        }
+        cst[(v0 + e) * N + cc] = (uint8_t)st0;
+      }
+    }
+  }
+  for (int64_t f = 0; f < n_fock; f++) fstart[f + 1] += fstart[f];
+  // every state inside its sector's basis; every term's Γ blocks shaped as
+  // its integrals (the kernels index both without further checks)
+  for (int64_t f = 0; f < n_fock; f++) {
+    for (int cc = 0; cc < N; cc++) {
+      const int na = var->fock[(f * N + cc) * 2], nb = var->fock[(f * N + cc) * 2 + 1];
+      if (na < 0 || nb < 0 || na >= S || nb >= S) {
+        ctx->err = "fcg_tpsci_pt2_sparse: Fock sector outside the clusters";
+        return FCG_EINVAL;
+      }
+      const int m = o->nstates[(int64_t)cc * S * S + na * S + nb];
+      for (int64_t j = fstart[f]; j < fstart[f + 1]; j++)
+        if (cst[j * N + cc] >= m || cst[j * N + cc] >= 255) {
+          ctx->err = "fcg_tpsci_pt2_sparse: state index outside its sector's basis";
+          return FCG_EINVAL;
+        }
+    }
+  }
+  {
+    const int64_t SS = (int64_t)S * S;
+    std::vector<uint8_t> seen((size_t)N * o->n_ops, 0);
+    for (int t = 0; t < h->n_terms; t++)
+      for (int q = 0; q < h->nbody[t]; q++) {
+        const int cl = h->clusters[4 * t + q], op = h->ops[4 * t + q];
+        if (cl < 0 || cl >= N || op < 0 || op >= o->n_ops) {
+          ctx->err = "fcg_tpsci_pt2_sparse: term cluster or operator out of range";
+          return FCG_EINVAL;
+        }
+        if (seen[(size_t)cl * o->n_ops + op]) continue;
+        seen[(size_t)cl * o->n_ops + op] = 1;
+        const int want = h->nbody[t] == 1 ? 1 : h->dims[4 * t + q];
+        for (int64_t e = 0; e < SS * SS; e++) {
+          const int blk = o->gblock[((int64_t)cl * o->n_ops + op) * SS * SS + e];
+          if (blk >= 0 && o->gb_dims[3 * blk] != want) {
+            ctx->err = "fcg_tpsci_pt2_sparse: a Γ block's leading dimension differs from its "
+                       "term's integrals";
+            return FCG_EINVAL;
+          }
+        }
+      }
+  }
+  // key widths: sector codes and states of every cluster must fit 64 bits each
+  int mstates = 1;
+  for (int v : o->nstates) mstates = std::max(mstates, v);
+  max_state = mstates;
+  const int fbits = bits_for((int64_t)S * S), sbits = bits_for(max_state);
+  if (fbits * N > 64 || sbits * N > 64) {
+    ctx->err = "fcg_tpsci_pt2_sparse: configuration keys need more than 128 bits";
+    return FCG_EOVERFLOW;
+  }
+  // screening thresholds per ket (the reference's thresh / max_r |c|)
+  std::vector<double> cabs(std::max<int64_t>(n_cfg, 1), 0.0), thr(std::max<int64_t>(n_cfg, 1));
+  std::vector<double> cmaxF(std::max<int64_t>(n_fock, 1), 0.0);
+  for (int64_t j = 0; j < n_cfg; j++) {
+    double m = 0.0;
+    for (int r = 0; r < R; r++) m = std::max(m, std::fabs(c[(int64_t)r * n_cfg + j]));
+    cabs[j] = m;
+    thr[j] = m > 0.0 ? thresh / m : INFINITY;
+  }
+  for (int64_t f = 0; f < n_fock; f++)
+    for (int64_t j = fstart[f]; j < fstart[f + 1]; j++) cmaxF[f] = std::max(cmaxF[f], cabs[j]);
+  // variational keys
+  std::vector<Key128> vk(std::max<int64_t>(n_cfg, 1));
+  for (int64_t f = 0; f < n_fock; f++) {
+    uint64_t hi = 0;
+    for (int cc = 0; cc < N; cc++)
+      hi |= (uint64_t)(var->fock[(f * N + cc) * 2] * S + var->fock[(f * N + cc) * 2 + 1])
+            << (fbits * cc);
+    for (int64_t j = fstart[f]; j < fstart[f + 1]; j++) {
+      uint64_t lo = 0;
+      for (int cc = 0; cc < N; cc++) lo |= (uint64_t)cst[j * N + cc] << (sbits * cc);
+      vk[j] = Key128{hi, lo};
+    }
+  }
+  std::sort(vk.begin(), vk.begin() + n_cfg, [](const Key128 &x, const Key128 &y) {
+    return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo);
+  });
+  // term table
+  std::vector<PTerm> terms(h->n_terms);
+  for (int g = 0; g < h->n_tc; g++)
+    for (int t = h->tc_start[g]; t < h->tc_start[g + 1]; t++) {
+      PTerm &T = terms[t];
+      T = PTerm{};
+      T.nbody = h->nbody[t];
+      T.group = g;
+      T.ioff = h->ioff[t];
+      for (int q = 0; q < 4; q++) {
+        T.cl[q] = h->clusters[4 * t + q];
+        T.op[q] = h->ops[4 * t + q];
+        T.dims[q] = h->dims[4 * t + q];
+        if (q < T.nbody && h->parity[4 * t + q] == 1) T.odd |= 1 << q;
+      }
+    }
+  // per-Γ-block maxima
+  const int64_t n_blocks = (int64_t)o->gb_off.size();
+  std::vector<int64_t> bmax_off(n_blocks + 1, 0), cmx_off(n_blocks + 1, 0);
+  for (int64_t b = 0; b < n_blocks; b++) {
+    bmax_off[b + 1] = bmax_off[b] + o->gb_dims[3 * b];
+    cmx_off[b + 1] = cmx_off[b] + (int64_t)o->gb_dims[3 * b] * o->gb_dims[3 * b + 2];
+  }
+  Buf<int64_t> d_fstart(ctx), d_bmax_off(ctx), d_cmx_off(ctx), d_item_count(ctx), d_item_ptr(ctx);
+  Buf<int16_t> d_fock(ctx), d_tc(ctx);
+  Buf<uint8_t> d_cst(ctx);
+  Buf<double> d_cabs(ctx), d_thr(ctx), d_cmaxF(ctx), d_c(ctx), d_bmax(ctx), d_cmx(ctx), d_e0(ctx);
+  Buf<PTerm> d_terms(ctx);
+  Buf<int32_t> d_tcs(ctx), d_gblock(ctx);
+  Buf<Key128> d_vkey(ctx);
+  Buf<unsigned long long> d_stats(ctx);
+  int rc = d_fstart.upload(fstart.data(), n_fock + 1);
+  if (!rc) rc = d_fock.upload(var->fock.data(), (int64_t)var->fock.size());
+  if (!rc) rc = d_cst.upload(cst.data(), n_cfg * N);
+  if (!rc) rc = d_cabs.upload(cabs.data(), n_cfg);
+  if (!rc) rc = d_thr.upload(thr.data(), n_cfg);
+  if (!rc) rc = d_cmaxF.upload(cmaxF.data(), n_fock);
+  if (!rc) rc = d_c.upload(c, (int64_t)R * n_cfg);
+  if (!rc) rc = d_e0.upload(e0, R);
+  if (!rc) rc = d_terms.upload(terms.data(), (int64_t)terms.size());
+  if (!rc) rc = d_tcs.upload(h->tc_start.data(), (int64_t)h->tc_start.size());
+  if (!rc) rc = d_tc.upload(h->tc.data(), (int64_t)h->tc.size());
+  if (!rc) rc = d_gblock.upload(o->gblock.data(), (int64_t)o->gblock.size());
+  if (!rc) rc = d_bmax_off.upload(bmax_off.data(), n_blocks + 1);
+  if (!rc) rc = d_cmx_off.upload(cmx_off.data(), n_blocks + 1);
+  if (!rc) rc = d_vkey.upload(vk.data(), n_cfg);
+  std::vector<int32_t> nstates(o->nstates.begin(), o->nstates.end());
+  Buf<int32_t> d_nstates(ctx);
+  if (!rc) rc = d_nstates.upload(nstates.data(), (int64_t)nstates.size());
+  if (rc) return rc;
+  if (!d_bmax.alloc(bmax_off[n_blocks]) || !d_cmx.alloc(cmx_off[n_blocks]) ||
+      !d_item_count.alloc(n_fock) || !d_stats.alloc(4)) {
+    ctx->err = "PT2 workspace allocation failed";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(unsigned long long), s), ctx);
+  if (n_blocks > 0)
+    hipLaunchKernelGGL(k_gamma_max, dim3((unsigned)n_blocks), dim3(256), 0, s, n_blocks,
+                       o->d_gb_off, o->d_gb_dims, o->d_gdata, d_bmax_off.p, d_bmax.p,
+                       d_cmx_off.p, d_cmx.p);
+  Pt2Args a{};
+  a.N = N;
+  a.S = S;
+  a.R = R;
+  a.nbody = nbody;
+  a.fbits = fbits;
+  a.sbits = sbits;
+  a.thresh = thresh;
+  a.n_fock = n_fock;
+  a.n_cfg = n_cfg;
+  a.fstart = d_fstart.p;
+  a.fock = d_fock.p;
+  a.cst = d_cst.p;
+  a.cabs = d_cabs.p;
+  a.thr = d_thr.p;
+  a.cmaxF = d_cmaxF.p;
+  a.c = d_c.p;
+  a.terms = d_terms.p;
+  a.tc_start = d_tcs.p;
+  a.tc = d_tc.p;
+  a.n_tc = h->n_tc;
+  a.ints = h->d_ints;
+  a.gblock = d_gblock.p;
+  a.n_ops = o->n_ops;
+  a.gb_off = o->d_gb_off;
+  a.gb_dims = o->d_gb_dims;
+  a.gdata = o->d_gdata;
+  a.nstates = d_nstates.p;
+  a.bmax_off = d_bmax_off.p;
+  a.bmax = d_bmax.p;
+  a.cmx_off = d_cmx_off.p;
+  a.cmx = d_cmx.p;
+  a.item_count = d_item_count.p;
+  a.stats = d_stats.p;
+  // items: count, prefix, fill
+  for (int64_t f0 = 0; f0 < n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    hipLaunchKernelGGL(k_pt2_items<false>, dim3((unsigned)std::min(FB_CHUNK, n_fock - f0)),
+                       dim3(PT), 0, s, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<int64_t> icnt(std::max<int64_t>(n_fock, 1));
+  FCG_HIP(hipMemcpyAsync(icnt.data(), d_item_count.p, sizeof(int64_t) * n_fock,
+                         hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  std::vector<int64_t> iptr(n_fock + 1, 0);
+  for (int64_t f = 0; f < n_fock; f++) iptr[f + 1] = iptr[f] + icnt[f];
+  const int64_t n_items = iptr[n_fock];
+  Buf<int32_t> d_item_f(ctx), d_item_t(ctx);
+  if ((rc = d_item_ptr.upload(iptr.data(), n_fock + 1))) return rc;
+  if (!d_item_f.alloc(n_items) || !d_item_t.alloc(n_items)) {
+    ctx->err = "PT2 item list allocation failed";
+    return FCG_ENOMEM;
+  }
+  a.item_ptr = d_item_ptr.p;
+  a.item_f = d_item_f.p;
+  a.item_t = d_item_t.p;
+  for (int64_t f0 = 0; f0 < n_fock; f0 += FB_CHUNK) {
+    a.fb_base = f0;
+    hipLaunchKernelGGL(k_pt2_items<true>, dim3((unsigned)std::min(FB_CHUNK, n_fock - f0)),
+                       dim3(PT), 0, s, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  // emission: count, prefix, fill
+  Buf<int64_t> d_ecount(ctx), d_eptr(ctx);
+  if (!d_ecount.alloc(n_items)) {
+    ctx->err = "PT2 emission counts allocation failed";
+    return FCG_ENOMEM;
+  }
+  a.items_f = d_item_f.p;
+  a.items_t = d_item_t.p;
+  a.n_items = n_items;
+  a.emit_count = d_ecount.p;
+  for (int64_t i0 = 0; i0 < n_items; i0 += ITEM_CHUNK) {
+    a.item_base = i0;
+    hipLaunchKernelGGL(k_pt2_emit<false>, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
+                       dim3(PT), 0, s, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<int64_t> ecnt(std::max<int64_t>(n_items, 1));
+  FCG_HIP(hipMemcpyAsync(ecnt.data(), d_ecount.p, sizeof(int64_t) * n_items,
+                         hipMemcpyDeviceToHost, s), ctx);
+  unsigned long long stc[4];
+  FCG_HIP(hipMemcpyAsync(stc, d_stats.p, sizeof(stc), hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  if (stc[2]) {
+    ctx->err = "fcg_tpsci_pt2_sparse: a term's shape exceeds the staged contraction";
+    return FCG_EOVERFLOW;
+  }
+  std::vector<int64_t> eptr(n_items + 1, 0);
+  for (int64_t i = 0; i < n_items; i++) eptr[i + 1] = eptr[i] + ecnt[i];
+  const int64_t E = eptr[n_items];
+  if ((rc = d_eptr.upload(eptr.data(), n_items + 1))) return rc;
+  Buf<Key128> d_ekey(ctx), d_skey(ctx);
+  Buf<double> d_eval(ctx);
+  Buf<uint64_t> d_idx(ctx), d_perm(ctx);
+  if (!d_ekey.alloc(E) || !d_eval.alloc(E * R) || !d_skey.alloc(E) || !d_idx.alloc(E) ||
+      !d_perm.alloc(E)) {
+    ctx->err = "PT2 emission buffers (" + std::to_string(E) + " contributions) do not fit";
+    return FCG_ENOMEM;
+  }
+  a.emit_ptr = d_eptr.p;
+  a.ekey = d_ekey.p;
+  a.eval = d_eval.p;
+  FCG_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(unsigned long long), s), ctx);
+  for (int64_t i0 = 0; i0 < n_items; i0 += ITEM_CHUNK) {
+    a.item_base = i0;
+    hipLaunchKernelGGL(k_pt2_emit<true>, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
+                       dim3(PT), 0, s, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  // stable sort of the contributions by x
+  hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, E, d_idx.p);
+  size_t tmp_bytes = 0;
+  FCG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
+                                    (size_t)E, KeyDecomposer{}, 0u, 128u, s),
+          ctx);
+  Buf<char> d_tmp(ctx);
+  if (!d_tmp.alloc((int64_t)tmp_bytes)) {
+    ctx->err = "PT2 sort workspace allocation failed";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
+                                    (size_t)E, KeyDecomposer{}, 0u, 128u, s),
+          ctx);
+  // segments of equal keys: heads, ids by scan, starts
+  Buf<uint32_t> d_head(ctx);
+  Buf<uint64_t> d_segid(ctx);
+  if (!d_head.alloc(E) || !d_segid.alloc(E)) {
+    ctx->err = "PT2 segment buffers allocation failed";
+    return FCG_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_heads, dim3(2048), dim3(256), 0, s, E, d_skey.p, d_head.p);
+  size_t scan_bytes = 0;
+  FCG_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
+                                  (size_t)E, rocprim::plus<uint64_t>(), s),
+          ctx);
+  if (!d_tmp.alloc((int64_t)std::max(scan_bytes, (size_t)1))) {
+    ctx->err = "PT2 scan workspace allocation failed";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(rocprim::exclusive_scan(d_tmp.p, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
+                                  (size_t)E, rocprim::plus<uint64_t>(), s),
+          ctx);
+  uint64_t last_id = 0;
+  uint32_t last_head = 0;
+  if (E > 0) {
+    FCG_HIP(hipMemcpyAsync(&last_id, d_segid.p + E - 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           s), ctx);
+    FCG_HIP(hipMemcpyAsync(&last_head, d_head.p + E - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s), ctx);
+  }
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  const int64_t n_seg = E > 0 ? (int64_t)last_id + last_head : 0;
+  Buf<int64_t> d_seg(ctx);
+  if (!d_seg.alloc(n_seg + 1)) {
+    ctx->err = "PT2 segment table allocation failed";
+    return FCG_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_seg_start, dim3(2048), dim3(256), 0, s, E, d_head.p, d_segid.p, d_seg.p);
+  FCG_HIP(hipMemcpyAsync(d_seg.p + n_seg, &E, sizeof(int64_t), hipMemcpyHostToDevice, s), ctx);
+  Buf<double> d_part(ctx);
+  Buf<unsigned long long> d_nout(ctx);
+  if (!d_part.alloc((int64_t)SUM_BLOCKS * R * 2) || !d_nout.alloc(1)) {
+    ctx->err = "PT2 sum buffers allocation failed";
+    return FCG_ENOMEM;
+  }
+  FCG_HIP(hipMemsetAsync(d_nout.p, 0, sizeof(unsigned long long), s), ctx);
+  SumArgs sa{};
+  sa.N = N;
+  sa.S = S;
+  sa.R = R;
+  sa.n_ops = o->n_ops;
+  sa.h0_op = h0_op;
+  sa.fbits = fbits;
+  sa.sbits = sbits;
+  sa.n_seg = n_seg;
+  sa.seg_start = d_seg.p;
+  sa.perm = d_perm.p;
+  sa.skey = d_skey.p;
+  sa.eval = d_eval.p;
+  sa.vkey = d_vkey.p;
+  sa.n_v = n_cfg;
+  sa.gblock = d_gblock.p;
+  sa.gb_off = o->d_gb_off;
+  sa.gb_dims = o->d_gb_dims;
+  sa.gdata = o->d_gdata;
+  sa.e0 = d_e0.p;
+  sa.part = d_part.p;
+  sa.n_out = d_nout.p;
+  hipLaunchKernelGGL(k_pt2_seg_sum, dim3(SUM_BLOCKS), dim3(256), 0, s, sa);
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<double> part((size_t)SUM_BLOCKS * R * 2);
+  unsigned long long nout = 0;
+  FCG_HIP(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * part.size(),
+                         hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipMemcpyAsync(&nout, d_nout.p, sizeof(nout), hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipMemcpyAsync(stc, d_stats.p, sizeof(stc), hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  for (int r = 0; r < R; r++) {
+    double sum = 0.0, comp = 0.0;
+    for (int blk = 0; blk < SUM_BLOCKS; blk++) {
+      const double v = part[((size_t)blk * R + r) * 2];
+      const double t = sum + v;
+      comp += std::fabs(sum) >= std::fabs(v) ? (sum - t) + v : (v - t) + sum;
+      sum = t;
+      comp += part[((size_t)blk * R + r) * 2 + 1];
+    }
+    e2[r] = sum + comp;
+  }
+  if (st) {
+    st->n_items = n_items;
+    st->n_groups = (int64_t)stc[0];
+    st->n_contributions = E;
+    st->n_configs = n_seg;
+    st->n_first_order = (int64_t)nout;
+    st->chain_flops = (double)stc[1];
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return FCG_OK;
+}

@@ -107,14 +107,17 @@ class Engine:
         return _lib.open_sigma(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
                                bra, ket, ket_state.coeffs, nbody=nbody, thresh=thresh)
 
-    def pt2(self, ci_vector, cluster_ops, clustered_ham, E0, nbody=4, H0="Hcmf", thresh=0.0):
-        """e2 of an orthonormal TPSCI state over its first-order space
-        (fcg_tpsci_pt2): (e2 [R], statistics dict)."""
+    def pt2(self, ci_vector, cluster_ops, clustered_ham, E0, nbody=4, H0="Hcmf", thresh=0.0,
+            method="dense"):
+        """e2 of an orthonormal TPSCI state over its first-order space:
+        method "dense" (fcg_tpsci_pt2, whole target Fock blocks) or "sparse"
+        (fcg_tpsci_pt2_sparse, the configurations screened contributions
+        reach).  Returns (e2 [R], statistics dict)."""
         from .cluster_ops import OP_NAMES
         space = _lib.DeviceSpace.tpsci(self.ctx, ci_vector)
-        return _lib.tpsci_pt2(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
-                              space, ci_vector.coeffs, E0, nbody=nbody,
-                              h0_op=OP_NAMES.index(H0), thresh=thresh)
+        fn = _lib.tpsci_pt2_sparse if method == "sparse" else _lib.tpsci_pt2
+        return fn(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops), space,
+                  ci_vector.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0), thresh=thresh)
 
     def drop_plans(self, keep=()):
         """Free the cached plans (their device blocks) except those whose
@@ -267,7 +270,7 @@ def compute_expectation_value(ci_vector, cluster_ops, clustered_ham, engine=None
 
 
 def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
-                       thresh_foi=1e-9, prescreen=True, verbose=1, engine=None):
+                       thresh_foi=1e-9, prescreen=True, verbose=1, method="auto", engine=None):
     """Second-order energy correction e2[r] of a TPSCI state over its
     first-order space (compute_pt2_energy, src/tpsci_pt2_energy.jl:10-175).
 
@@ -280,8 +283,14 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     configuration j enters only when its magnitude exceeds
     thresh_foi / max_r |c[j, r]|); `prescreen` is accepted for call
     compatibility: the reference's bounds only skip contributions that fail
-    that test.  Returns e2 (length R)."""
+    that test.  `method`: "dense" forms sigma on every configuration of each
+    target Fock block (few clusters), "sparse" only on the configurations
+    screened contributions reach, generated per (ket configuration, term)
+    as the reference does (many clusters); "auto" takes "sparse" above 4
+    clusters.  Both give the same e2.  Returns e2 (length R)."""
     eng = engine or default_engine()
+    if method == "auto":
+        method = "sparse" if ci_vector.N > 4 else "dense"
     ci = _copy_state(ci_vector)
     ci.set_vector(orthonormalize(ci.coeffs))
     if E0 is None:
@@ -289,11 +298,15 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
                                        engine=eng)
     E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
     e2, stats = eng.pt2(ci, cluster_ops, clustered_ham, E0, nbody=nbody, H0=H0,
-                        thresh=thresh_foi)
+                        thresh=thresh_foi, method=method)
     if verbose:
         evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
-        print(" first-order space: %d Fock blocks, %d configurations; %.3f s"
-              % (stats["n_target_fock"], stats["n_first_order"], stats["seconds"]))
+        if method == "sparse":
+            print(" first-order space: %d configurations reached by %d contributions; %.3f s"
+                  % (stats["n_first_order"], stats["n_contributions"], stats["seconds"]))
+        else:
+            print(" first-order space: %d Fock blocks, %d configurations; %.3f s"
+                  % (stats["n_target_fock"], stats["n_first_order"], stats["seconds"]))
         print(" %5s %12s %12s" % ("Root", "E(0)", "E(2)"))
         for r in range(len(e2)):
             print(" %5d %12.8f %12.8f" % (r + 1, evar[r], evar[r] + e2[r]))

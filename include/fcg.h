@@ -242,6 +242,32 @@ int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, 
                   const double *c, const double *e0, int nbody, int h0_op, double thresh,
                   double *e2, fcg_pt2_stats *stats);
 
+/* compute_pt2_energy over a sparse first-order space (the reference's own
+ * formulation, src/tpsci_pt2_energy.jl:10-282 with contract_matvec_thread
+ * and its bounds, src/tpsci_matvec_thread.jl:279-1283): contributions are
+ * generated per (ket configuration, term) as every bra state combination of
+ * the term's active clusters, pruned by upper bounds that never drop an
+ * element passing the test |element| > thresh / max_r |c[j, r]|, kept by that
+ * test, summed per first-order configuration x in a fixed order, and e2[r] =
+ * sum over x outside the variational space of sigma[x,r]^2 /
+ * (e0[r] - H0_diag[x]).  Arguments as fcg_tpsci_pt2; it never forms a target
+ * Fock block whole, so it serves many clusters.  FCG_EOVERFLOW when the
+ * configuration keys need more than 2 x 64 bits or a term's shape exceeds
+ * the staged contraction (then use fcg_tpsci_pt2). */
+typedef struct {
+  int64_t n_items;          /* (ket Fock block, term) pairs passing the first bound */
+  int64_t n_groups;         /* ket groups (equal active states) contracted */
+  int64_t n_contributions;  /* kept (x, ket, term) contributions */
+  int64_t n_configs;        /* distinct x reached */
+  int64_t n_first_order;    /* ... outside the variational space */
+  double chain_flops;       /* flops of the kept contractions */
+  double seconds;           /* wall time of the call */
+} fcg_pt2_sparse_stats;
+
+int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
+                         const double *c, const double *e0, int nbody, int h0_op, double thresh,
+                         double *e2, fcg_pt2_sparse_stats *stats);
+
 /* Open matvec of a TPSCI state onto the configurations of `bra` (replaces
  * open_matvec_thread, src/tpsci_matvec_thread.jl:11-129, for the target
  * configurations the caller lists; prescreen does not change the result).

@@ -66,3 +66,23 @@ def three_clusters_m4(seed=0, R=2, full_blocks=False):
     sector (the TPSCI benchmark problem), variational blocks within one
     transferred electron."""
     return make_case(12, 3, ((2, 2), (2, 2), (2, 2)), 4, seed, R=R, full_blocks=full_blocks)
+
+
+def six_clusters_rank1(seed=21, R=2, frac=0.6):
+    """24 orbitals in 6 clusters of 4 at half filling, lowest 4 states per
+    sector, a hash-selected rank <= 1 variational space over the Fock blocks
+    within one transferred electron (710 configurations), and an orthonormal
+    state with a dominant reference configuration and small seeded
+    coefficients elsewhere (the shape screening is made for)."""
+    from fermicg_amd.workloads import build_problem, rank1_space
+    p = build_problem(24, 6, 4, 48, seed=seed, max_transfer=1)
+    st = rank1_space(p, R=R, frac=frac, seed=seed)
+    rng = np.random.default_rng(seed + 5)
+    v = rng.normal(size=(len(st), R)) * 0.05
+    for r in range(R):
+        v[r, r] += 1.0
+    st.set_vector(orthonormalize(v))
+    c = Case()
+    c.ints, c.clusters, c.ref, c.bases = p.ints, p.clusters, p.ref_fock, p.bases
+    c.ops, c.H, c.state = p.cluster_ops, p.clustered_ham, st
+    return c

@@ -1,0 +1,72 @@
+"""GPU parity of the sparse first-order-space PT2 (fcg_tpsci_pt2_sparse).
+
+Oracle: orc_tpsci_pt2, the C restatement of compute_pt2_energy with the
+_collect_significant_thread! element test (pinned by the dense closed form
+and, through tpsci_ci, by the reference's own h12 values), on a 6-cluster
+case at several thresholds and on the 3- and 4-cluster cases; the dense GPU
+PT2 on the same inputs; the reference's h12 e0 + e2 (test/test_tpsci.jl:
+123-131) through compute_pt2_energy with the sparse method.  Bars: e2 within
+1e-11 relative of the oracle; repeated calls bit-identical.
+"""
+import numpy as np
+import pytest
+
+from fermicg_amd.marshal import flatten_ham, flatten_ops
+from fermicg_amd.tpsci import Engine, compute_pt2_energy
+from oracle.pyoracle import Oracle
+from tests.pt2_cases import four_clusters, six_clusters_rank1, three_clusters_m4
+from tests.test_pt2_oracle import _e0
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    return Engine(0)
+
+
+@pytest.fixture(scope="module")
+def six():
+    return six_clusters_rank1()
+
+
+def _oracle(case, e0, thresh, nbody=4):
+    st = case.state
+    fo = flatten_ops(case.ops, [len(c) for c in case.clusters])
+    e2, _ = Oracle(fo, flatten_ham(case.H, nbody), st).pt2(st.coeffs, e0, nbody=nbody,
+                                                           thresh=thresh)
+    return e2
+
+
+@pytest.mark.parametrize("thresh", [0.0, 1e-6, 1e-4, 1e-3])
+def test_sparse_pt2_six_clusters(eng, six, thresh):
+    e0 = _e0(six, six.state)
+    ref = _oracle(six, e0, thresh)
+    e2 = compute_pt2_energy(six.state, six.ops, six.H, E0=e0, thresh_foi=thresh, verbose=0,
+                            method="sparse", engine=eng)
+    st = compute_pt2_energy.last_stats
+    print(thresh, e2, ref, st, flush=True)
+    assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
+    again = compute_pt2_energy(six.state, six.ops, six.H, E0=e0, thresh_foi=thresh, verbose=0,
+                               method="sparse", engine=eng)
+    assert np.array_equal(again, e2)   # fixed-order accumulation
+
+
+def test_sparse_equals_dense_six_clusters(eng, six):
+    e0 = _e0(six, six.state)
+    kw = dict(E0=e0, thresh_foi=1e-5, verbose=0, engine=eng)
+    d = compute_pt2_energy(six.state, six.ops, six.H, method="dense", **kw)
+    s = compute_pt2_energy(six.state, six.ops, six.H, method="sparse", **kw)
+    assert np.abs(s - d).max() < 1e-11 * np.abs(d).max()
+
+
+@pytest.mark.parametrize("name,nbody,thresh", [("four", 4, 0.0), ("four", 2, 0.0),
+                                               ("three", 4, 0.0), ("three", 4, 1e-4)])
+def test_sparse_pt2_small_cases(eng, name, nbody, thresh):
+    case = {"four": four_clusters, "three": three_clusters_m4}[name]()
+    e0 = _e0(case, case.state)
+    ref = _oracle(case, e0, thresh, nbody)
+    e2 = compute_pt2_energy(case.state, case.ops, case.H, nbody=nbody, E0=e0, thresh_foi=thresh,
+                            verbose=0, method="sparse", engine=eng)
+    print(name, nbody, thresh, e2, ref, flush=True)
+    assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
