@@ -35,6 +35,7 @@ EXPORTS = [
     "fcg_open_sigma",
     "fcg_sigma_device_noalloc",
     "fcg_tpsci_pt2_sparse",
+    "fcg_plan_create_matrix_free",
 ]
 
 
@@ -118,6 +119,8 @@ def load():
     L.fcg_plan_budget.argtypes = [P(c_i64), P(c_i64)]
     L.fcg_apply_variant_counts.argtypes = [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
     L.fcg_set_build_class_cap.argtypes = [VP, ctypes.c_int]
+    L.fcg_plan_create_matrix_free.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int,
+                                              P(VP)]
     L.fcg_tpsci_pt2_sparse.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64),
                                        ctypes.c_int, ctypes.c_int, c_f64, P(c_f64),
                                        P(PT2SparseStats)]
@@ -279,6 +282,21 @@ class Plan(_Handle):
         h = VP()
         ctx.check(ctx.L.fcg_plan_create(ctx.h, ham.h, ops.h, space.h, nbody, mask,
                                         ctypes.byref(h)), "fcg_plan_create")
+        pl = cls(ctx, h, keep=(ham, ops, space))
+        pl.nvar = pl.nvar_ket = space.nvar()
+        return pl
+
+    @classmethod
+    def create_matrix_free(cls, ctx, ham, ops, space, nbody=4, precisions=("fp64",)):
+        """Matrix-free plan of a rank <= 1 TPSCI space (fcg_plan_create_matrix_free):
+        sigma formed per call, nothing stored; FP64 and FP32."""
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISION[p]
+        h = VP()
+        ctx.check(ctx.L.fcg_plan_create_matrix_free(ctx.h, ham.h, ops.h, space.h, nbody, mask,
+                                                    ctypes.byref(h)),
+                  "fcg_plan_create_matrix_free")
         pl = cls(ctx, h, keep=(ham, ops, space))
         pl.nvar = pl.nvar_ket = space.nvar()
         return pl

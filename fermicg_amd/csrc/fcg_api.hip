@@ -176,6 +176,8 @@ void fcg_destroy(fcg_ctx_t ctx) {
   dev_free(ctx, ctx->d_c);
   dev_free(ctx, ctx->d_s);
   if (ctx->d_ci) fcg_dfree(ctx, ctx->d_ci);
+  if (ctx->d_r1x) fcg_dfree(ctx, ctx->d_r1x);
+  if (ctx->d_r1y) fcg_dfree(ctx, ctx->d_r1y);
   (void)hipStreamSynchronize(ctx->stream);
   forget_user(ctx);
   for (auto &e : ctx->slots)
@@ -217,6 +219,7 @@ int fcg_ops_upload(fcg_ctx_t ctx, int n_clusters, int S, int n_ops, const int32_
       const int blk = o->block(c, 0 /* "H" */, code, code);
       if (blk >= 0) o->nstates[c * SS + code] = gb_dims[3 * blk + 1];
     }
+  o->n_gdata = n_gdata;
   int rc = dev_upload(ctx, &o->d_gdata, gdata, n_gdata);
   if (!rc) rc = dev_upload(ctx, &o->d_gb_off, gb_off, n_blocks);
   if (!rc) rc = dev_upload(ctx, &o->d_gb_dims, gb_dims, 3 * n_blocks);
@@ -643,6 +646,7 @@ void fcg_plan_destroy(fcg_plan_t pl) {
     for (fcg_ctx *u : pl->users) (void)hipStreamSynchronize(u->stream);
   }
   fcg_ctx *ctx = pl->ctx;
+  fcg_rank1_free(ctx, pl);
   dev_free(ctx, pl->d_row_ptr);
   dev_free(ctx, pl->d_pair_B);
   dev_free(ctx, pl->d_pair_K);
@@ -770,6 +774,10 @@ int fcg_sigma_device(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *c_dev, d
   }
   (void)hipSetDevice(ctx->device);
   fcg_plan_note_user(pl, ctx);
+  if (pl->r1) {  // matrix-free plan of a rank <= 1 space
+    ctx->last_launches = 1;
+    return fcg_rank1_sigma(ctx, pl, R, c_dev, s_dev, precision);
+  }
   const int64_t n = (int64_t)R * pl->ksp->nvar;
   if (ctx->ci_bytes < n * 8) {  // interleaved copy of c, sized for the FP64 form
     // stream-ordered on this context: the old copy's readers ran on it
@@ -803,6 +811,11 @@ int fcg_sigma_device_noalloc(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *
   }
   if (!pl->holds_blocks) {
     ctx->err = "plan holds no blocks";
+    return FCG_EINVAL;
+  }
+  if (pl->r1) {
+    ctx->err = "fcg_sigma_device_noalloc: matrix-free plans use layout buffers; use "
+               "fcg_sigma_device";
     return FCG_EINVAL;
   }
   const int64_t n = (int64_t)R * pl->ksp->nvar;
@@ -858,7 +871,10 @@ int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t pl, double *hd) {
   fcg_plan_note_user(pl, ctx);
   const int64_t n = pl->sp->nvar;
   TRY(ensure_scratch(ctx, n));
-  FCG_HIP(fcg_launch_diag(pl, ctx->d_s, ctx->stream), ctx);
+  if (pl->r1)
+    TRY(fcg_rank1_diag(ctx, pl, ctx->d_s));
+  else
+    FCG_HIP(fcg_launch_diag(pl, ctx->d_s, ctx->stream), ctx);
   FCG_HIP(hipMemcpyAsync(hd, ctx->d_s, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream),
           ctx);
   FCG_HIP(hipStreamSynchronize(ctx->stream), ctx);

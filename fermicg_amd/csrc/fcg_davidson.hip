@@ -289,8 +289,11 @@ extern "C" int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t pl, int R, const double *v
     FCG_HIP(hipMemcpyAsync(d.hd, o->precond_diag, sizeof(double) * n, hipMemcpyHostToDevice,
                            ctx->stream),
             ctx);
-  else
+  else if (pl->r1) {
+    if (int rc = fcg_rank1_diag(ctx, pl, d.hd)) return rc;
+  } else {
     FCG_HIP(fcg_launch_diag(pl, d.hd, ctx->stream), ctx);
+  }
   FCG_HIP(hipMemcpyAsync(d.T, v0, sizeof(double) * n * R, hipMemcpyHostToDevice, ctx->stream),
           ctx);
   int kept = 0;

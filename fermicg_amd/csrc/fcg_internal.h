@@ -72,6 +72,9 @@ struct fcg_ctx {
   // device address of the fixed scratch array of fcg_sigma_device_noalloc
   // on this context's device
   void *noalloc_scratch = nullptr;
+  // layout buffers of matrix-free (rank <= 1) sigma calls
+  double *d_r1x = nullptr, *d_r1y = nullptr;
+  int64_t r1_elems = 0;
   int build_class_cap = 3;  // largest LDS class plans on this context may use
 };
 
@@ -100,6 +103,7 @@ struct fcg_ops {
   std::vector<int32_t> gb_dims;  // [n_blocks][3]
   std::vector<int32_t> nstates;  // [N][S*S] from the "H" blocks
   double *d_gdata = nullptr;
+  int64_t n_gdata = 0;
   int64_t *d_gb_off = nullptr;
   int32_t *d_gb_dims = nullptr;
   int32_t block(int c, int op, int code_b, int code_k) const {
@@ -172,6 +176,11 @@ struct fcg_plan {
   uint32_t *d_Kbf = nullptr;      // hi bf16 << 16 | lo bf16
   uint16_t *d_Kb16 = nullptr;     // bf16 (the hi part alone)
   int64_t reserved_bytes = 0;     // held against the process-wide plan budget
+  // matrix-free plan of a rank <= 1 space (fcg_rank1.hip); nullptr for plans
+  // that store their Hamiltonian blocks
+  struct fcg_rank1 *r1 = nullptr;
+  const fcg_ham *r1_ham = nullptr;
+  const fcg_ops *r1_ops = nullptr;
   const double *d_thr = nullptr;  // build only: per-ket-variable screening threshold
   // apply variants chosen by timing on the first sigma of each precision and R
   // (tune_mu: one context times them while others wait)
@@ -256,6 +265,11 @@ int fcg_pt2_target(const fcg_ham *h, const fcg_ops *o, const fcg_space *var, int
 int fcg_fock_pairs(const fcg_ham *h, const fcg_space *bra, const fcg_space *ket, int nbody,
                    int S, std::vector<int64_t> *ptr, std::vector<int32_t> *fk,
                    std::vector<int32_t> *g);
+// connected Fock pairs (bra block, ket block bra - TransferConfig, group) of
+// a space with itself, as CSR over bra blocks on the device; the caller owns
+// the three arrays (fcg_dfree)
+int fcg_device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int nbody, int S,
+                          int64_t **ptr, int32_t **fk, int32_t **g, int64_t *n);
 // plan of a space whose boxes are all single configurations, joined and
 // built on the device (fcg_sparse.hip)
 int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, int nbody,
@@ -312,6 +326,13 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
                     int64_t *need = nullptr, const double *thr = nullptr,
                     const FusedSigma *fused = nullptr);
+
+// matrix-free sigma and diagonal of rank <= 1 plans (fcg_rank1.hip);
+// c, sigma: device, n_var x R column-major
+int fcg_rank1_sigma(fcg_ctx *ctx, fcg_plan *pl, int R, const double *c, double *sigma,
+                    int precision);
+int fcg_rank1_diag(fcg_ctx *ctx, fcg_plan *pl, double *hd);
+void fcg_rank1_free(fcg_ctx *ctx, fcg_plan *pl);
 
 // kernels (fcg_kernels.hip)
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,

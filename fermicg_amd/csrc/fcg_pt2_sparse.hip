@@ -46,6 +46,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <string>
 #include <vector>
@@ -814,6 +816,23 @@ struct Buf {
   }
 };
 
+// FCG_VERBOSE=1 in the environment: phase timings on stderr
+bool verbose() {
+  static const bool v = [] {
+    const char *e = std::getenv("FCG_VERBOSE");
+    return e && e[0] == '1';
+  }();
+  return v;
+}
+
+void vlog(const char *what, std::chrono::steady_clock::time_point t0, double x = -1.0) {
+  if (!verbose()) return;
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (x >= 0.0) std::fprintf(stderr, "[fcg pt2] %-28s %10.3f s  %.6g\n", what, s, x);
+  else std::fprintf(stderr, "[fcg pt2] %-28s %10.3f s\n", what, s);
+  std::fflush(stderr);
+}
+
 int bits_for(int64_t n) {  // bits to hold values 0 .. n-1
   int b = 0;
   while ((int64_t(1) << b) < n) b++;
@@ -1055,6 +1074,7 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   std::vector<int64_t> iptr(n_fock + 1, 0);
   for (int64_t f = 0; f < n_fock; f++) iptr[f + 1] = iptr[f] + icnt[f];
   const int64_t n_items = iptr[n_fock];
+  vlog("items counted", t0, (double)n_items);
   Buf<int32_t> d_item_f(ctx), d_item_t(ctx);
   if ((rc = d_item_ptr.upload(iptr.data(), n_fock + 1))) return rc;
   if (!d_item_f.alloc(n_items) || !d_item_t.alloc(n_items)) {
@@ -1084,6 +1104,10 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
     a.item_base = i0;
     hipLaunchKernelGGL(k_pt2_emit<false>, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
                        dim3(PT), 0, s, a);
+    if (verbose()) {
+      FCG_HIP(hipStreamSynchronize(s), ctx);
+      vlog("emission count chunk", t0, (double)std::min(i0 + ITEM_CHUNK, n_items));
+    }
   }
   FCG_HIP(hipGetLastError(), ctx);
   std::vector<int64_t> ecnt(std::max<int64_t>(n_items, 1));
@@ -1099,6 +1123,7 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   std::vector<int64_t> eptr(n_items + 1, 0);
   for (int64_t i = 0; i < n_items; i++) eptr[i + 1] = eptr[i] + ecnt[i];
   const int64_t E = eptr[n_items];
+  vlog("contributions counted", t0, (double)E);
   if ((rc = d_eptr.upload(eptr.data(), n_items + 1))) return rc;
   Buf<Key128> d_ekey(ctx), d_skey(ctx);
   Buf<double> d_eval(ctx);
@@ -1118,6 +1143,10 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
                        dim3(PT), 0, s, a);
   }
   FCG_HIP(hipGetLastError(), ctx);
+  if (verbose()) {
+    FCG_HIP(hipStreamSynchronize(s), ctx);
+    vlog("contributions written", t0);
+  }
   // stable sort of the contributions by x
   hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, E, d_idx.p);
   size_t tmp_bytes = 0;
@@ -1161,6 +1190,7 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   }
   FCG_HIP(hipStreamSynchronize(s), ctx);
   const int64_t n_seg = E > 0 ? (int64_t)last_id + last_head : 0;
+  vlog("sorted, configurations", t0, (double)n_seg);
   Buf<int64_t> d_seg(ctx);
   if (!d_seg.alloc(n_seg + 1)) {
     ctx->err = "PT2 segment table allocation failed";

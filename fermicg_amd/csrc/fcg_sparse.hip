@@ -644,6 +644,27 @@ int device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int n
 
 }  // namespace
 
+int fcg_device_fock_pairs(fcg_ctx *ctx, const fcg_ham *h, const fcg_space *sp, int nbody, int S,
+                          int64_t **ptr, int32_t **fk, int32_t **g, int64_t *n) {
+  *ptr = nullptr;
+  *fk = *g = nullptr;
+  *n = 0;
+  if (sp->N > 16 || S > 16) {
+    ctx->err = "device Fock pairs: at most 16 clusters and 15 electrons per cluster and spin";
+    return FCG_EOVERFLOW;
+  }
+  DBuf<int64_t> d_ptr;
+  DBuf<int32_t> d_fk, d_g;
+  if (int rc = device_fock_pairs(ctx, h, sp, nbody, S, d_ptr, d_fk, d_g, n)) return rc;
+  *ptr = d_ptr.p;
+  *fk = d_fk.p;
+  *g = d_g.p;
+  d_ptr.p = nullptr;  // ownership passes to the caller
+  d_fk.p = nullptr;
+  d_g.p = nullptr;
+  return FCG_OK;
+}
+
 bool fcg_sparse_plan_fits(const fcg_space *sp) {
   const int N = sp->N;
   int64_t run = 0;

@@ -63,14 +63,19 @@ class Engine:
             self._ham[key] = hit
         return hit[1]
 
-    def tpsci_plan(self, state, cluster_ops, clustered_ham, nbody=4, precisions=("fp64",)):
+    def tpsci_plan(self, state, cluster_ops, clustered_ham, nbody=4, precisions=("fp64",),
+                   matrix_free=False):
+        """The space's plan: stored Hamiltonian blocks, or with matrix_free a
+        plan that forms sigma per call (rank <= 1 spaces only)."""
         sig = (state.fock.tobytes(), state.fock_start.tobytes(), state.cfg.tobytes())
-        key = (id(cluster_ops), id(clustered_ham), nbody, hash(sig), tuple(sorted(precisions)))
+        key = (id(cluster_ops), id(clustered_ham), nbody, hash(sig), tuple(sorted(precisions)),
+               bool(matrix_free))
         pl = self._plans.get(key)
         if pl is None:
             sp = _lib.DeviceSpace.tpsci(self.ctx, state)
-            pl = _lib.Plan.create(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
-                                  sp, nbody=nbody, precisions=precisions)
+            make = _lib.Plan.create_matrix_free if matrix_free else _lib.Plan.create
+            pl = make(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops), sp,
+                      nbody=nbody, precisions=precisions)
             self._plans[key] = pl
         return pl
 

@@ -154,3 +154,39 @@ def tpsci_sparse(seed=2, R=4, n_clusters=8, max_roots=16, max_transfer=3, frac=0
     p.meta = {"workload": "tpsci_%do_%dc_M%d_rank1" % (4 * n_clusters, n_clusters, max_roots),
               "n_fock": len(p.focks), "n_cfg": len(st)}
     return p, st
+
+
+def pt2_12c_M32(seed=4, R=1, M=32):
+    """48 orbitals / 12 clusters of 4 at half filling / M states per sector;
+    variational space = every rank <= 1 configuration of the 26,533 Fock
+    configurations within 2 transferred electrons (9,020,329 at M=32).  The
+    state is built on the GPU by cmf_first_order_state."""
+    p = build_problem(48, 12, M, 96, seed, max_transfer=2)
+    st = rank1_space(p, R=R, frac=1.0, seed=seed)
+    p.meta = {"workload": "pt2_48o_12c_M%d_rank1" % M, "n_fock": len(p.focks), "n_cfg": len(st)}
+    return p, st
+
+
+def cmf_first_order_state(p, st, engine):
+    """The CMF reference (the ground configuration of the reference Fock
+    block) plus its first-order correction projected on the space, each
+    configuration J weighted (H|ref>)_J / (<ref|H0|ref> - H0_JJ) with H0 the
+    clusters' Hcmf -- formed with the matrix-free sigma of the rank <= 1
+    space (the Hamiltonian applied to the reference alone) and its H0
+    diagonal; normalised.  Returns the coefficient vector (n x 1)."""
+    from .tpsci import cluster_sizes_of  # noqa: F401
+    ref_f = p.focks.index(p.ref_fock)
+    ref_row = int(st.fock_start[ref_f])
+    assert all(int(x) == 1 for x in st.cfg[ref_row])
+    x = np.zeros((len(st), 1))
+    x[ref_row, 0] = 1.0
+    pl = engine.tpsci_plan(st, p.cluster_ops, p.clustered_ham, matrix_free=True)
+    sig = pl.sigma(x)[:, 0]
+    h0 = engine.local_operator(p.clustered_ham, "Hcmf")
+    hd0 = engine.tpsci_plan(st, p.cluster_ops, h0, matrix_free=True).diagonal()
+    den = hd0[ref_row] - hd0
+    den = np.where(np.abs(den) < 1e-3, np.copysign(1e-3, den + 1e-300), den)
+    c = sig / den
+    c[ref_row] = 1.0
+    c /= np.linalg.norm(c)
+    return c[:, None]

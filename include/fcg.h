@@ -130,6 +130,19 @@ int fcg_plan_create_pair(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_
 int fcg_plan_create_limited(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp,
                             int nbody, int precisions, int64_t max_bytes, int64_t *need_bytes,
                             fcg_plan_t *out);
+/* Matrix-free plan of a TPSCI space whose configurations are all of rank
+ * <= 1 (every cluster in the lowest state of its sector except at most
+ * one): sigma = H c is formed per call from the Γ tensors and integrals,
+ * factorised over the term's active clusters, with nothing stored (the
+ * reference's tps_ci_matvec, src/tpsci_outer.jl:462-527, restricted to such
+ * spaces; for spaces whose stored Hamiltonian does not fit in HBM).
+ * precisions: (1<<FCG_FP64) | optionally (1<<FCG_FP32) (FP32 arithmetic on
+ * FP32 copies of Γ and the integrals).  fcg_sigma*, fcg_diagonal and
+ * fcg_davidson accept the plan.  FCG_EINVAL if a configuration has two
+ * excited clusters; FCG_EOVERFLOW beyond 512 rank <= 1 configurations in a
+ * Fock block.  No counterpart in the reference beyond tps_ci_matvec. */
+int fcg_plan_create_matrix_free(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp,
+                                int nbody, int precisions, fcg_plan_t *out);
 /* Device bytes held by a plan (blocks, column indices, row tables). */
 int fcg_plan_device_bytes(fcg_plan_t plan, int64_t *bytes);
 /* Process-wide budget for the device bytes of all live plans (default
