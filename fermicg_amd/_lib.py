@@ -60,7 +60,8 @@ class PT2Stats(ctypes.Structure):
 class PT2SparseStats(ctypes.Structure):
     _fields_ = [("n_items", c_i64), ("n_groups", c_i64), ("n_contributions", c_i64),
                 ("n_configs", c_i64), ("n_first_order", c_i64), ("chain_flops", c_f64),
-                ("seconds", c_f64)]
+                ("seconds", c_f64), ("n_items_kept", c_i64), ("n_kets_kept", c_i64),
+                ("n_chunks", c_i64), ("contraction_seconds", c_f64), ("sort_seconds", c_f64)]
 
 
 class FCGError(RuntimeError):

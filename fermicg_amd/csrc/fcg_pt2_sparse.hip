@@ -23,21 +23,36 @@
 //           cluster sector in ClusterOps (the job filter, :49-79) and whose
 //           Γ blocks exist (missing blocks contribute nothing, as in
 //           contract_matvec_thread); kept when
-//           sum_p |V[p]| prod_a max|Γ_a[p_a, :, :]| * max_{J in F} |c_J| > thresh;
-//   groups  inside an item, the kets with equal states on the active
-//           clusters share every element; a group is kept when
-//           sum_p |V[p]| prod_a max_I |Γ_a[p_a, I, J_a]| * max_J |c_J| > thresh;
-//   indices bra states I_a whose per-index bound cannot pass are dropped
-//           before the contraction (upper_bound2_thread's survivor lists).
+//           B_t = sum_p |V[p]| prod_a max|Γ_a[p_a, :, :]|  satisfies
+//           B_t * max_{J in F} |c_J| > thresh;
+//   kets    each Fock block's kets are ordered by max_r |c| descending, so
+//           the kets with B_t |c_J| > thresh are a prefix; each of them is
+//           kept when its own bound
+//           B_t(J) = sum_p |V[p]| prod_a max_I |Γ_a[p_a, I, J_a]|
+//           satisfies B_t(J) |c_J| > thresh.  A light screening kernel
+//           counts these per item at high occupancy; only items with a kept
+//           ket reach the contraction kernel;
+//   indices kept kets with equal states on the active clusters share every
+//           element (a group); bra states I_a whose per-index bound cannot
+//           pass for the group's largest |c| are dropped before the
+//           contraction (upper_bound2_thread's survivor lists).
 // The kept contraction is the reference's chain (contract the last active
 // cluster first), tiled over the last cluster's surviving states so the
-// intermediates stay in LDS.
+// intermediates stay in LDS.  The coarse stages (items, screening) keep a
+// wider rounding margin than the per-ket and per-index tests, so a stage
+// never drops what a later one would keep.
 //
-// Accumulation is deterministic: a count pass and a fill pass over the same
-// work write every kept contribution (key = target Fock sectors | states of
-// x, value = H_t[x,J] c[J,:]) at positions fixed by prefix sums; a stable
-// radix sort on the 128-bit keys keeps equal keys in that order, and each
-// x's contributions are summed in it.  So e2 does not depend on scheduling.
+// Accumulation is deterministic and streams in memory: the items with a kept
+// ket are ordered by their target Fock block (a stable sort), a count pass
+// gives every item's number of kept contributions, and the items are cut
+// into chunks at target-block boundaries, each holding at most a fixed
+// number of contributions (the cut depends on the counts only).  Per chunk a
+// fill pass writes every kept contribution (key = target Fock sectors |
+// states of x, value = H_t[x,J] c[J,:]) at positions fixed by prefix sums, a
+// stable radix sort on the 128-bit keys keeps equal keys in that order, and
+// each x's contributions are summed in it.  Every x lies in one target block,
+// so a chunk holds all of its contributions, and e2 does not depend on
+// scheduling or on the memory free at the time of the call.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -49,6 +64,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -56,12 +72,15 @@
 
 namespace {
 
-constexpr int PT = 256;          // threads per workgroup
-constexpr int KCH = 256;         // kets per chunk of a ket Fock block
+constexpr int PT = 256;          // threads per contraction workgroup
+constexpr int SW = 64;           // threads per screening workgroup (one wavefront)
+constexpr int KCH = PT;          // candidate kets per batch of the contraction kernel
 constexpr int ARENA = 8192;      // chain intermediates (doubles)
 constexpr int VMAX = 1024;       // staged integrals (doubles)
 constexpr int SMAX = 64;         // bra states per cluster the survivor lists hold
-constexpr double BOUND_SLACK = 1.0 + 1e-10;  // bounds are inflated against rounding
+constexpr int NPMAX = 64;        // leading Γ dimension the bound weights hold
+constexpr double BOUND_SLACK = 1.0 + 1e-10;   // bounds are inflated against rounding
+constexpr double SCREEN_SLACK = 1.0 + 1e-8;   // ... and the coarse stages by more
 
 struct PTerm {
   int32_t nbody;
@@ -91,8 +110,9 @@ struct Pt2Args {
   const int64_t *fstart;  // [n_fock+1]
   const int16_t *fock;    // [n_fock][N][2]
   const uint8_t *cst;     // [n_cfg][N] 0-based states
-  const double *cabs;     // [n_cfg] max_r |c|
-  const double *thr;      // [n_cfg] thresh / cabs (the reference's thresh_curr)
+  const int32_t *kord;    // [n_cfg] each Fock block's kets by max_r |c| descending
+  const double *csort;    // [n_cfg] their max_r |c|
+  const double *thr;      // [n_cfg] thresh / max_r |c| (the reference's thresh_curr), by ket
   const double *cmaxF;    // [n_fock]
   const double *c;        // [R][n_cfg]
   // operator
@@ -115,14 +135,21 @@ struct Pt2Args {
   int64_t *item_count;      // count pass: items per ket Fock block
   const int64_t *item_ptr;  // fill pass
   int32_t *item_f, *item_t;
+  double *item_b;           // fill pass: the item bound B_t (without |c|)
+  uint64_t *item_key;       // fill pass: target Fock sector codes
   int64_t fb_base;
-  // emission
+  // screening and contraction
   const int32_t *items_f, *items_t;
+  const double *items_b;
   int64_t item_base, n_items;
-  int64_t *emit_count;      // count pass: kept contributions per item
-  const int64_t *emit_ptr;  // fill pass: first position of each item
+  int32_t *nsurv;             // screening: kept kets per item
+  const uint32_t *slot_item;  // contraction slot -> item (items ordered by target)
+  int64_t slot_base, n_slots;
+  int64_t *emit_count;        // count pass: kept contributions per slot
+  const int64_t *emit_ptr;    // fill pass: first position of each slot
+  int64_t emit_base;          // ... less this (the chunk's first position)
   Key128 *ekey;
-  double *eval;             // [E][R]
+  double *eval;               // [E][R]
   unsigned long long *stats;  // [0] groups kept, [1] chain flops, [2] overflow
 };
 
@@ -131,6 +158,33 @@ __device__ __forceinline__ int sector_code(int na, int nb, int S) { return na * 
 __device__ __forceinline__ int32_t gblk(const Pt2Args &a, int c, int op, int cb, int ck) {
   const int SS = a.S * a.S;
   return a.gblock[(((int64_t)c * a.n_ops + op) * SS + cb) * SS + ck];
+}
+
+// item bound B_t = sum_p |V[p]| prod_q bmax_q[p_q] for a term whose Γ blocks exist
+__device__ double item_bound(const Pt2Args &a, const PTerm &T, const int32_t *blk) {
+  if (T.nbody == 1) return a.bmax[a.bmax_off[blk[0]]];
+  const double *m0 = a.bmax + a.bmax_off[blk[0]];
+  const double *m1 = a.bmax + a.bmax_off[blk[1]];
+  const double *m2 = T.nbody > 2 ? a.bmax + a.bmax_off[blk[2]] : nullptr;
+  const double *m3 = T.nbody > 3 ? a.bmax + a.bmax_off[blk[3]] : nullptr;
+  const int d0 = T.dims[0], d1 = T.dims[1];
+  const int d2 = T.nbody > 2 ? T.dims[2] : 1, d3 = T.nbody > 3 ? T.dims[3] : 1;
+  const double *V = a.ints + T.ioff;
+  double B = 0.0;
+  for (int p3 = 0; p3 < d3; p3++)
+    for (int p2 = 0; p2 < d2; p2++) {
+      const double f23 = (m2 ? m2[p2] : 1.0) * (m3 ? m3[p3] : 1.0);
+      if (f23 == 0.0) continue;
+      for (int p1 = 0; p1 < d1; p1++) {
+        const double f = f23 * m1[p1];
+        if (f == 0.0) continue;
+        const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
+        double s = 0.0;
+        for (int p0 = 0; p0 < d0; p0++) s += fabs(Vp[p0]) * m0[p0];
+        B += s * f;
+      }
+    }
+  return B;
 }
 
 // ---- items: (ket Fock block, term) pairs that can contribute ----
@@ -151,8 +205,10 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
     const int g = g0 + tid;
     int cnt = 0;
     int32_t tlist[8];
+    double blist[8];
     int nl = 0;
     bool more = false;
+    uint64_t tkey = 0;
     if (g < a.n_tc && cm > 0.0) {
       const int16_t *d = a.tc + (int64_t)g * 2 * N;
       bool ok = true;
@@ -160,6 +216,7 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
         const int na = sF[2 * c] + d[2 * c], nb = sF[2 * c + 1] + d[2 * c + 1];
         ok = na >= 0 && nb >= 0 && na < S && nb < S &&
              a.nstates[(int64_t)c * S * S + sector_code(na, nb, S)] > 0;
+        if (ok) tkey |= (uint64_t)sector_code(na, nb, S) << (a.fbits * c);
       }
       if (ok) {
         for (int t = a.tc_start[g]; t < a.tc_start[g + 1]; t++) {
@@ -175,35 +232,15 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
             have = have && blk[q] >= 0;
           }
           if (!have) continue;
-          double B = 0.0;
-          if (T.nbody == 1) {
-            B = a.bmax[a.bmax_off[blk[0]]];
-          } else {
-            const double *m0 = a.bmax + a.bmax_off[blk[0]];
-            const double *m1 = a.bmax + a.bmax_off[blk[1]];
-            const double *m2 = T.nbody > 2 ? a.bmax + a.bmax_off[blk[2]] : nullptr;
-            const double *m3 = T.nbody > 3 ? a.bmax + a.bmax_off[blk[3]] : nullptr;
-            const int d0 = T.dims[0], d1 = T.dims[1];
-            const int d2 = T.nbody > 2 ? T.dims[2] : 1, d3 = T.nbody > 3 ? T.dims[3] : 1;
-            const double *V = a.ints + T.ioff;
-            for (int p3 = 0; p3 < d3; p3++)
-              for (int p2 = 0; p2 < d2; p2++) {
-                const double f23 = (m2 ? m2[p2] : 1.0) * (m3 ? m3[p3] : 1.0);
-                if (f23 == 0.0) continue;
-                for (int p1 = 0; p1 < d1; p1++) {
-                  const double f = f23 * m1[p1];
-                  if (f == 0.0) continue;
-                  const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
-                  double s = 0.0;
-                  for (int p0 = 0; p0 < d0; p0++) s += fabs(Vp[p0]) * m0[p0];
-                  B += s * f;
-                }
-              }
-          }
-          if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) continue;
+          const double B = item_bound(a, T, blk);
+          if (!(B * SCREEN_SLACK * cm > a.thresh) || B == 0.0) continue;
           if (FILL) {
-            if (nl < 8) tlist[nl++] = t;
-            else more = true;
+            if (nl < 8) {
+              tlist[nl] = t;
+              blist[nl++] = B;
+            } else {
+              more = true;
+            }
           }
           cnt++;
         }
@@ -227,6 +264,8 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
           for (int i = 0; i < nl; i++) {
             a.item_f[at + i] = (int32_t)fj;
             a.item_t[at + i] = tlist[i];
+            a.item_b[at + i] = blist[i];
+            a.item_key[at + i] = tkey;
           }
         } else {
           // more kept terms than the register list holds: recompute in order
@@ -244,34 +283,12 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
               have = have && blk[q] >= 0;
             }
             if (!have) continue;
-            double B = 0.0;
-            if (T.nbody == 1) {
-              B = a.bmax[a.bmax_off[blk[0]]];
-            } else {
-              const double *m0 = a.bmax + a.bmax_off[blk[0]];
-              const double *m1 = a.bmax + a.bmax_off[blk[1]];
-              const double *m2 = T.nbody > 2 ? a.bmax + a.bmax_off[blk[2]] : nullptr;
-              const double *m3 = T.nbody > 3 ? a.bmax + a.bmax_off[blk[3]] : nullptr;
-              const int d0 = T.dims[0], d1 = T.dims[1];
-              const int d2 = T.nbody > 2 ? T.dims[2] : 1, d3 = T.nbody > 3 ? T.dims[3] : 1;
-              const double *V = a.ints + T.ioff;
-              for (int p3 = 0; p3 < d3; p3++)
-                for (int p2 = 0; p2 < d2; p2++) {
-                  const double f23 = (m2 ? m2[p2] : 1.0) * (m3 ? m3[p3] : 1.0);
-                  if (f23 == 0.0) continue;
-                  for (int p1 = 0; p1 < d1; p1++) {
-                    const double f = f23 * m1[p1];
-                    if (f == 0.0) continue;
-                    const double *Vp = V + d0 * (p1 + d1 * (p2 + d2 * p3));
-                    double s = 0.0;
-                    for (int p0 = 0; p0 < d0; p0++) s += fabs(Vp[p0]) * m0[p0];
-                    B += s * f;
-                  }
-                }
-            }
-            if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) continue;
+            const double B = item_bound(a, T, blk);
+            if (!(B * SCREEN_SLACK * cm > a.thresh) || B == 0.0) continue;
             a.item_f[at] = (int32_t)fj;
             a.item_t[at] = t;
+            a.item_b[at] = B;
+            a.item_key[at] = tkey;
             at++;
           }
         }
@@ -292,6 +309,97 @@ __global__ __launch_bounds__(PT) void k_pt2_items(Pt2Args a) {
   }
 }
 
+// What the per-ket bound of one item needs: the term's integrals and, per
+// active cluster, the column maxima of its Γ block.
+struct KetBound {
+  int nb, cl[4], np[4], d0, d1, d2, d3;
+  const double *V;
+  const double *cmx[4];  // cmx of the Γ block; column J_q at np[q] * J_q
+};
+
+__device__ __forceinline__ void ket_bound_setup(const Pt2Args &a, const PTerm &T, int fj,
+                                                KetBound &k) {
+  const int N = a.N, S = a.S;
+  const int16_t *F = a.fock + (int64_t)fj * 2 * N;
+  const int16_t *D = a.tc + (int64_t)T.group * 2 * N;
+  k.nb = T.nbody;
+  for (int q = 0; q < 4; q++) {
+    k.cl[q] = 0;
+    k.np[q] = 1;
+    k.cmx[q] = nullptr;
+  }
+  for (int q = 0; q < T.nbody; q++) {
+    const int c = T.cl[q];
+    const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
+    const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
+    const int blk = gblk(a, c, T.op[q], cb, ck);
+    k.cl[q] = c;
+    k.np[q] = a.gb_dims[3 * blk];
+    k.cmx[q] = a.cmx + a.cmx_off[blk];
+  }
+  k.d0 = T.dims[0];
+  k.d1 = T.nbody > 1 ? T.dims[1] : 1;
+  k.d2 = T.nbody > 2 ? T.dims[2] : 1;
+  k.d3 = T.nbody > 3 ? T.dims[3] : 1;
+  k.V = a.ints + T.ioff;
+}
+
+// B_t(J) = sum_p |V[p]| prod_q max_I |Γ_q[p_q, I, J_q]| (for one body: max_I |H[I, J]|)
+__device__ __forceinline__ double ket_bound(const KetBound &k, const uint8_t *st) {
+  const double *c0 = k.cmx[0] + (int64_t)k.np[0] * st[k.cl[0]];
+  if (k.nb == 1) return c0[0];
+  const double *c1 = k.cmx[1] + (int64_t)k.np[1] * st[k.cl[1]];
+  const double *c2 = k.nb > 2 ? k.cmx[2] + (int64_t)k.np[2] * st[k.cl[2]] : nullptr;
+  const double *c3 = k.nb > 3 ? k.cmx[3] + (int64_t)k.np[3] * st[k.cl[3]] : nullptr;
+  double B = 0.0;
+  for (int p3 = 0; p3 < k.d3; p3++) {
+    const double f3 = c3 ? c3[p3] : 1.0;
+    if (f3 == 0.0) continue;
+    for (int p2 = 0; p2 < k.d2; p2++) {
+      const double f2 = f3 * (c2 ? c2[p2] : 1.0);
+      if (f2 == 0.0) continue;
+      for (int p1 = 0; p1 < k.d1; p1++) {
+        const double f1 = f2 * c1[p1];
+        if (f1 == 0.0) continue;
+        const double *Vp = k.V + k.d0 * (p1 + k.d1 * (p2 + k.d2 * p3));
+        double s = 0.0;
+        for (int p0 = 0; p0 < k.d0; p0++) s += fabs(Vp[p0]) * c0[p0];
+        B += s * f1;
+      }
+    }
+  }
+  return B;
+}
+
+// ---- screening: kets per item that pass their own bound (one wavefront per item) ----
+__global__ __launch_bounds__(SW) void k_pt2_screen(Pt2Args a) {
+  const int64_t it = a.item_base + blockIdx.x;
+  if (it >= a.n_items) return;  // uniform
+  const int lane = threadIdx.x;
+  const int fj = a.items_f[it];
+  const PTerm T = a.terms[a.items_t[it]];
+  KetBound kb;
+  ket_bound_setup(a, T, fj, kb);
+  const double Bi = a.items_b[it] * SCREEN_SLACK;
+  const int64_t k0 = a.fstart[fj], k1 = a.fstart[fj + 1];
+  int count = 0;
+  for (int64_t base = k0; base < k1; base += SW) {
+    const int64_t k = base + lane;
+    bool cand = false, pass = false;
+    if (k < k1) {
+      const double cj = a.csort[k];
+      cand = cj * Bi > a.thresh;
+      if (cand) {
+        const double B = ket_bound(kb, a.cst + (int64_t)a.kord[k] * a.N);
+        pass = B != 0.0 && B * SCREEN_SLACK * cj > a.thresh;
+      }
+    }
+    count += __popcll(__ballot(pass));
+    if (!__all(cand)) break;  // kets are ordered by |c|: no later candidate
+  }
+  if (lane == 0) a.nsurv[it] = count;
+}
+
 // block-wide exclusive scan of one int per thread; returns the total
 __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sScan, int64_t *out_excl) {
   const int tid = threadIdx.x;
@@ -309,25 +417,28 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sScan, int64_t
   return tot;
 }
 
-// ---- emission: every kept contribution of an item ----
-// One workgroup per item.  COUNT: emit_count[item] = number of kept
-// contributions; FILL: writes them from emit_ptr[item] on, in the order
-// (ket chunk, group, tile, output element, ket), the same in both passes.
+// ---- contraction and emission: every kept contribution of an item ----
+// One workgroup per slot (an item with a kept ket).  COUNT:
+// emit_count[slot] = number of kept contributions; FILL: writes them from
+// emit_ptr[slot] - emit_base on, in the order (candidate batch, group in
+// active-state order, tile, output element, ket in |c| order), the same in
+// both passes.
 template <bool FILL>
 __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ double sV[VMAX];
   __shared__ double sA[ARENA];
-  __shared__ double sW[4][64];        // per-cluster bound weights w_q[p]
-  __shared__ double sCM[4][64];       // column maxima of the group's ket states
+  __shared__ double sW[4][NPMAX];     // per-cluster bound weights w_q[p]
+  __shared__ double sCM[4][NPMAX];    // column maxima of the group's ket states
   __shared__ int16_t sSurv[4][SMAX];  // surviving bra states per active cluster
+  __shared__ uint8_t sFlag[4][SMAX];
   __shared__ int sNS[4];
-  __shared__ int32_t sKet[KCH];       // kets of the chunk, sorted by active-state key
-  __shared__ uint32_t sKey[KCH];
+  __shared__ int32_t sBKet[KCH];      // kept kets of the batch, in candidate order
+  __shared__ double sBCab[KCH];       // their max_r |c|
+  __shared__ uint32_t sKey[KCH];      // active-state keys, sorted with sPos
+  __shared__ int16_t sPos[KCH];
   __shared__ int sGrp[KCH + 1];
-  __shared__ int sNG;
-  __shared__ double sRed[PT];
+  __shared__ int sNG, sDone;
   __shared__ int64_t sScan[PT];
-  __shared__ int sCodeX[FCG_MAXN];
   __shared__ uint64_t sFockKey;
   __shared__ int sBlk[4], sNp[4], sMb[4], sMk[4];
   __shared__ double sSign;
@@ -335,8 +446,9 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ int sSkip;
 
   const int tid = threadIdx.x;
-  const int64_t it = a.item_base + blockIdx.x;
-  if (it >= a.n_items) return;  // uniform
+  const int64_t slot = a.slot_base + blockIdx.x;
+  if (slot >= a.n_slots) return;  // uniform
+  const int64_t it = a.slot_item[slot];
   const int fj = a.items_f[it];
   const PTerm T = a.terms[a.items_t[it]];
   const int N = a.N, S = a.S, nb = T.nbody;
@@ -346,7 +458,6 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
     uint64_t fk = 0;
     for (int c = 0; c < N; c++) {
       const int cx = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
-      sCodeX[c] = cx;
       fk |= (uint64_t)cx << (a.fbits * c);
     }
     sFockKey = fk;
@@ -367,18 +478,18 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       }
     }
     sSign = par ? -1.0 : 1.0;
-    sEmit = FILL ? a.emit_ptr[it] : 0;
+    sEmit = FILL ? a.emit_ptr[slot] - a.emit_base : 0;
     // shapes the staged chain cannot hold are reported, not computed
     int nv = 1;
     for (int q = 0; q < nb; q++) nv *= (nb == 1 ? 1 : T.dims[q]);
     int skip = nv > VMAX;
-    for (int q = 0; q < nb; q++) skip |= sMb[q] > SMAX || sNp[q] > 64 || sMk[q] > 255;
+    for (int q = 0; q < nb; q++) skip |= sMb[q] > SMAX || sNp[q] > NPMAX || sMk[q] > 255;
     sSkip = skip;
-    if (skip) atomicAdd(&a.stats[2], 1ull);
+    if (skip && !FILL) atomicAdd(&a.stats[2], 1ull);
   }
   __syncthreads();
   if (sSkip) {
-    if (!FILL && tid == 0) a.emit_count[it] = 0;
+    if (!FILL && tid == 0) a.emit_count[slot] = 0;
     return;
   }
   const double sign = sSign;
@@ -386,139 +497,156 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   if (nb > 1)
     for (int q = 0; q < nb; q++) nv *= T.dims[q];
   for (int e = tid; e < nv; e += PT) sV[e] = nb > 1 ? a.ints[T.ioff + e] : 1.0;
+  KetBound kb;
+  ket_bound_setup(a, T, fj, kb);
+  const double Bi = a.items_b[it] * SCREEN_SLACK;
   const int64_t k0 = a.fstart[fj], k1 = a.fstart[fj + 1];
   unsigned long long groups_kept = 0, flops = 0;
-  for (int64_t c0 = k0; c0 < k1; c0 += KCH) {
-    const int nk = (int)min((int64_t)KCH, k1 - c0);
+  for (int64_t cb0 = k0; cb0 < k1; cb0 += KCH) {
     __syncthreads();
-    // group the chunk's kets by their states on the active clusters
-    for (int j = tid; j < nk; j += PT) {
-      const uint8_t *st = a.cst + (c0 + j) * N;
+    // candidates of this batch and their own bounds
+    const int64_t k = cb0 + tid;
+    bool cand = false;
+    int pass = 0;
+    int32_t J = 0;
+    double cj = 0.0;
+    if (k < k1) {
+      cj = a.csort[k];
+      cand = cj * Bi > a.thresh;
+      if (cand) {
+        J = a.kord[k];
+        const double B = ket_bound(kb, a.cst + (int64_t)J * N);
+        pass = B != 0.0 && B * BOUND_SLACK * cj > a.thresh;
+      }
+    }
+    if (tid == PT - 1) sDone = !cand;  // kets are ordered by |c|
+    int64_t pos;
+    const int nbat = (int)block_scan(pass, sScan, &pos);
+    if (pass) {
+      sBKet[pos] = J;
+      sBCab[pos] = cj;
+      const uint8_t *st = a.cst + (int64_t)J * N;
       uint32_t key = 0;  // states below 255 (checked on the host): never the sentinel
       for (int q = 0; q < nb; q++) key |= (uint32_t)st[T.cl[q]] << (8 * q);
-      sKey[j] = key;
-      sKet[j] = j;
+      sKey[pos] = key;
+      sPos[pos] = (int16_t)pos;
     }
-    for (int j = nk + tid; j < KCH; j += PT) {
-      sKey[j] = 0xFFFFFFFFu;
-      sKet[j] = j;
+    int P2 = 1;
+    if (nbat > 1) {
+      P2 = 2;
+      while (P2 < nbat) P2 <<= 1;
+      if (tid >= nbat && tid < P2) {
+        sKey[tid] = 0xFFFFFFFFu;
+        sPos[tid] = (int16_t)tid;
+      }
     }
     __syncthreads();
-    // bitonic sort of (key, ket) over KCH entries; ties keep ket order since
-    // ket index is the secondary key
-    for (int k = 2; k <= KCH; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int i = tid; i < KCH; i += PT) {
-          const int l = i ^ j;
-          if (l > i) {
-            const bool up = (i & k) == 0;
-            const uint32_t ki = sKey[i], kl = sKey[l];
-            const int vi = sKet[i], vl = sKet[l];
-            const bool gt = ki > kl || (ki == kl && vi > vl);
-            if (gt == up) {
-              sKey[i] = kl;
-              sKey[l] = ki;
-              sKet[i] = vl;
-              sKet[l] = vi;
+    if (nbat > 1) {
+      // bitonic sort of (key, position) over the next power of two
+      for (int kk = 2; kk <= P2; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+          const int i = tid;
+          if (i < P2) {
+            const int l = i ^ j;
+            if (l > i) {
+              const bool up = (i & kk) == 0;
+              const uint32_t ki = sKey[i], kl = sKey[l];
+              const int vi = sPos[i], vl = sPos[l];
+              const bool gt = ki > kl || (ki == kl && vi > vl);
+              if (gt == up) {
+                sKey[i] = kl;
+                sKey[l] = ki;
+                sPos[i] = (int16_t)vl;
+                sPos[l] = (int16_t)vi;
+              }
             }
           }
+          __syncthreads();
         }
-        __syncthreads();
-      }
+    }
     if (tid == 0) {
       int ng = 0;
-      for (int j = 0; j < nk; j++)
+      for (int j = 0; j < nbat; j++)
         if (j == 0 || sKey[j] != sKey[j - 1]) sGrp[ng++] = j;
-      sGrp[ng] = nk;
+      sGrp[ng] = nbat;
       sNG = ng;
     }
     __syncthreads();
     const int ng = sNG;
     for (int gi = 0; gi < ng; gi++) {
       const int j0 = sGrp[gi], j1 = sGrp[gi + 1];
-      // group's active ket states and its largest |c|
-      const uint8_t *st0 = a.cst + (c0 + sKet[j0]) * N;
-      double cm = 0.0;
-      for (int j = j0 + tid; j < j1; j += PT) cm = fmax(cm, a.cabs[c0 + sKet[j]]);
-      sRed[tid] = cm;
-      __syncthreads();
-      for (int s = PT / 2; s > 0; s >>= 1) {
-        if (tid < s) sRed[tid] = fmax(sRed[tid], sRed[tid + s]);
-        __syncthreads();
-      }
-      cm = sRed[0];
+      // the group's active ket states; its largest |c| is its first member's
+      // (the batch is in |c| order and the sort keeps that order in a group)
+      const uint8_t *st0 = a.cst + (int64_t)sBKet[sPos[j0]] * N;
+      const double cm = sBCab[sPos[j0]];
       // column maxima max_I |Γ_q[p, I, J_q]| for the group's ket states
-      for (int q = 0; q < nb; q++) {
-        const int Jq = st0[T.cl[q]];
-        const double *cmx = a.cmx + a.cmx_off[sBlk[q]] + (int64_t)sNp[q] * Jq;
-        for (int p = tid; p < sNp[q]; p += PT) sCM[q][p] = cmx[p];
+      for (int e = tid; e < 4 * NPMAX; e += PT) {
+        const int q = e / NPMAX, p = e % NPMAX;
+        if (q < nb && p < sNp[q])
+          sCM[q][p] = a.cmx[a.cmx_off[sBlk[q]] + (int64_t)sNp[q] * st0[T.cl[q]] + p];
       }
       __syncthreads();
-      // group bound and per-cluster weights w_q[p_q] = sum over the other
-      // indices of |V| prod_{r != q} colmax_r
-      if (nb == 1) {
-        if (tid == 0) sW[0][0] = 1.0;
-      } else {
-        for (int e = tid; e < 4 * 64; e += PT) {
-          const int q = e / 64, pq = e % 64;
+      // per-cluster weights w_q[p_q] = sum over the other indices of
+      // |V| prod_{r != q} colmax_r
+      if (nb > 1) {
+        for (int e = tid; e < 4 * NPMAX; e += PT) {
+          const int q = e / NPMAX, pq = e % NPMAX;
           if (q >= nb || pq >= sNp[q]) continue;
-          int strides[4], st = 1;
-          for (int r = 0; r < nb; r++) {
-            strides[r] = st;
-            st *= T.dims[r];
+          int d[4], stride[4], idx[4];
+          int sacc = 1;
+          for (int r = 0; r < 4; r++) {
+            d[r] = r < nb ? T.dims[r] : 1;
+            stride[r] = sacc;
+            sacc *= d[r];
+            idx[r] = 0;
           }
+          idx[q] = pq;
+          const int total = nv / d[q];
           double s = 0.0;
-          for (int v = 0; v < nv; v++) {
-            int rem = v;
-            double f = fabs(sV[v]);
-            bool mine = true;
+          for (int v = 0; v < total; v++) {
+            int off = 0;
+            double f = 1.0;
             for (int r = 0; r < nb; r++) {
-              const int pr = rem % T.dims[r];
-              rem /= T.dims[r];
-              if (r == q) mine = pr == pq;
-              else f *= sCM[r][pr];
+              off += idx[r] * stride[r];
+              if (r != q) f *= sCM[r][idx[r]];
             }
-            if (mine) s += f;
+            s += fabs(sV[off]) * f;
+            for (int r = 0; r < nb; r++) {  // next combination of the other indices
+              if (r == q) continue;
+              if (++idx[r] < d[r]) break;
+              idx[r] = 0;
+            }
           }
           sW[q][pq] = s;
         }
+      } else if (tid == 0) {
+        sW[0][0] = 1.0;
       }
       __syncthreads();
-      double B;
-      if (nb == 1) {
-        B = sCM[0][0];
-      } else {
-        B = 0.0;
-        for (int p = 0; p < sNp[0]; p++) B += sW[0][p] * sCM[0][p];
-      }
-      if (!(B * BOUND_SLACK * cm > a.thresh) || B == 0.0) {
-        __syncthreads();
-        continue;  // uniform
-      }
       // survivor lists: bra states I_q whose bound can pass
-      if (tid < 4) sNS[tid] = 0;
+      {
+        const int q = tid / SMAX, I = tid % SMAX;
+        if (q < nb && I < sMb[q]) {
+          const int Jq = st0[T.cl[q]];
+          const int np = sNp[q];
+          const double *Gi = a.gdata + a.gb_off[sBlk[q]] + (int64_t)np * (I + (int64_t)sMb[q] * Jq);
+          double s = 0.0;
+          for (int p = 0; p < np; p++) s += sW[q][p] * fabs(Gi[p]);
+          sFlag[q][I] = s != 0.0 && s * BOUND_SLACK * cm > a.thresh;
+        }
+      }
       __syncthreads();
       if (tid < nb) {
         const int q = tid;
-        const int Jq = st0[T.cl[q]];
-        const double *G = a.gdata + a.gb_off[sBlk[q]];
-        const int np = sNp[q], mb = sMb[q];
         int n = 0;
-        for (int I = 0; I < mb; I++) {
-          double s = 0.0;
-          const double *Gi = G + (int64_t)np * (I + (int64_t)mb * Jq);
-          for (int p = 0; p < np; p++) s += sW[q][p] * fabs(Gi[p]);
-          if (s * BOUND_SLACK * cm > a.thresh && s != 0.0) sSurv[q][n++] = (int16_t)I;
-        }
+        for (int I = 0; I < sMb[q]; I++)
+          if (sFlag[q][I]) sSurv[q][n++] = (int16_t)I;
         sNS[q] = n;
       }
       __syncthreads();
       bool empty = false;
       for (int q = 0; q < nb; q++) empty |= sNS[q] == 0;
-      if (empty) {
-        __syncthreads();
-        continue;  // uniform
-      }
+      if (empty) continue;  // uniform; the next group starts with a barrier-free read
       groups_kept++;
       // chain: contract the last active cluster first, tiled over its
       // surviving states so every stored stage fits half the arena.  Stored
@@ -539,7 +667,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
         }
       }
       if (need > ARENA / 2) {  // uniform: flags the call, computes nothing wrong
-        if (tid == 0) atomicAdd(&a.stats[2], 1ull);
+        if (tid == 0 && !FILL) atomicAdd(&a.stats[2], 1ull);
         __syncthreads();
         continue;
       }
@@ -598,9 +726,9 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
         // output e = i0 + ns0 * col, col = i_1 + |S_1| (i_2 + ... (tile index));
         // kept contributions counted (and, filling, written in this order)
         const int64_t base = sEmit;  // read before the scan's barriers
-        for (int pass = 0; pass < (FILL ? 2 : 1); pass++) {
-          int64_t pos = 0, cnt = 0;
-          if (pass == 1) pos = base + sScan[tid];  // exclusive prefix of pass 0
+        for (int pass2 = 0; pass2 < (FILL ? 2 : 1); pass2++) {
+          int64_t wpos = 0, cnt = 0;
+          if (pass2 == 1) wpos = base + sScan[tid];  // exclusive prefix of pass 0
           for (int e = tid; e < nout; e += PT) {
             const int i0 = e % ns0, col = e / ns0;
             double x;
@@ -624,12 +752,12 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
               bstate[last] = sSurv[last][t0 + rem];
             }
             for (int j = j0; j < j1; j++) {
-              const int64_t J = c0 + sKet[j];
-              if (!(ax > a.thr[J])) continue;  // _collect_significant_thread!'s test
-              if (pass == 1) {
+              const int64_t Jm = sBKet[sPos[j]];
+              if (!(ax > a.thr[Jm])) continue;  // _collect_significant_thread!'s test
+              if (pass2 == 1) {
                 // key: target Fock sectors | bra states on the active
                 // clusters, the ket's states elsewhere
-                const uint8_t *sj = a.cst + J * N;
+                const uint8_t *sj = a.cst + Jm * N;
                 uint64_t lo = 0;
                 for (int cc = 0; cc < N; cc++) {
                   int sv = sj[cc];
@@ -637,34 +765,40 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
                     if (T.cl[q] == cc) sv = bstate[q];
                   lo |= (uint64_t)sv << (a.sbits * cc);
                 }
-                a.ekey[pos] = Key128{sFockKey, lo};
+                a.ekey[wpos] = Key128{sFockKey, lo};
                 const double v = sign * x;
                 for (int r = 0; r < a.R; r++)
-                  a.eval[pos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + J];
-                pos++;
+                  a.eval[wpos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + Jm];
+                wpos++;
               }
               cnt++;
             }
           }
-          if (pass == 0) {
+          if (pass2 == 0) {
             int64_t excl;
             const int64_t tot = block_scan(cnt, sScan, &excl);
             // block_scan ends with a barrier: every thread has read sEmit
-            if (FILL) {
-              sScan[tid] = excl;  // own slot only; read back by this thread in pass 1
-            }
+            if (FILL) sScan[tid] = excl;  // own slot only; read back by this thread in pass 1
             if (tid == 0) sEmit = base + tot;
           }
         }
         __syncthreads();
       }
     }
+    if (sDone) break;  // uniform: written before the batch's scan barriers
   }
   __syncthreads();
-  if (!FILL && tid == 0) a.emit_count[it] = sEmit;
-  if (tid == 0 && groups_kept) atomicAdd(&a.stats[0], groups_kept);
+  if (!FILL && tid == 0) a.emit_count[slot] = sEmit;
+  if (!FILL && tid == 0 && groups_kept) atomicAdd(&a.stats[0], groups_kept);
   // every thread counted the same (uniform) chain flops
-  if (tid == 0 && flops) atomicAdd(&a.stats[1], flops);
+  if (!FILL && tid == 0 && flops) atomicAdd(&a.stats[1], flops);
+}
+
+__global__ void k_gather_keys(int64_t n, const uint32_t *idx, const uint64_t *key,
+                              uint64_t *out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = key[idx[i]];
 }
 
 // per Γ block: bmax[p] = max_{I,J} |Γ[p,I,J]|, cmx[p + np J] = max_I |Γ[p,I,J]|
@@ -840,8 +974,25 @@ int bits_for(int64_t n) {  // bits to hold values 0 .. n-1
 }
 
 constexpr int64_t FB_CHUNK = 65535;
-constexpr int64_t ITEM_CHUNK = 1 << 20;
+constexpr int64_t ITEM_CHUNK = 1 << 24;      // workgroups per screening launch
+constexpr int64_t SLOT_CHUNK = 1 << 22;      // workgroups per contraction launch
+constexpr int64_t CHUNK_E = int64_t(1) << 27;  // contributions per accumulation chunk
 constexpr int SUM_BLOCKS = 1024;
+
+// contributions per accumulation chunk: FCG_PT2_CHUNK in the environment
+// lowers it for devices with little free memory
+int64_t chunk_limit() {
+  const char *e = std::getenv("FCG_PT2_CHUNK");
+  if (e) {
+    const long long v = std::atoll(e);
+    if (v > 0) return v;
+  }
+  return CHUNK_E;
+}
+
+double seconds_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+}
 
 }  // namespace
 
@@ -864,9 +1015,12 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   // configurations of the space (the boxes enumerate them in the caller's
   // order), grouped by Fock block
   const int64_t n_cfg = var->nvar, n_fock = var->n_fock;
+  if (n_cfg >= (int64_t(1) << 31)) {
+    ctx->err = "fcg_tpsci_pt2_sparse: more than 2^31 configurations";
+    return FCG_EOVERFLOW;
+  }
   std::vector<uint8_t> cst((size_t)std::max<int64_t>(n_cfg, 1) * N);
   std::vector<int64_t> fstart(n_fock + 1, 0);
-  int max_state = 1;
   for (int64_t b = 0; b < var->n_box; b++) {
     const int32_t f = var->box_fock[b];
     if (b > 0 && f < var->box_fock[b - 1]) {
@@ -931,27 +1085,39 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
       }
   }
   // key widths: sector codes and states of every cluster must fit 64 bits each
-  int mstates = 1;
-  for (int v : o->nstates) mstates = std::max(mstates, v);
-  max_state = mstates;
+  int max_state = 1;
+  for (int v : o->nstates) max_state = std::max(max_state, v);
   const int fbits = bits_for((int64_t)S * S), sbits = bits_for(max_state);
   if (fbits * N > 64 || sbits * N > 64) {
     ctx->err = "fcg_tpsci_pt2_sparse: configuration keys need more than 128 bits";
     return FCG_EOVERFLOW;
   }
-  // screening thresholds per ket (the reference's thresh / max_r |c|)
+  // screening thresholds per ket (the reference's thresh / max_r |c|), and
+  // each Fock block's kets ordered by max_r |c| descending (ties by position)
   std::vector<double> cabs(std::max<int64_t>(n_cfg, 1), 0.0), thr(std::max<int64_t>(n_cfg, 1));
   std::vector<double> cmaxF(std::max<int64_t>(n_fock, 1), 0.0);
+  std::vector<int32_t> kord(std::max<int64_t>(n_cfg, 1));
+  std::vector<double> csort(std::max<int64_t>(n_cfg, 1));
+#pragma omp parallel for schedule(static)
   for (int64_t j = 0; j < n_cfg; j++) {
     double m = 0.0;
     for (int r = 0; r < R; r++) m = std::max(m, std::fabs(c[(int64_t)r * n_cfg + j]));
     cabs[j] = m;
     thr[j] = m > 0.0 ? thresh / m : INFINITY;
   }
-  for (int64_t f = 0; f < n_fock; f++)
-    for (int64_t j = fstart[f]; j < fstart[f + 1]; j++) cmaxF[f] = std::max(cmaxF[f], cabs[j]);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t f = 0; f < n_fock; f++) {
+    const int64_t j0 = fstart[f], j1 = fstart[f + 1];
+    std::iota(kord.begin() + j0, kord.begin() + j1, (int32_t)j0);
+    std::sort(kord.begin() + j0, kord.begin() + j1, [&](int32_t x, int32_t y) {
+      return cabs[x] > cabs[y] || (cabs[x] == cabs[y] && x < y);
+    });
+    for (int64_t j = j0; j < j1; j++) csort[j] = cabs[kord[j]];
+    cmaxF[f] = j1 > j0 ? csort[j0] : 0.0;
+  }
   // variational keys
   std::vector<Key128> vk(std::max<int64_t>(n_cfg, 1));
+#pragma omp parallel for schedule(dynamic, 64)
   for (int64_t f = 0; f < n_fock; f++) {
     uint64_t hi = 0;
     for (int cc = 0; cc < N; cc++)
@@ -989,10 +1155,12 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
     bmax_off[b + 1] = bmax_off[b] + o->gb_dims[3 * b];
     cmx_off[b + 1] = cmx_off[b] + (int64_t)o->gb_dims[3 * b] * o->gb_dims[3 * b + 2];
   }
+  vlog("host preparation", t0);
   Buf<int64_t> d_fstart(ctx), d_bmax_off(ctx), d_cmx_off(ctx), d_item_count(ctx), d_item_ptr(ctx);
   Buf<int16_t> d_fock(ctx), d_tc(ctx);
   Buf<uint8_t> d_cst(ctx);
-  Buf<double> d_cabs(ctx), d_thr(ctx), d_cmaxF(ctx), d_c(ctx), d_bmax(ctx), d_cmx(ctx), d_e0(ctx);
+  Buf<int32_t> d_kord(ctx);
+  Buf<double> d_csort(ctx), d_thr(ctx), d_cmaxF(ctx), d_c(ctx), d_bmax(ctx), d_cmx(ctx), d_e0(ctx);
   Buf<PTerm> d_terms(ctx);
   Buf<int32_t> d_tcs(ctx), d_gblock(ctx);
   Buf<Key128> d_vkey(ctx);
@@ -1000,7 +1168,8 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   int rc = d_fstart.upload(fstart.data(), n_fock + 1);
   if (!rc) rc = d_fock.upload(var->fock.data(), (int64_t)var->fock.size());
   if (!rc) rc = d_cst.upload(cst.data(), n_cfg * N);
-  if (!rc) rc = d_cabs.upload(cabs.data(), n_cfg);
+  if (!rc) rc = d_kord.upload(kord.data(), n_cfg);
+  if (!rc) rc = d_csort.upload(csort.data(), n_cfg);
   if (!rc) rc = d_thr.upload(thr.data(), n_cfg);
   if (!rc) rc = d_cmaxF.upload(cmaxF.data(), n_fock);
   if (!rc) rc = d_c.upload(c, (int64_t)R * n_cfg);
@@ -1039,7 +1208,8 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   a.fstart = d_fstart.p;
   a.fock = d_fock.p;
   a.cst = d_cst.p;
-  a.cabs = d_cabs.p;
+  a.kord = d_kord.p;
+  a.csort = d_csort.p;
   a.thr = d_thr.p;
   a.cmaxF = d_cmaxF.p;
   a.c = d_c.p;
@@ -1074,130 +1244,166 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   std::vector<int64_t> iptr(n_fock + 1, 0);
   for (int64_t f = 0; f < n_fock; f++) iptr[f + 1] = iptr[f] + icnt[f];
   const int64_t n_items = iptr[n_fock];
+  if (n_items >= (int64_t(1) << 32)) {
+    ctx->err = "fcg_tpsci_pt2_sparse: more than 2^32 (ket Fock block, term) items";
+    return FCG_EOVERFLOW;
+  }
   vlog("items counted", t0, (double)n_items);
-  Buf<int32_t> d_item_f(ctx), d_item_t(ctx);
+  Buf<int32_t> d_item_f(ctx), d_item_t(ctx), d_nsurv(ctx);
+  Buf<double> d_item_b(ctx);
+  Buf<uint64_t> d_item_key(ctx);
   if ((rc = d_item_ptr.upload(iptr.data(), n_fock + 1))) return rc;
-  if (!d_item_f.alloc(n_items) || !d_item_t.alloc(n_items)) {
+  if (!d_item_f.alloc(n_items) || !d_item_t.alloc(n_items) || !d_item_b.alloc(n_items) ||
+      !d_item_key.alloc(n_items) || !d_nsurv.alloc(n_items)) {
     ctx->err = "PT2 item list allocation failed";
     return FCG_ENOMEM;
   }
   a.item_ptr = d_item_ptr.p;
   a.item_f = d_item_f.p;
   a.item_t = d_item_t.p;
+  a.item_b = d_item_b.p;
+  a.item_key = d_item_key.p;
   for (int64_t f0 = 0; f0 < n_fock; f0 += FB_CHUNK) {
     a.fb_base = f0;
     hipLaunchKernelGGL(k_pt2_items<true>, dim3((unsigned)std::min(FB_CHUNK, n_fock - f0)),
                        dim3(PT), 0, s, a);
   }
   FCG_HIP(hipGetLastError(), ctx);
-  // emission: count, prefix, fill
+  // screening: kept kets per item
+  a.items_f = d_item_f.p;
+  a.items_t = d_item_t.p;
+  a.items_b = d_item_b.p;
+  a.n_items = n_items;
+  a.nsurv = d_nsurv.p;
+  for (int64_t i0 = 0; i0 < n_items; i0 += ITEM_CHUNK) {
+    a.item_base = i0;
+    hipLaunchKernelGGL(k_pt2_screen, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
+                       dim3(SW), 0, s, a);
+  }
+  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<int32_t> nsurv(std::max<int64_t>(n_items, 1));
+  FCG_HIP(hipMemcpyAsync(nsurv.data(), d_nsurv.p, sizeof(int32_t) * n_items,
+                         hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  std::vector<uint32_t> kept;
+  int64_t kept_kets = 0;
+  for (int64_t i = 0; i < n_items; i++)
+    if (nsurv[i] > 0) {
+      kept.push_back((uint32_t)i);
+      kept_kets += nsurv[i];
+    }
+  const int64_t n_slots = (int64_t)kept.size();
+  std::vector<int32_t>().swap(nsurv);
+  vlog("items screened, kept", t0, (double)n_slots);
+  vlog("... (item, ket) pairs kept", t0, (double)kept_kets);
+  // slots: the kept items ordered by target Fock block (stable)
+  Buf<uint32_t> d_kept(ctx), d_slot(ctx);
+  Buf<uint64_t> d_kkey(ctx), d_skkey(ctx);
+  if ((rc = d_kept.upload(kept.data(), n_slots))) return rc;
+  if (!d_slot.alloc(n_slots) || !d_kkey.alloc(n_slots) || !d_skkey.alloc(n_slots)) {
+    ctx->err = "PT2 slot allocation failed";
+    return FCG_ENOMEM;
+  }
+  std::vector<uint64_t> slot_key(std::max<int64_t>(n_slots, 1));
+  Buf<char> d_tmp(ctx);
+  if (n_slots > 0) {
+    hipLaunchKernelGGL(k_gather_keys, dim3(1024), dim3(256), 0, s, n_slots, d_kept.p,
+                       d_item_key.p, d_kkey.p);
+    size_t tb = 0;
+    FCG_HIP(rocprim::radix_sort_pairs(nullptr, tb, d_kkey.p, d_skkey.p, d_kept.p, d_slot.p,
+                                      (size_t)n_slots, 0u, 64u, s),
+            ctx);
+    if (!d_tmp.alloc((int64_t)tb)) {
+      ctx->err = "PT2 slot sort workspace allocation failed";
+      return FCG_ENOMEM;
+    }
+    FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_kkey.p, d_skkey.p, d_kept.p, d_slot.p,
+                                      (size_t)n_slots, 0u, 64u, s),
+            ctx);
+    FCG_HIP(hipMemcpyAsync(slot_key.data(), d_skkey.p, sizeof(uint64_t) * n_slots,
+                           hipMemcpyDeviceToHost, s), ctx);
+  }
+  // contraction, count pass
   Buf<int64_t> d_ecount(ctx), d_eptr(ctx);
-  if (!d_ecount.alloc(n_items)) {
+  if (!d_ecount.alloc(n_slots)) {
     ctx->err = "PT2 emission counts allocation failed";
     return FCG_ENOMEM;
   }
-  a.items_f = d_item_f.p;
-  a.items_t = d_item_t.p;
-  a.n_items = n_items;
+  a.slot_item = d_slot.p;
+  a.n_slots = n_slots;
   a.emit_count = d_ecount.p;
-  for (int64_t i0 = 0; i0 < n_items; i0 += ITEM_CHUNK) {
-    a.item_base = i0;
-    hipLaunchKernelGGL(k_pt2_emit<false>, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
+  const auto te = std::chrono::steady_clock::now();
+  for (int64_t i0 = 0; i0 < n_slots; i0 += SLOT_CHUNK) {
+    a.slot_base = i0;
+    hipLaunchKernelGGL(k_pt2_emit<false>, dim3((unsigned)std::min(SLOT_CHUNK, n_slots - i0)),
                        dim3(PT), 0, s, a);
     if (verbose()) {
       FCG_HIP(hipStreamSynchronize(s), ctx);
-      vlog("emission count chunk", t0, (double)std::min(i0 + ITEM_CHUNK, n_items));
+      vlog("contraction count chunk", t0, (double)std::min(i0 + SLOT_CHUNK, n_slots));
     }
   }
   FCG_HIP(hipGetLastError(), ctx);
-  std::vector<int64_t> ecnt(std::max<int64_t>(n_items, 1));
-  FCG_HIP(hipMemcpyAsync(ecnt.data(), d_ecount.p, sizeof(int64_t) * n_items,
+  std::vector<int64_t> ecnt(std::max<int64_t>(n_slots, 1));
+  FCG_HIP(hipMemcpyAsync(ecnt.data(), d_ecount.p, sizeof(int64_t) * n_slots,
                          hipMemcpyDeviceToHost, s), ctx);
   unsigned long long stc[4];
   FCG_HIP(hipMemcpyAsync(stc, d_stats.p, sizeof(stc), hipMemcpyDeviceToHost, s), ctx);
   FCG_HIP(hipStreamSynchronize(s), ctx);
+  double emit_s = seconds_since(te);
   if (stc[2]) {
     ctx->err = "fcg_tpsci_pt2_sparse: a term's shape exceeds the staged contraction";
     return FCG_EOVERFLOW;
   }
-  std::vector<int64_t> eptr(n_items + 1, 0);
-  for (int64_t i = 0; i < n_items; i++) eptr[i + 1] = eptr[i] + ecnt[i];
-  const int64_t E = eptr[n_items];
+  std::vector<int64_t> eptr(n_slots + 1, 0);
+  for (int64_t i = 0; i < n_slots; i++) eptr[i + 1] = eptr[i] + ecnt[i];
+  const int64_t E = eptr[n_slots];
   vlog("contributions counted", t0, (double)E);
-  if ((rc = d_eptr.upload(eptr.data(), n_items + 1))) return rc;
+  // chunks of whole target blocks, at most CHUNK_E contributions unless one
+  // block alone holds more
+  std::vector<int64_t> cut{0};
+  {
+    const int64_t limit = chunk_limit();
+    int64_t i = 0;
+    while (i < n_slots) {
+      int64_t j = i;
+      while (j < n_slots) {
+        int64_t k = j + 1;  // end of the target block starting at j
+        while (k < n_slots && slot_key[k] == slot_key[j]) k++;
+        if (j > i && eptr[k] - eptr[i] > limit) break;
+        j = k;
+      }
+      cut.push_back(j);
+      i = j;
+    }
+  }
+  const int n_chunks = (int)cut.size() - 1;
+  int64_t maxE = 0;
+  for (int ch = 0; ch < n_chunks; ch++) maxE = std::max(maxE, eptr[cut[ch + 1]] - eptr[cut[ch]]);
+  if ((rc = d_eptr.upload(eptr.data(), n_slots + 1))) return rc;
   Buf<Key128> d_ekey(ctx), d_skey(ctx);
   Buf<double> d_eval(ctx);
-  Buf<uint64_t> d_idx(ctx), d_perm(ctx);
-  if (!d_ekey.alloc(E) || !d_eval.alloc(E * R) || !d_skey.alloc(E) || !d_idx.alloc(E) ||
-      !d_perm.alloc(E)) {
-    ctx->err = "PT2 emission buffers (" + std::to_string(E) + " contributions) do not fit";
+  Buf<uint64_t> d_idx(ctx), d_perm(ctx), d_segid(ctx);
+  Buf<uint32_t> d_head(ctx);
+  Buf<int64_t> d_seg(ctx);
+  if (!d_ekey.alloc(maxE) || !d_eval.alloc(maxE * R) || !d_skey.alloc(maxE) ||
+      !d_idx.alloc(maxE) || !d_perm.alloc(maxE) || !d_head.alloc(maxE) || !d_segid.alloc(maxE) ||
+      !d_seg.alloc(maxE + 1)) {
+    ctx->err = "PT2 emission buffers (" + std::to_string(maxE) + " contributions) do not fit";
     return FCG_ENOMEM;
   }
-  a.emit_ptr = d_eptr.p;
-  a.ekey = d_ekey.p;
-  a.eval = d_eval.p;
-  FCG_HIP(hipMemsetAsync(d_stats.p, 0, 4 * sizeof(unsigned long long), s), ctx);
-  for (int64_t i0 = 0; i0 < n_items; i0 += ITEM_CHUNK) {
-    a.item_base = i0;
-    hipLaunchKernelGGL(k_pt2_emit<true>, dim3((unsigned)std::min(ITEM_CHUNK, n_items - i0)),
-                       dim3(PT), 0, s, a);
-  }
-  FCG_HIP(hipGetLastError(), ctx);
-  if (verbose()) {
-    FCG_HIP(hipStreamSynchronize(s), ctx);
-    vlog("contributions written", t0);
-  }
-  // stable sort of the contributions by x
-  hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, E, d_idx.p);
-  size_t tmp_bytes = 0;
-  FCG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
-                                    (size_t)E, KeyDecomposer{}, 0u, 128u, s),
+  size_t sort_bytes = 0, scan_bytes = 0;
+  FCG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
+                                    (size_t)std::max<int64_t>(maxE, 1), KeyDecomposer{}, 0u,
+                                    128u, s),
           ctx);
-  Buf<char> d_tmp(ctx);
-  if (!d_tmp.alloc((int64_t)tmp_bytes)) {
+  FCG_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
+                                  (size_t)std::max<int64_t>(maxE, 1), rocprim::plus<uint64_t>(),
+                                  s),
+          ctx);
+  if (!d_tmp.alloc((int64_t)std::max(sort_bytes, scan_bytes))) {
     ctx->err = "PT2 sort workspace allocation failed";
     return FCG_ENOMEM;
   }
-  FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tmp_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
-                                    (size_t)E, KeyDecomposer{}, 0u, 128u, s),
-          ctx);
-  // segments of equal keys: heads, ids by scan, starts
-  Buf<uint32_t> d_head(ctx);
-  Buf<uint64_t> d_segid(ctx);
-  if (!d_head.alloc(E) || !d_segid.alloc(E)) {
-    ctx->err = "PT2 segment buffers allocation failed";
-    return FCG_ENOMEM;
-  }
-  hipLaunchKernelGGL(k_heads, dim3(2048), dim3(256), 0, s, E, d_skey.p, d_head.p);
-  size_t scan_bytes = 0;
-  FCG_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
-                                  (size_t)E, rocprim::plus<uint64_t>(), s),
-          ctx);
-  if (!d_tmp.alloc((int64_t)std::max(scan_bytes, (size_t)1))) {
-    ctx->err = "PT2 scan workspace allocation failed";
-    return FCG_ENOMEM;
-  }
-  FCG_HIP(rocprim::exclusive_scan(d_tmp.p, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
-                                  (size_t)E, rocprim::plus<uint64_t>(), s),
-          ctx);
-  uint64_t last_id = 0;
-  uint32_t last_head = 0;
-  if (E > 0) {
-    FCG_HIP(hipMemcpyAsync(&last_id, d_segid.p + E - 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                           s), ctx);
-    FCG_HIP(hipMemcpyAsync(&last_head, d_head.p + E - 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           s), ctx);
-  }
-  FCG_HIP(hipStreamSynchronize(s), ctx);
-  const int64_t n_seg = E > 0 ? (int64_t)last_id + last_head : 0;
-  vlog("sorted, configurations", t0, (double)n_seg);
-  Buf<int64_t> d_seg(ctx);
-  if (!d_seg.alloc(n_seg + 1)) {
-    ctx->err = "PT2 segment table allocation failed";
-    return FCG_ENOMEM;
-  }
-  hipLaunchKernelGGL(k_seg_start, dim3(2048), dim3(256), 0, s, E, d_head.p, d_segid.p, d_seg.p);
-  FCG_HIP(hipMemcpyAsync(d_seg.p + n_seg, &E, sizeof(int64_t), hipMemcpyHostToDevice, s), ctx);
   Buf<double> d_part(ctx);
   Buf<unsigned long long> d_nout(ctx);
   if (!d_part.alloc((int64_t)SUM_BLOCKS * R * 2) || !d_nout.alloc(1)) {
@@ -1205,6 +1411,9 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
     return FCG_ENOMEM;
   }
   FCG_HIP(hipMemsetAsync(d_nout.p, 0, sizeof(unsigned long long), s), ctx);
+  a.emit_ptr = d_eptr.p;
+  a.ekey = d_ekey.p;
+  a.eval = d_eval.p;
   SumArgs sa{};
   sa.N = N;
   sa.S = S;
@@ -1213,7 +1422,6 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   sa.h0_op = h0_op;
   sa.fbits = fbits;
   sa.sbits = sbits;
-  sa.n_seg = n_seg;
   sa.seg_start = d_seg.p;
   sa.perm = d_perm.p;
   sa.skey = d_skey.p;
@@ -1227,34 +1435,88 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   sa.e0 = d_e0.p;
   sa.part = d_part.p;
   sa.n_out = d_nout.p;
-  hipLaunchKernelGGL(k_pt2_seg_sum, dim3(SUM_BLOCKS), dim3(256), 0, s, sa);
-  FCG_HIP(hipGetLastError(), ctx);
+  std::vector<double> sum(R, 0.0), comp(R, 0.0);
   std::vector<double> part((size_t)SUM_BLOCKS * R * 2);
-  unsigned long long nout = 0;
-  FCG_HIP(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * part.size(),
-                         hipMemcpyDeviceToHost, s), ctx);
-  FCG_HIP(hipMemcpyAsync(&nout, d_nout.p, sizeof(nout), hipMemcpyDeviceToHost, s), ctx);
-  FCG_HIP(hipMemcpyAsync(stc, d_stats.p, sizeof(stc), hipMemcpyDeviceToHost, s), ctx);
-  FCG_HIP(hipStreamSynchronize(s), ctx);
-  for (int r = 0; r < R; r++) {
-    double sum = 0.0, comp = 0.0;
-    for (int blk = 0; blk < SUM_BLOCKS; blk++) {
-      const double v = part[((size_t)blk * R + r) * 2];
-      const double t = sum + v;
-      comp += std::fabs(sum) >= std::fabs(v) ? (sum - t) + v : (v - t) + sum;
-      sum = t;
-      comp += part[((size_t)blk * R + r) * 2 + 1];
+  int64_t n_seg_total = 0;
+  double sort_s = 0.0;
+  for (int ch = 0; ch < n_chunks; ch++) {
+    const int64_t s0 = cut[ch], s1 = cut[ch + 1];
+    const int64_t Ec = eptr[s1] - eptr[s0];
+    if (Ec == 0) continue;
+    const auto tf = std::chrono::steady_clock::now();
+    a.emit_base = eptr[s0];
+    a.n_slots = s1;
+    for (int64_t i0 = s0; i0 < s1; i0 += SLOT_CHUNK) {
+      a.slot_base = i0;
+      hipLaunchKernelGGL(k_pt2_emit<true>, dim3((unsigned)std::min(SLOT_CHUNK, s1 - i0)),
+                         dim3(PT), 0, s, a);
     }
-    e2[r] = sum + comp;
+    FCG_HIP(hipGetLastError(), ctx);
+    if (verbose()) {
+      FCG_HIP(hipStreamSynchronize(s), ctx);
+      vlog("contributions written", t0, (double)Ec);
+    }
+    FCG_HIP(hipStreamSynchronize(s), ctx);
+    emit_s += seconds_since(tf);
+    const auto ts = std::chrono::steady_clock::now();
+    // stable sort of the chunk's contributions by x
+    hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, Ec, d_idx.p);
+    size_t tb = sort_bytes;
+    FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
+                                      (size_t)Ec, KeyDecomposer{}, 0u, 128u, s),
+            ctx);
+    // segments of equal keys: heads, ids by scan, starts
+    hipLaunchKernelGGL(k_heads, dim3(2048), dim3(256), 0, s, Ec, d_skey.p, d_head.p);
+    size_t sb = scan_bytes;
+    FCG_HIP(rocprim::exclusive_scan(d_tmp.p, sb, d_head.p, d_segid.p, (uint64_t)0, (size_t)Ec,
+                                    rocprim::plus<uint64_t>(), s),
+            ctx);
+    uint64_t last_id = 0;
+    uint32_t last_head = 0;
+    FCG_HIP(hipMemcpyAsync(&last_id, d_segid.p + Ec - 1, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           s), ctx);
+    FCG_HIP(hipMemcpyAsync(&last_head, d_head.p + Ec - 1, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s), ctx);
+    FCG_HIP(hipStreamSynchronize(s), ctx);
+    const int64_t n_seg = (int64_t)last_id + last_head;
+    hipLaunchKernelGGL(k_seg_start, dim3(2048), dim3(256), 0, s, Ec, d_head.p, d_segid.p,
+                       d_seg.p);
+    FCG_HIP(hipMemcpyAsync(d_seg.p + n_seg, &Ec, sizeof(int64_t), hipMemcpyHostToDevice, s), ctx);
+    sa.n_seg = n_seg;
+    hipLaunchKernelGGL(k_pt2_seg_sum, dim3(SUM_BLOCKS), dim3(256), 0, s, sa);
+    FCG_HIP(hipGetLastError(), ctx);
+    FCG_HIP(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, s), ctx);
+    FCG_HIP(hipStreamSynchronize(s), ctx);
+    sort_s += seconds_since(ts);
+    n_seg_total += n_seg;
+    for (int r = 0; r < R; r++)
+      for (int blk = 0; blk < SUM_BLOCKS; blk++) {
+        const double v = part[((size_t)blk * R + r) * 2];
+        const double t = sum[r] + v;
+        comp[r] += std::fabs(sum[r]) >= std::fabs(v) ? (sum[r] - t) + v : (v - t) + sum[r];
+        sum[r] = t;
+        comp[r] += part[((size_t)blk * R + r) * 2 + 1];
+      }
+    if (verbose()) vlog("chunk summed, configurations", t0, (double)n_seg);
   }
+  unsigned long long nout = 0;
+  FCG_HIP(hipMemcpyAsync(&nout, d_nout.p, sizeof(nout), hipMemcpyDeviceToHost, s), ctx);
+  FCG_HIP(hipStreamSynchronize(s), ctx);
+  for (int r = 0; r < R; r++) e2[r] = sum[r] + comp[r];
   if (st) {
     st->n_items = n_items;
     st->n_groups = (int64_t)stc[0];
     st->n_contributions = E;
-    st->n_configs = n_seg;
+    st->n_configs = n_seg_total;
     st->n_first_order = (int64_t)nout;
     st->chain_flops = (double)stc[1];
-    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->seconds = seconds_since(t0);
+    st->n_items_kept = n_slots;
+    st->n_kets_kept = kept_kets;
+    st->n_chunks = n_chunks;
+    st->contraction_seconds = emit_s;
+    st->sort_seconds = sort_s;
   }
   return FCG_OK;
 }

@@ -167,14 +167,17 @@ def pt2_12c_M32(seed=4, R=1, M=32):
     return p, st
 
 
-def cmf_first_order_state(p, st, engine):
+def cmf_first_order_state(p, st, engine, shift=1.0, select=0.0):
     """The CMF reference (the ground configuration of the reference Fock
-    block) plus its first-order correction projected on the space, each
-    configuration J weighted (H|ref>)_J / (<ref|H0|ref> - H0_JJ) with H0 the
-    clusters' Hcmf -- formed with the matrix-free sigma of the rank <= 1
-    space (the Hamiltonian applied to the reference alone) and its H0
-    diagonal; normalised.  Returns the coefficient vector (n x 1)."""
-    from .tpsci import cluster_sizes_of  # noqa: F401
+    block) plus a regularised first-order correction on the space: every
+    other configuration J gets a_J = -(H|ref>)_J / (|H0_JJ - H0_ref| + shift),
+    with H0 the clusters' Hcmf and shift in Eh (synthetic integrals put some
+    charge-transfer blocks below the reference in H0, so the plain
+    denominators change sign and come near zero).  Amplitudes with
+    |a_J| <= select are set to zero (one TPSCI selection step from the
+    reference; the configurations stay in the space).  H|ref> is the
+    matrix-free sigma of the rank <= 1 space applied to the reference alone.
+    Normalised; returns the coefficient vector (n x 1)."""
     ref_f = p.focks.index(p.ref_fock)
     ref_row = int(st.fock_start[ref_f])
     assert all(int(x) == 1 for x in st.cfg[ref_row])
@@ -184,9 +187,8 @@ def cmf_first_order_state(p, st, engine):
     sig = pl.sigma(x)[:, 0]
     h0 = engine.local_operator(p.clustered_ham, "Hcmf")
     hd0 = engine.tpsci_plan(st, p.cluster_ops, h0, matrix_free=True).diagonal()
-    den = hd0[ref_row] - hd0
-    den = np.where(np.abs(den) < 1e-3, np.copysign(1e-3, den + 1e-300), den)
-    c = sig / den
+    c = -sig / (np.abs(hd0 - hd0[ref_row]) + shift)
+    c[np.abs(c) <= select] = 0.0
     c[ref_row] = 1.0
     c /= np.linalg.norm(c)
     return c[:, None]

@@ -264,7 +264,11 @@ int fcg_tpsci_pt2(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, 
  * test, summed per first-order configuration x in a fixed order, and e2[r] =
  * sum over x outside the variational space of sigma[x,r]^2 /
  * (e0[r] - H0_diag[x]).  Arguments as fcg_tpsci_pt2; it never forms a target
- * Fock block whole, so it serves many clusters.  FCG_EOVERFLOW when the
+ * Fock block whole, so it serves many clusters, and it accumulates in
+ * chunks of whole target Fock blocks, so device memory bounds the largest
+ * target block's contributions rather than all of them (2^27 per chunk, or
+ * FCG_PT2_CHUNK from the environment).  e2 is the same on every call with
+ * the same inputs and chunk size (fixed summation order).  FCG_EOVERFLOW when the
  * configuration keys need more than 2 x 64 bits or a term's shape exceeds
  * the staged contraction (then use fcg_tpsci_pt2). */
 typedef struct {
@@ -275,6 +279,11 @@ typedef struct {
   int64_t n_first_order;    /* ... outside the variational space */
   double chain_flops;       /* flops of the kept contractions */
   double seconds;           /* wall time of the call */
+  int64_t n_items_kept;     /* items with a ket passing its own bound */
+  int64_t n_kets_kept;      /* (item, ket) pairs passing the ket bound */
+  int64_t n_chunks;         /* accumulation chunks (whole target Fock blocks) */
+  double contraction_seconds; /* count and fill passes of the contraction */
+  double sort_seconds;      /* sorting and summing the contributions */
 } fcg_pt2_sparse_stats;
 
 int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,

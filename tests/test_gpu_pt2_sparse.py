@@ -70,3 +70,25 @@ def test_sparse_pt2_small_cases(eng, name, nbody, thresh):
                             verbose=0, method="sparse", engine=eng)
     print(name, nbody, thresh, e2, ref, flush=True)
     assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("chunk", [1, 997])
+def test_sparse_pt2_chunked_accumulation(eng, six, chunk, monkeypatch):
+    """Accumulation cut into many chunks of whole target Fock blocks (chunk=1
+    puts every target block in a chunk of its own) agrees with the oracle and
+    keeps the contribution and configuration counts of the unchunked call."""
+    e0 = _e0(six, six.state)
+    thresh = 1e-6
+    ref = _oracle(six, e0, thresh)
+    kw = dict(E0=e0, thresh_foi=thresh, verbose=0, method="sparse", engine=eng)
+    whole = compute_pt2_energy(six.state, six.ops, six.H, **kw)
+    st0 = dict(compute_pt2_energy.last_stats)
+    monkeypatch.setenv("FCG_PT2_CHUNK", str(chunk))
+    e2 = compute_pt2_energy(six.state, six.ops, six.H, **kw)
+    st = compute_pt2_energy.last_stats
+    print(chunk, e2, whole, ref, st["n_chunks"], st0["n_chunks"], flush=True)
+    assert st0["n_chunks"] == 1 and st["n_chunks"] > 1
+    for k in ("n_contributions", "n_configs", "n_first_order", "n_items_kept", "n_kets_kept"):
+        assert st[k] == st0[k], k
+    assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.abs(e2 - whole).max() < 1e-13 * np.abs(whole).max()
