@@ -75,7 +75,7 @@ namespace {
 constexpr int PT = 256;          // threads per contraction workgroup
 constexpr int SW = 64;           // threads per screening workgroup (one wavefront)
 constexpr int KCH = PT;          // candidate kets per batch of the contraction kernel
-constexpr int ARENA = 8192;      // chain intermediates (doubles)
+constexpr int ARENA = 2048;      // chain intermediates (doubles)
 constexpr int VMAX = 1024;       // staged integrals (doubles)
 constexpr int SMAX = 64;         // bra states per cluster the survivor lists hold
 constexpr int NPMAX = 64;        // leading Γ dimension the bound weights hold
@@ -323,19 +323,20 @@ __device__ __forceinline__ void ket_bound_setup(const Pt2Args &a, const PTerm &T
   const int16_t *F = a.fock + (int64_t)fj * 2 * N;
   const int16_t *D = a.tc + (int64_t)T.group * 2 * N;
   k.nb = T.nbody;
-  for (int q = 0; q < 4; q++) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {  // constant trip count: the arrays stay in registers
     k.cl[q] = 0;
     k.np[q] = 1;
     k.cmx[q] = nullptr;
-  }
-  for (int q = 0; q < T.nbody; q++) {
-    const int c = T.cl[q];
-    const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
-    const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
-    const int blk = gblk(a, c, T.op[q], cb, ck);
-    k.cl[q] = c;
-    k.np[q] = a.gb_dims[3 * blk];
-    k.cmx[q] = a.cmx + a.cmx_off[blk];
+    if (q < T.nbody) {
+      const int c = T.cl[q];
+      const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
+      const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
+      const int blk = gblk(a, c, T.op[q], cb, ck);
+      k.cl[q] = c;
+      k.np[q] = a.gb_dims[3 * blk];
+      k.cmx[q] = a.cmx + a.cmx_off[blk];
+    }
   }
   k.d0 = T.dims[0];
   k.d1 = T.nbody > 1 ? T.dims[1] : 1;
@@ -400,20 +401,25 @@ __global__ __launch_bounds__(SW) void k_pt2_screen(Pt2Args a) {
   if (lane == 0) a.nsurv[it] = count;
 }
 
-// block-wide exclusive scan of one int per thread; returns the total
-__device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sScan, int64_t *out_excl) {
-  const int tid = threadIdx.x;
-  sScan[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < PT; off <<= 1) {
-    const int64_t w = tid >= off ? sScan[tid - off] : 0;
-    __syncthreads();
-    sScan[tid] += w;
-    __syncthreads();
+// block-wide exclusive scan of one value per thread (wavefront shuffles, then
+// the wavefront totals through LDS); returns the total.  Two barriers.
+__device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sWave, int64_t *out_excl) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
   }
-  *out_excl = sScan[tid] - v;
-  const int64_t tot = sScan[PT - 1];
+  if (lane == 63) sWave[w] = x;
   __syncthreads();
+  int64_t pre = 0, tot = 0;
+  for (int k = 0; k < PT / 64; k++) {
+    const int64_t t = sWave[k];
+    if (k < w) pre += t;
+    tot += t;
+  }
+  *out_excl = pre + x - v;
+  __syncthreads();  // sWave is free again
   return tot;
 }
 
@@ -423,6 +429,12 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sScan, int64_t
 // emit_ptr[slot] - emit_base on, in the order (candidate batch, group in
 // active-state order, tile, output element, ket in |c| order), the same in
 // both passes.
+//
+// The chain contracts the last active cluster first.  Its intermediates are
+// tiled over the surviving states of the last cluster (T = 1) or, when a
+// 4-body term's stage after the second cluster would not fit, over the
+// pairs of surviving states of the last two clusters (T = 2), so that every
+// stored stage fits half the arena and several workgroups share a CU.
 template <bool FILL>
 __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ double sV[VMAX];
@@ -434,13 +446,15 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ int sNS[4];
   __shared__ int32_t sBKet[KCH];      // kept kets of the batch, in candidate order
   __shared__ double sBCab[KCH];       // their max_r |c|
+  __shared__ double sBThr[KCH];       // their element thresholds
+  __shared__ uint64_t sBLo[KCH];      // their state keys with the active clusters cleared
   __shared__ uint32_t sKey[KCH];      // active-state keys, sorted with sPos
   __shared__ int16_t sPos[KCH];
   __shared__ int sGrp[KCH + 1];
   __shared__ int sNG, sDone;
-  __shared__ int64_t sScan[PT];
+  __shared__ int64_t sWave[PT / 64];
   __shared__ uint64_t sFockKey;
-  __shared__ int sBlk[4], sNp[4], sMb[4], sMk[4];
+  __shared__ int sBlk[4], sNp[4], sMb[4], sMk[4], sCl[4], sDims[4];
   __shared__ double sSign;
   __shared__ int64_t sEmit;  // kept contributions so far (count) / next position (fill)
   __shared__ int sSkip;
@@ -461,29 +475,33 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       fk |= (uint64_t)cx << (a.fbits * c);
     }
     sFockKey = fk;
-    int par = 0;
-    for (int q = 0; q < nb; q++) {
-      const int c = T.cl[q];
-      const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
-      const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
-      const int blk = gblk(a, c, T.op[q], cb, ck);
-      sBlk[q] = blk;
-      sNp[q] = a.gb_dims[3 * blk];
-      sMb[q] = a.gb_dims[3 * blk + 1];
-      sMk[q] = a.gb_dims[3 * blk + 2];
-      if ((T.odd >> q) & 1) {
-        int ne = 0;
-        for (int c2 = 0; c2 < c; c2++) ne += F[2 * c2] + F[2 * c2 + 1];
-        par ^= ne & 1;
+    int par = 0, nv = 1, skip = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {  // constant trip count: the term stays in registers
+      sCl[q] = q < nb ? T.cl[q] : 0;
+      sDims[q] = q < nb ? T.dims[q] : 1;
+      if (q < nb) {
+        const int c = T.cl[q];
+        const int ck = sector_code(F[2 * c], F[2 * c + 1], S);
+        const int cb = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
+        const int blk = gblk(a, c, T.op[q], cb, ck);
+        sBlk[q] = blk;
+        sNp[q] = a.gb_dims[3 * blk];
+        sMb[q] = a.gb_dims[3 * blk + 1];
+        sMk[q] = a.gb_dims[3 * blk + 2];
+        if ((T.odd >> q) & 1) {
+          int ne = 0;
+          for (int c2 = 0; c2 < c; c2++) ne += F[2 * c2] + F[2 * c2 + 1];
+          par ^= ne & 1;
+        }
+        if (nb > 1) nv *= T.dims[q];
+        // shapes the staged chain cannot hold are reported, not computed
+        skip |= sMb[q] > SMAX || sNp[q] > NPMAX || sMk[q] > 255;
       }
     }
     sSign = par ? -1.0 : 1.0;
     sEmit = FILL ? a.emit_ptr[slot] - a.emit_base : 0;
-    // shapes the staged chain cannot hold are reported, not computed
-    int nv = 1;
-    for (int q = 0; q < nb; q++) nv *= (nb == 1 ? 1 : T.dims[q]);
-    int skip = nv > VMAX;
-    for (int q = 0; q < nb; q++) skip |= sMb[q] > SMAX || sNp[q] > NPMAX || sMk[q] > 255;
+    skip |= nv > VMAX;
     sSkip = skip;
     if (skip && !FILL) atomicAdd(&a.stats[2], 1ull);
   }
@@ -495,7 +513,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   const double sign = sSign;
   int nv = 1;
   if (nb > 1)
-    for (int q = 0; q < nb; q++) nv *= T.dims[q];
+    for (int q = 0; q < nb; q++) nv *= sDims[q];
   for (int e = tid; e < nv; e += PT) sV[e] = nb > 1 ? a.ints[T.ioff + e] : 1.0;
   KetBound kb;
   ket_bound_setup(a, T, fj, kb);
@@ -521,13 +539,22 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
     }
     if (tid == PT - 1) sDone = !cand;  // kets are ordered by |c|
     int64_t pos;
-    const int nbat = (int)block_scan(pass, sScan, &pos);
+    const int nbat = (int)block_scan(pass, sWave, &pos);
     if (pass) {
       sBKet[pos] = J;
       sBCab[pos] = cj;
+      sBThr[pos] = a.thr[J];
       const uint8_t *st = a.cst + (int64_t)J * N;
       uint32_t key = 0;  // states below 255 (checked on the host): never the sentinel
-      for (int q = 0; q < nb; q++) key |= (uint32_t)st[T.cl[q]] << (8 * q);
+      uint64_t lo = 0;
+      for (int cc = 0; cc < N; cc++) lo |= (uint64_t)st[cc] << (a.sbits * cc);
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (q < nb) {
+          key |= (uint32_t)st[sCl[q]] << (8 * q);
+          lo &= ~(((uint64_t(1) << a.sbits) - 1) << (a.sbits * sCl[q]));
+        }
+      sBLo[pos] = lo;
       sKey[pos] = key;
       sPos[pos] = (int16_t)pos;
     }
@@ -583,7 +610,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       for (int e = tid; e < 4 * NPMAX; e += PT) {
         const int q = e / NPMAX, p = e % NPMAX;
         if (q < nb && p < sNp[q])
-          sCM[q][p] = a.cmx[a.cmx_off[sBlk[q]] + (int64_t)sNp[q] * st0[T.cl[q]] + p];
+          sCM[q][p] = a.cmx[a.cmx_off[sBlk[q]] + (int64_t)sNp[q] * st0[sCl[q]] + p];
       }
       __syncthreads();
       // per-cluster weights w_q[p_q] = sum over the other indices of
@@ -592,29 +619,33 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
         for (int e = tid; e < 4 * NPMAX; e += PT) {
           const int q = e / NPMAX, pq = e % NPMAX;
           if (q >= nb || pq >= sNp[q]) continue;
-          int d[4], stride[4], idx[4];
-          int sacc = 1;
+          // the other three indices (unused ones of extent 1) as an odometer
+          int dA = 1, dB = 1, dC = 1, sA_ = 0, sB_ = 0, sC_ = 0, rA = 0, rB = 0, rC = 0;
+          int sq = 1, acc = 1, n_other = 0;
+#pragma unroll
           for (int r = 0; r < 4; r++) {
-            d[r] = r < nb ? T.dims[r] : 1;
-            stride[r] = sacc;
-            sacc *= d[r];
-            idx[r] = 0;
-          }
-          idx[q] = pq;
-          const int total = nv / d[q];
-          double s = 0.0;
-          for (int v = 0; v < total; v++) {
-            int off = 0;
-            double f = 1.0;
-            for (int r = 0; r < nb; r++) {
-              off += idx[r] * stride[r];
-              if (r != q) f *= sCM[r][idx[r]];
+            const int dr = r < nb ? sDims[r] : 1;
+            if (r == q) {
+              sq = acc;
+            } else {
+              if (n_other == 0) { dA = dr; sA_ = acc; rA = r; }
+              else if (n_other == 1) { dB = dr; sB_ = acc; rB = r; }
+              else { dC = dr; sC_ = acc; rC = r; }
+              n_other++;
             }
-            s += fabs(sV[off]) * f;
-            for (int r = 0; r < nb; r++) {  // next combination of the other indices
-              if (r == q) continue;
-              if (++idx[r] < d[r]) break;
-              idx[r] = 0;
+            acc *= dr;
+          }
+          const double *cA = sCM[rA], *cB = sCM[rB], *cC = sCM[rC];
+          const bool hB = nb > 2, hC = nb > 3;
+          double s = 0.0;
+          for (int iC = 0; iC < dC; iC++) {
+            const double fC = hC ? cC[iC] : 1.0;
+            for (int iB = 0; iB < dB; iB++) {
+              const double fB = fC * (hB ? cB[iB] : 1.0);
+              const int oB = pq * sq + iB * sB_ + iC * sC_;
+              double t = 0.0;
+              for (int iA = 0; iA < dA; iA++) t += fabs(sV[oB + iA * sA_]) * cA[iA];
+              s += t * fB;
             }
           }
           sW[q][pq] = s;
@@ -627,7 +658,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       {
         const int q = tid / SMAX, I = tid % SMAX;
         if (q < nb && I < sMb[q]) {
-          const int Jq = st0[T.cl[q]];
+          const int Jq = st0[sCl[q]];
           const int np = sNp[q];
           const double *Gi = a.gdata + a.gb_off[sBlk[q]] + (int64_t)np * (I + (int64_t)sMb[q] * Jq);
           double s = 0.0;
@@ -646,50 +677,60 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       __syncthreads();
       bool empty = false;
       for (int q = 0; q < nb; q++) empty |= sNS[q] == 0;
-      if (empty) continue;  // uniform; the next group starts with a barrier-free read
+      if (empty) continue;  // uniform; nothing below the last barrier is rewritten first
       groups_kept++;
-      // chain: contract the last active cluster first, tiled over its
-      // surviving states so every stored stage fits half the arena.  Stored
-      // stage sizes per tile state: after the last cluster prod_{q<last} d_q,
-      // after cluster q (last-1 .. 1) prod_{r<q} d_r * prod_{r=q}^{last-1} |S_r|;
+      // tiling (see above).  Stored stage sizes per tile state: after the
+      // last cluster pre_full = prod_{q<last} d_q; with T = 2 after cluster
+      // last-1 pre_full / d_{last-1}; after an untiled cluster q (the
+      // highest untiled down to 1) prod_{r<q} d_r * prod_{r=q}^{hi} |S_r|;
       // cluster 0 is contracted per output during the emission
       const int last = nb - 1;
-      int need = 1;
+      int pre_full = 1;
+      for (int q = 0; q < last; q++) pre_full *= nb > 1 ? sDims[q] : 1;
+      int TT = 1, need = 1;
       if (nb > 1) {
-        int pre = 1;
-        for (int q = 0; q < last; q++) pre *= T.dims[q];
-        need = pre;
-        int ps = 1;
-        for (int q = last - 1; q >= 1; q--) {
-          pre /= T.dims[q];
-          ps *= sNS[q];
-          need = max(need, pre * ps);
+        for (TT = 1; TT <= 2; TT++) {
+          if (TT == 2 && nb < 3) break;
+          int pre = pre_full;
+          need = pre;
+          if (TT == 2) {
+            pre /= sDims[last - 1];
+            need = max(need, pre);
+          }
+          int ps = 1;
+          for (int q = last - TT; q >= 1; q--) {
+            pre /= sDims[q];
+            ps *= sNS[q];
+            need = max(need, pre * ps);
+          }
+          if (need <= ARENA / 2) break;
         }
       }
-      if (need > ARENA / 2) {  // uniform: flags the call, computes nothing wrong
+      if (need > ARENA / 2 || TT > 2) {  // uniform: flags the call, computes nothing wrong
         if (tid == 0 && !FILL) atomicAdd(&a.stats[2], 1ull);
         __syncthreads();
         continue;
       }
-      const int ntile = nb == 1 ? 1 : sNS[last];
+      const int nsA = nb > 1 && TT == 2 ? sNS[last - 1] : 1;  // combination m = iA + nsA * iL
+      const int ntile = nb == 1 ? 1 : sNS[last] * nsA;
       const int tk = nb == 1 ? 1 : min(ntile, (ARENA / 2) / need);
-      const int J0 = st0[T.cl[0]];
+      const int J0 = st0[sCl[0]];
       const double *G0 = a.gdata + a.gb_off[sBlk[0]];
       for (int t0 = 0; t0 < ntile; t0 += tk) {
         const int tn = nb == 1 ? 1 : min(tk, ntile - t0);
         double *cur = sA, *nxt = sA + ARENA / 2;
         int cols = 1;
         if (nb > 1) {
-          int pre = 1;
-          for (int q = 0; q < last; q++) pre *= T.dims[q];
-          {  // the last cluster: cur[r + pre * i] = sum_p V[r + pre * p] Γ[p, S[t0+i], J]
+          int pre = pre_full;
+          {  // the last cluster: cur[r + pre * i] = sum_p V[r + pre * p] Γ[p, S[m_L], J]
             const int q = last;
             const int np = sNp[q], mb = sMb[q];
-            const int Jq = st0[T.cl[q]];
+            const int Jq = st0[sCl[q]];
             const double *G = a.gdata + a.gb_off[sBlk[q]];
             for (int e = tid; e < pre * tn; e += PT) {
               const int r = e % pre, i = e / pre;
-              const double *Gi = G + (int64_t)np * (sSurv[q][t0 + i] + (int64_t)mb * Jq);
+              const int iL = (t0 + i) / nsA;
+              const double *Gi = G + (int64_t)np * (sSurv[q][iL] + (int64_t)mb * Jq);
               double v = 0.0;
               for (int p = 0; p < np; p++) v += sV[r + pre * p] * Gi[p];
               cur[e] = v;
@@ -698,10 +739,31 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
           }
           cols = tn;
           __syncthreads();
-          for (int q = last - 1; q >= 1; q--) {
+          if (TT == 2) {  // cluster last-1 at the tile's own state: no expansion
+            const int q = last - 1;
+            const int np = sNp[q], mb = sMb[q];
+            const int Jq = st0[sCl[q]];
+            const double *G = a.gdata + a.gb_off[sBlk[q]];
+            const int pre2 = pre / np;
+            for (int e = tid; e < pre2 * tn; e += PT) {
+              const int r2 = e % pre2, i = e / pre2;
+              const int iA = (t0 + i) % nsA;
+              const double *Gi = G + (int64_t)np * (sSurv[q][iA] + (int64_t)mb * Jq);
+              double v = 0.0;
+              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * i] * Gi[p];
+              nxt[e] = v;
+            }
+            flops += 2ull * pre2 * tn * np;
+            __syncthreads();
+            double *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            pre = pre2;
+          }
+          for (int q = last - TT; q >= 1; q--) {
             // nxt[r2 + pre2 * (i + ns * col)] = sum_p cur[r2 + pre2 * p + pre * col] Γ_q[p, S_q[i], J]
             const int np = sNp[q], mb = sMb[q];
-            const int Jq = st0[T.cl[q]];
+            const int Jq = st0[sCl[q]];
             const double *G = a.gdata + a.gb_off[sBlk[q]];
             const int pre2 = pre / np, ns = sNS[q];
             for (int e = tid; e < pre2 * ns * cols; e += PT) {
@@ -726,9 +788,9 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
         // output e = i0 + ns0 * col, col = i_1 + |S_1| (i_2 + ... (tile index));
         // kept contributions counted (and, filling, written in this order)
         const int64_t base = sEmit;  // read before the scan's barriers
+        int64_t excl0 = 0;           // this thread's first position (pass 1)
         for (int pass2 = 0; pass2 < (FILL ? 2 : 1); pass2++) {
-          int64_t wpos = 0, cnt = 0;
-          if (pass2 == 1) wpos = base + sScan[tid];  // exclusive prefix of pass 0
+          int64_t wpos = base + excl0, cnt = 0;
           for (int e = tid; e < nout; e += PT) {
             const int i0 = e % ns0, col = e / ns0;
             double x;
@@ -742,43 +804,37 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
               x = v;
             }
             const double ax = fabs(x);
-            int bstate[4] = {sSurv[0][i0], 0, 0, 0};
+            if (pass2 == 0) {
+              for (int j = j0; j < j1; j++) cnt += ax > sBThr[sPos[j]];
+              continue;
+            }
+            // key: target Fock sectors | bra states on the active clusters,
+            // the ket's states elsewhere
+            uint64_t act = (uint64_t)sSurv[0][i0] << (a.sbits * sCl[0]);
             if (nb > 1) {
               int rem = col;
-              for (int q = 1; q < last; q++) {
-                bstate[q] = sSurv[q][rem % sNS[q]];
+              for (int q = 1; q <= last - TT; q++) {
+                act |= (uint64_t)sSurv[q][rem % sNS[q]] << (a.sbits * sCl[q]);
                 rem /= sNS[q];
               }
-              bstate[last] = sSurv[last][t0 + rem];
+              const int m = t0 + rem;
+              act |= (uint64_t)sSurv[last][m / nsA] << (a.sbits * sCl[last]);
+              if (TT == 2) act |= (uint64_t)sSurv[last - 1][m % nsA] << (a.sbits * sCl[last - 1]);
             }
+            const double v = sign * x;
             for (int j = j0; j < j1; j++) {
-              const int64_t Jm = sBKet[sPos[j]];
-              if (!(ax > a.thr[Jm])) continue;  // _collect_significant_thread!'s test
-              if (pass2 == 1) {
-                // key: target Fock sectors | bra states on the active
-                // clusters, the ket's states elsewhere
-                const uint8_t *sj = a.cst + Jm * N;
-                uint64_t lo = 0;
-                for (int cc = 0; cc < N; cc++) {
-                  int sv = sj[cc];
-                  for (int q = 0; q < nb; q++)
-                    if (T.cl[q] == cc) sv = bstate[q];
-                  lo |= (uint64_t)sv << (a.sbits * cc);
-                }
-                a.ekey[wpos] = Key128{sFockKey, lo};
-                const double v = sign * x;
-                for (int r = 0; r < a.R; r++)
-                  a.eval[wpos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + Jm];
-                wpos++;
-              }
-              cnt++;
+              const int pj = sPos[j];
+              if (!(ax > sBThr[pj])) continue;  // _collect_significant_thread!'s test
+              const int64_t Jm = sBKet[pj];
+              a.ekey[wpos] = Key128{sFockKey, sBLo[pj] | act};
+              for (int r = 0; r < a.R; r++)
+                a.eval[wpos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + Jm];
+              wpos++;
             }
           }
           if (pass2 == 0) {
-            int64_t excl;
-            const int64_t tot = block_scan(cnt, sScan, &excl);
-            // block_scan ends with a barrier: every thread has read sEmit
-            if (FILL) sScan[tid] = excl;  // own slot only; read back by this thread in pass 1
+            const int64_t tot = block_scan(cnt, sWave, &excl0);
+            // block_scan's barriers: every thread has read sEmit
             if (tid == 0) sEmit = base + tot;
           }
         }

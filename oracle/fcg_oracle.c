@@ -10,7 +10,8 @@
  *   compute_diagonal           src/tpsci_outer.jl:694-715
  *   FockConfig - TransferConfig  src/Indexing.jl:21-23
  *   compute_pt2_energy         src/tpsci_pt2_energy.jl:10-175, _pt2_job :178-266,
- *                              _sum_pt2 :269-282
+ *                              _sum_pt2 :269-282 (dense target blocks, and
+ *                              over the configurations reached)
  *   contract_matvec_thread     src/tpsci_matvec_thread.jl:279-820 (prescreen=false)
  *   _collect_significant_thread!  src/tpsci_matvec_thread.jl:830-895
  *   project_out!               src/tpsci_outer.jl:869-889
@@ -325,10 +326,79 @@ static int64_t fock_intern(int64_t *slot, int64_t cap, int16_t *keys, int64_t *n
  * block.  Returns -3 when a 1-body Γ block is missing (the reference reads
  * it unconditionally); multi-body terms with a missing block are skipped
  * (their `haskey(...) || return`). */
+/* configuration -> sigma[R] map of one target Fock block, keyed by the
+ * configuration's linear index in the block (first cluster fastest);
+ * open addressing, insertion order kept in `order` */
+typedef struct {
+  int64_t cap, n, R;
+  int64_t *key;   /* [cap], -1 = empty */
+  int64_t *idx;   /* [cap] entry of the slot */
+  int64_t *order; /* [n] keys by insertion */
+  double *val;    /* [n][R] */
+} lin_map;
+
+static void lin_map_init(lin_map *m, int R) {
+  m->cap = 1024;
+  m->n = 0;
+  m->R = R;
+  m->key = (int64_t *)malloc(sizeof(int64_t) * m->cap);
+  m->idx = (int64_t *)malloc(sizeof(int64_t) * m->cap);
+  for (int64_t i = 0; i < m->cap; i++) m->key[i] = -1;
+  m->order = (int64_t *)malloc(sizeof(int64_t) * (m->cap / 2));
+  m->val = (double *)malloc(sizeof(double) * (m->cap / 2) * R);
+}
+
+static void lin_map_free(lin_map *m) {
+  free(m->key);
+  free(m->idx);
+  free(m->order);
+  free(m->val);
+}
+
+static int64_t lin_map_slot(const lin_map *m, int64_t k) {
+  uint64_t hsh = ((uint64_t)k * 0x9E3779B97F4A7C15ull) >> 17;
+  int64_t i = (int64_t)(hsh & (uint64_t)(m->cap - 1));
+  while (m->key[i] >= 0 && m->key[i] != k) i = (i + 1) & (m->cap - 1);
+  return i;
+}
+
+/* the sigma entry of configuration k, inserted as zeros when absent */
+static double *lin_map_get(lin_map *m, int64_t k) {
+  int64_t i = lin_map_slot(m, k);
+  if (m->key[i] == k) return m->val + m->idx[i] * m->R;
+  if (2 * (m->n + 1) > m->cap) {
+    const int64_t old = m->cap;
+    int64_t *ok = m->key, *oi = m->idx;
+    m->cap *= 2;
+    m->key = (int64_t *)malloc(sizeof(int64_t) * m->cap);
+    m->idx = (int64_t *)malloc(sizeof(int64_t) * m->cap);
+    for (int64_t j = 0; j < m->cap; j++) m->key[j] = -1;
+    for (int64_t j = 0; j < old; j++)
+      if (ok[j] >= 0) {
+        const int64_t s = lin_map_slot(m, ok[j]);
+        m->key[s] = ok[j];
+        m->idx[s] = oi[j];
+      }
+    free(ok);
+    free(oi);
+    m->order = (int64_t *)realloc(m->order, sizeof(int64_t) * (m->cap / 2));
+    m->val = (double *)realloc(m->val, sizeof(double) * (m->cap / 2) * m->R);
+    i = lin_map_slot(m, k);
+  }
+  m->key[i] = k;
+  m->idx[i] = m->n;
+  m->order[m->n] = k;
+  double *v = m->val + m->n * m->R;
+  for (int64_t r = 0; r < m->R; r++) v[r] = 0.0;
+  m->n++;
+  return v;
+}
+
 static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx,
                     const int16_t *fk, const int16_t *ket, const double *cket /* R */, int R,
                     double thresh, const int64_t *stride, int64_t nx, double *sig,
                     unsigned char *hit /* nullable: marks inserted configurations */,
+                    lin_map *hm /* non-null: accumulate here instead of sig */,
                     double **w1, double **w2, int64_t *wcap) {
   const int N = o->n_clusters;
   const int k = h->nbody[t];
@@ -409,7 +479,12 @@ static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx
       lin += (r0 % mb[a]) * stride[cl[a]];
       r0 /= mb[a];
     }
-    for (int r = 0; r < R; r++) sig[lin + nx * r] += sgn * v * cket[r];
+    if (hm) {
+      double *acc = lin_map_get(hm, lin);
+      for (int r = 0; r < R; r++) acc[r] += sgn * v * cket[r];
+    } else {
+      for (int r = 0; r < R; r++) sig[lin + nx * r] += sgn * v * cket[r];
+    }
     if (hit) hit[lin] = 1;
   }
   return 0;
@@ -479,7 +554,7 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
           for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
             for (int r = 0; r < R; r++) cket[r] = c[(int64_t)r * sp->n_cfg + kj];
             rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, sig,
-                          NULL, &w1, &w2, &wcap);
+                          NULL, NULL, &w1, &w2, &wcap);
           }
         }
       }
@@ -540,6 +615,174 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
   return err;
 }
 
+/* compute_pt2_energy over a sparse first-order space: the jobs, terms and
+ * element test of orc_tpsci_pt2, with each target block's sigma held in a
+ * map over the configurations reached (lin_map) instead of a dense array,
+ * and ket configurations whose coefficients are all zero skipped (their
+ * threshold thresh / 0 admits no element).  So it serves target Fock blocks
+ * too large to form (many clusters) for states with few nonzero
+ * coefficients.  Per job the configurations are summed in the order they
+ * were first reached.  n_x (nullable): configurations reached outside the
+ * variational space.  Returns 0, -3 on a missing 1-body or H0 block, -4 when
+ * a target block has more than 2^62 configurations. */
+int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
+                         const double *c, const double *e0, int nbody, int h0_op, double thresh,
+                         double *e2, int64_t *n_jobs, int64_t *n_x, int nthreads) {
+  const int N = o->n_clusters, S = o->S;
+  if (N > MAXN || R < 1 || R > 64) return -1;
+  /* ket Fock blocks holding a nonzero coefficient */
+  int64_t *act = (int64_t *)malloc(sizeof(int64_t) * (sp->n_fock > 0 ? sp->n_fock : 1));
+  int64_t na = 0;
+  for (int64_t f = 0; f < sp->n_fock; f++) {
+    int nz = 0;
+    for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !nz; kj++)
+      for (int r = 0; r < R; r++) nz |= c[(int64_t)r * sp->n_cfg + kj] != 0.0;
+    if (nz) act[na++] = f;
+  }
+  /* jobs: target Fock blocks F_ket + TransferConfig (tpsci_pt2_energy.jl:49-79) */
+  const int64_t n_cand = na * (int64_t)h->n_tc;
+  int64_t cap = 16;
+  while (cap < 2 * n_cand) cap <<= 1;
+  int64_t *slot = (int64_t *)malloc(sizeof(int64_t) * cap);
+  for (int64_t i = 0; i < cap; i++) slot[i] = -1;
+  int16_t *jfock = (int16_t *)malloc(sizeof(int16_t) * 2 * N * (n_cand > 0 ? n_cand : 1));
+  int64_t *cand_job = (int64_t *)malloc(sizeof(int64_t) * (n_cand > 0 ? n_cand : 1));
+  int64_t nj = 0;
+  for (int64_t ai = 0; ai < na; ai++) {
+    const int16_t *fk = sp->fock + act[ai] * 2 * N;
+    for (int g = 0; g < h->n_tc; g++) {
+      int16_t fx[2 * MAXN];
+      int ok = 1;
+      for (int i = 0; i < 2 * N && ok; i++) {
+        fx[i] = fk[i] + h->tc[(int64_t)g * 2 * N + i];
+        ok = fx[i] >= 0 && fx[i] < S;
+      }
+      for (int ci = 0; ci < N && ok; ci++) {
+        const int32_t *d;
+        ok = gamma_block(o, ci, 0, fx, fx, &d) != NULL;
+      }
+      cand_job[ai * h->n_tc + g] = ok ? fock_intern(slot, cap, jfock, &nj, fx, N) : -1;
+    }
+  }
+  /* the (ket block, TransferConfig) pairs of each job, in (block, group) order */
+  int64_t *jstart = (int64_t *)calloc((size_t)nj + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < n_cand; i++)
+    if (cand_job[i] >= 0) jstart[cand_job[i] + 1]++;
+  for (int64_t j = 0; j < nj; j++) jstart[j + 1] += jstart[j];
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (nj > 0 ? nj : 1));
+  for (int64_t j = 0; j < nj; j++) fill[j] = jstart[j];
+  int64_t *pairs = (int64_t *)malloc(sizeof(int64_t) * (jstart[nj] > 0 ? jstart[nj] : 1));
+  for (int64_t i = 0; i < n_cand; i++)
+    if (cand_job[i] >= 0) pairs[fill[cand_job[i]]++] = i;
+  fock_map vm;
+  fock_map_build(&vm, sp->fock, sp->n_fock, N);
+  double *je2 = (double *)calloc((size_t)(nj > 0 ? nj : 1) * R, sizeof(double));
+  int err = 0;
+  int64_t nxs = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nxs)
+  for (int64_t j = 0; j < nj; j++) {
+    const int16_t *fx = jfock + j * 2 * N;
+    int64_t stride[MAXN], nx = 1;
+    int M[MAXN];
+    int rc = 0;
+    for (int ci = 0; ci < N; ci++) {
+      const int32_t *d = NULL;
+      if (!gamma_block(o, ci, 0, fx, fx, &d)) continue; /* excluded when the job was formed */
+      M[ci] = d[1];
+      stride[ci] = nx;
+      if (nx > ((int64_t)1 << 62) / M[ci]) rc = -4;
+      nx *= M[ci];
+    }
+    lin_map hm;
+    lin_map_init(&hm, R);
+    double *w1 = NULL, *w2 = NULL;
+    int64_t wcap = 0;
+    double cket[64];
+    /* _pt2_job: every (TransferConfig group, ket Fock block) feeding F_x */
+    for (int64_t pi = jstart[j]; pi < jstart[j + 1] && !rc; pi++) {
+      const int64_t f = act[pairs[pi] / h->n_tc];
+      const int g = (int)(pairs[pi] % h->n_tc);
+      const int16_t *fk = sp->fock + f * 2 * N;
+      for (int t = h->tc_start[g]; t < h->tc_start[g + 1] && !rc; t++) {
+        if (h->nbody[t] > nbody) continue;
+        for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
+          double cmax = 0.0;
+          for (int r = 0; r < R; r++) {
+            cket[r] = c[(int64_t)r * sp->n_cfg + kj];
+            cmax = fabs(cket[r]) > cmax ? fabs(cket[r]) : cmax;
+          }
+          if (cmax == 0.0) continue;
+          rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, NULL,
+                        NULL, &hm, &w1, &w2, &wcap);
+        }
+      }
+    }
+    /* project_out! (src/tpsci_outer.jl:869-889): drop variational configurations */
+    unsigned char *drop = (unsigned char *)calloc((size_t)(hm.n > 0 ? hm.n : 1), 1);
+    const int64_t vf = fock_map_find(&vm, fx);
+    if (vf >= 0 && !rc)
+      for (int64_t i = sp->fock_start[vf]; i < sp->fock_start[vf + 1]; i++) {
+        int64_t lin = 0;
+        for (int ci = 0; ci < N; ci++) lin += (int64_t)(sp->cfg[i * N + ci] - 1) * stride[ci];
+        const int64_t sl = lin_map_slot(&hm, lin);
+        if (hm.key[sl] == lin) drop[hm.idx[sl]] = 1;
+      }
+    /* compute_diagonal! with the H0 operator (src/tpsci_outer.jl:735-757) */
+    const double *hb[MAXN];
+    for (int ci = 0; ci < N && !rc; ci++) {
+      const int32_t *d;
+      hb[ci] = gamma_block(o, ci, h0_op, fx, fx, &d);
+      if (!hb[ci]) rc = -3;
+    }
+    /* _sum_pt2 (src/tpsci_pt2_energy.jl:269-282) */
+    if (!rc)
+      for (int64_t e = 0; e < hm.n; e++) {
+        if (drop[e]) continue;
+        nxs++;
+        double hd = 0.0;
+        int64_t rem = hm.order[e];
+        for (int ci = 0; ci < N; ci++) {
+          const int64_t i = rem % M[ci];
+          rem /= M[ci];
+          hd += hb[ci][i * (M[ci] + 1)];
+        }
+        for (int r = 0; r < R; r++) {
+          const double sv = hm.val[e * R + r];
+          je2[j * R + r] += sv * sv / (e0[r] - hd);
+        }
+      }
+    if (rc) err = rc;
+    free(drop);
+    lin_map_free(&hm);
+    free(w1);
+    free(w2);
+  }
+  for (int r = 0; r < R; r++) {
+    double s = 0.0, comp = 0.0;
+    for (int64_t j = 0; j < nj; j++) {
+      const double v = je2[j * R + r], t = s + v;
+      comp += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    e2[r] = s + comp;
+  }
+  if (n_jobs) *n_jobs = nj;
+  if (n_x) *n_x = nxs;
+  free(je2);
+  free(vm.slot);
+  free(pairs);
+  free(fill);
+  free(jstart);
+  free(cand_job);
+  free(jfock);
+  free(slot);
+  free(act);
+  return err;
+}
+
 /* Open matvec (open_matvec_thread, src/tpsci_matvec_thread.jl:11-129, with
  * prescreen=false) on caller-listed target Fock blocks: every (TransferConfig
  * group, ket Fock block) with F_ket + TransferConfig = F_x feeds block x
@@ -595,7 +838,7 @@ int orc_tpsci_open_sigma(const orc_ops *o, const orc_ham *h, const orc_space *sp
           for (int64_t kj = sp->fock_start[f]; kj < sp->fock_start[f + 1] && !rc; kj++) {
             for (int r = 0; r < R; r++) cket[r] = c[(int64_t)r * sp->n_cfg + kj];
             rc = pt2_term(o, h, t, fx, fk, sp->cfg + kj * N, cket, R, thresh, stride, nx, js,
-                          hit + off[j], &w1, &w2, &wcap);
+                          hit + off[j], NULL, &w1, &w2, &wcap);
           }
         }
       }

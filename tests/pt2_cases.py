@@ -86,3 +86,26 @@ def six_clusters_rank1(seed=21, R=2, frac=0.6):
     c.ints, c.clusters, c.ref, c.bases = p.ints, p.clusters, p.ref_fock, p.bases
     c.ops, c.H, c.state = p.cluster_ops, p.clustered_ham, st
     return c
+
+
+def twelve_clusters_rank1(M=4, nnz=6, seed=4, frac=0.05):
+    """The 48-orbital, 12-cluster problem (clusters of 4 at half filling,
+    lowest M states per sector, Fock blocks within 2 transferred electrons)
+    with a hash-selected rank <= 1 variational space and a state whose
+    coefficients are zero except on the reference configuration (1) and
+    nnz - 1 seeded configurations (N(0, 0.05)), normalised: the shape of the
+    12-cluster first-order workload, small enough for the oracle."""
+    from fermicg_amd.workloads import build_problem, rank1_space
+    p = build_problem(48, 12, M, 96, seed, max_transfer=2)
+    st = rank1_space(p, R=1, frac=frac, seed=seed)
+    ref_row = int(st.fock_start[p.focks.index(p.ref_fock)])
+    rng = np.random.default_rng(seed + 9)
+    v = np.zeros((len(st), 1))
+    v[ref_row, 0] = 1.0
+    pick = rng.choice(len(st), size=nnz - 1, replace=False)
+    v[pick, 0] += rng.normal(size=nnz - 1) * 0.05
+    st.set_vector(v / np.linalg.norm(v))
+    c = Case()
+    c.ints, c.clusters, c.ref, c.bases = p.ints, p.clusters, p.ref_fock, p.bases
+    c.ops, c.H, c.state = p.cluster_ops, p.clustered_ham, st
+    return c

@@ -92,3 +92,23 @@ def test_sparse_pt2_chunked_accumulation(eng, six, chunk, monkeypatch):
         assert st[k] == st0[k], k
     assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
     assert np.abs(e2 - whole).max() < 1e-13 * np.abs(whole).max()
+
+
+@pytest.mark.parametrize("M,nnz,thresh", [(4, 6, 1e-3), (4, 6, 1e-5), (8, 10, 1e-5)])
+def test_sparse_pt2_twelve_clusters(eng, M, nnz, thresh):
+    """The 12-cluster, 48-orbital shape against the oracle's restatement over
+    the configurations reached (orc_tpsci_pt2_sparse, pinned to the dense
+    target-block restatement on the small cases): e2 to 1e-11 relative and
+    the number of first-order configurations exactly (1.2e8 at M=8)."""
+    from tests.pt2_cases import twelve_clusters_rank1
+    case = twelve_clusters_rank1(M=M, nnz=nnz)
+    st = case.state
+    e0 = np.array([-30000.0])   # below every H0 diagonal: no small denominators
+    fo = flatten_ops(case.ops, [len(c) for c in case.clusters])
+    ref, _, nx = Oracle(fo, flatten_ham(case.H, 4), st).pt2_sparse(st.coeffs, e0, thresh=thresh)
+    e2 = compute_pt2_energy(st, case.ops, case.H, E0=e0, thresh_foi=thresh, verbose=0,
+                            method="sparse", engine=eng)
+    stats = compute_pt2_energy.last_stats
+    print(M, nnz, thresh, e2, ref, nx, stats, flush=True)
+    assert stats["n_first_order"] == nx
+    assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()

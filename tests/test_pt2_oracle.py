@@ -17,7 +17,7 @@ import pytest
 
 from fermicg_amd.marshal import flatten_ham, flatten_ops
 from fermicg_amd.operator import extract_1body_operator
-from fermicg_amd.state import TPSCIstate
+from fermicg_amd.state import TPSCIstate, copy_state
 from oracle.pyoracle import Oracle
 from tests.pt2_cases import four_clusters, three_clusters_m4
 
@@ -96,3 +96,37 @@ def test_pt2_invariants(oracle_lib):
     v = np.random.Generator(np.random.PCG64(1)).normal(size=(len(full), 1))
     v /= np.linalg.norm(v)
     assert _orc(case, full).pt2(v, [-1.0])[0][0] == 0.0
+
+
+def _with_zeros(st, frac, seed):
+    """The state with a seeded fraction of its configurations zeroed (the
+    reference configuration of each root kept)."""
+    rng = np.random.default_rng(seed)
+    c = st.coeffs.copy()
+    zero = rng.random(len(c)) < frac
+    zero[np.abs(c).argmax(axis=0)] = False
+    c[zero] = 0.0
+    out = copy_state(st)
+    out.coeffs = c
+    return out
+
+
+@pytest.mark.parametrize("name,nbody,thresh,zeros", [
+    ("four", 4, 0.0, 0.0), ("four", 2, 1e-4, 0.5), ("three", 4, 0.0, 0.7),
+    ("six", 4, 1e-5, 0.0), ("six", 4, 1e-3, 0.8)])
+def test_pt2_sparse_restatement_matches_dense_targets(oracle_lib, name, nbody, thresh, zeros):
+    """orc_tpsci_pt2_sparse (a map over the configurations reached, zero
+    kets skipped) against orc_tpsci_pt2 (dense target blocks): the same
+    jobs, terms and element test, so equal to rounding."""
+    from tests.pt2_cases import six_clusters_rank1
+    case = {"four": four_clusters, "three": three_clusters_m4, "six": six_clusters_rank1}[name]()
+    st = case.state
+    if zeros:
+        st = _with_zeros(st, zeros, 3)
+    e0 = _e0(case, st) if name != "six" else np.full(st.R, -30.0)
+    orc = _orc(case, st, nbody=nbody)
+    ref, nj = orc.pt2(st.coeffs, e0, nbody=nbody, thresh=thresh)
+    e2, nj2, nx = orc.pt2_sparse(st.coeffs, e0, nbody=nbody, thresh=thresh)
+    print(name, thresh, zeros, e2, ref, nj, nj2, nx)
+    assert nx > 0 and nj2 <= nj
+    assert np.abs(e2 - ref).max() < 1e-12 * np.abs(ref).max()
