@@ -102,6 +102,18 @@ struct KeyDecomposer {
   }
 };
 
+// Sort key of a contribution: K = tloc * 2^sN + states, tloc the index of
+// its target Fock block among the chunk's targets, sN = sbits * N in [1, 64].
+__host__ __device__ __forceinline__ Key128 chunk_key(uint64_t tloc, uint64_t states, int sN) {
+  return Key128{sN == 64 ? tloc : tloc >> (64 - sN), sN == 64 ? states : states | (tloc << sN)};
+}
+__host__ __device__ __forceinline__ uint64_t chunk_key_states(const Key128 &k, int sN) {
+  return sN == 64 ? k.lo : k.lo & ((uint64_t(1) << sN) - 1);
+}
+__host__ __device__ __forceinline__ uint64_t chunk_key_target(const Key128 &k, int sN) {
+  return sN == 64 ? k.hi : (k.hi << (64 - sN)) | (k.lo >> sN);
+}
+
 struct Pt2Args {
   int N, S, R, nbody, fbits, sbits;
   double thresh;
@@ -144,6 +156,8 @@ struct Pt2Args {
   int64_t item_base, n_items;
   int32_t *nsurv;             // screening: kept kets per item
   const uint32_t *slot_item;  // contraction slot -> item (items ordered by target)
+  const uint32_t *slot_tloc;  // contraction slot -> its target's index in its chunk
+  int sN;                     // state bits of a key (sbits * N)
   int64_t slot_base, n_slots;
   int64_t *emit_count;        // count pass: kept contributions per slot
   const int64_t *emit_ptr;    // fill pass: first position of each slot
@@ -453,7 +467,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ int sGrp[KCH + 1];
   __shared__ int sNG, sDone;
   __shared__ int64_t sWave[PT / 64];
-  __shared__ uint64_t sFockKey;
+  __shared__ uint64_t sTloc;
   __shared__ int sBlk[4], sNp[4], sMb[4], sMk[4], sCl[4], sDims[4];
   __shared__ double sSign;
   __shared__ int64_t sEmit;  // kept contributions so far (count) / next position (fill)
@@ -469,12 +483,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   const int16_t *F = a.fock + (int64_t)fj * 2 * N;
   const int16_t *D = a.tc + (int64_t)T.group * 2 * N;
   if (tid == 0) {
-    uint64_t fk = 0;
-    for (int c = 0; c < N; c++) {
-      const int cx = sector_code(F[2 * c] + D[2 * c], F[2 * c + 1] + D[2 * c + 1], S);
-      fk |= (uint64_t)cx << (a.fbits * c);
-    }
-    sFockKey = fk;
+    sTloc = FILL ? a.slot_tloc[slot] : 0;
     int par = 0, nv = 1, skip = 0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {  // constant trip count: the term stays in registers
@@ -826,7 +835,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
               const int pj = sPos[j];
               if (!(ax > sBThr[pj])) continue;  // _collect_significant_thread!'s test
               const int64_t Jm = sBKet[pj];
-              a.ekey[wpos] = Key128{sFockKey, sBLo[pj] | act};
+              a.ekey[wpos] = chunk_key(sTloc, sBLo[pj] | act, a.sN);
               for (int r = 0; r < a.R; r++)
                 a.eval[wpos * a.R + r] = v * a.c[(int64_t)r * a.n_cfg + Jm];
               wpos++;
@@ -904,10 +913,12 @@ __device__ __forceinline__ bool key_less(const Key128 &a, const Key128 &b) {
 }
 
 struct SumArgs {
-  int N, S, R, n_ops, h0_op, fbits, sbits;
+  int N, S, R, n_ops, h0_op, fbits, sbits, sN;
   int64_t n_seg;
   const int64_t *seg_start;  // [n_seg + 1]
-  const uint64_t *perm;      // sorted position -> emission index
+  const uint64_t *tkeys;     // the chunk's target Fock keys by index
+  const double *sval;        // one root: values in sorted order (perm unused)
+  const uint64_t *perm;      // several roots: sorted position -> emission index
   const Key128 *skey;        // sorted keys
   const double *eval;        // [E][R] in emission order
   const Key128 *vkey;        // variational configurations, sorted
@@ -935,7 +946,8 @@ __global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n_seg;
        s += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = a.seg_start[s], e = a.seg_start[s + 1];
-    const Key128 k = a.skey[b];
+    const Key128 kc = a.skey[b];
+    const Key128 k{a.tkeys[chunk_key_target(kc, a.sN)], chunk_key_states(kc, a.sN)};
     // variational configurations are projected out
     int64_t lo = 0, hi = a.n_v;
     while (lo < hi) {
@@ -956,7 +968,10 @@ __global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
     }
     for (int r = 0; r < R; r++) {
       double v = 0.0;
-      for (int64_t i = b; i < e; i++) v += a.eval[a.perm[i] * R + r];
+      if (a.sval)
+        for (int64_t i = b; i < e; i++) v += a.sval[i];
+      else
+        for (int64_t i = b; i < e; i++) v += a.eval[a.perm[i] * R + r];
       neumaier(sum[r], comp[r], v * v / (a.e0[r] - hd));
     }
   }
@@ -1436,22 +1451,54 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   int64_t maxE = 0;
   for (int ch = 0; ch < n_chunks; ch++) maxE = std::max(maxE, eptr[cut[ch + 1]] - eptr[cut[ch]]);
   if ((rc = d_eptr.upload(eptr.data(), n_slots + 1))) return rc;
+  // each chunk's targets, and every slot's index among its chunk's targets
+  // (the sort key holds that index, not the Fock sectors)
+  std::vector<int64_t> tstart{0};
+  std::vector<uint64_t> tkeys;
+  std::vector<uint32_t> tloc(std::max<int64_t>(n_slots, 1));
+  for (int ch = 0; ch < n_chunks; ch++) {
+    uint32_t n = 0;
+    for (int64_t i = cut[ch]; i < cut[ch + 1]; i++) {
+      if (i == cut[ch] || slot_key[i] != slot_key[i - 1]) {
+        tkeys.push_back(slot_key[i]);
+        n++;
+      }
+      tloc[i] = n - 1;
+    }
+    tstart.push_back((int64_t)tkeys.size());
+  }
+  Buf<uint32_t> d_tloc(ctx);
+  Buf<uint64_t> d_tkeys(ctx);
+  if ((rc = d_tloc.upload(tloc.data(), n_slots))) return rc;
+  if ((rc = d_tkeys.upload(tkeys.data(), (int64_t)tkeys.size()))) return rc;
+  const int sN = sbits * N;
+  a.slot_tloc = d_tloc.p;
+  a.sN = sN;
+  // one root: values sorted with their keys; several: an index permutation
+  const bool direct = R == 1;
   Buf<Key128> d_ekey(ctx), d_skey(ctx);
-  Buf<double> d_eval(ctx);
+  Buf<double> d_eval(ctx), d_sval(ctx);
   Buf<uint64_t> d_idx(ctx), d_perm(ctx), d_segid(ctx);
   Buf<uint32_t> d_head(ctx);
   Buf<int64_t> d_seg(ctx);
   if (!d_ekey.alloc(maxE) || !d_eval.alloc(maxE * R) || !d_skey.alloc(maxE) ||
-      !d_idx.alloc(maxE) || !d_perm.alloc(maxE) || !d_head.alloc(maxE) || !d_segid.alloc(maxE) ||
+      !d_idx.alloc(direct ? 0 : maxE) || !d_perm.alloc(direct ? 0 : maxE) ||
+      !d_sval.alloc(direct ? maxE : 0) || !d_head.alloc(maxE) || !d_segid.alloc(maxE) ||
       !d_seg.alloc(maxE + 1)) {
     ctx->err = "PT2 emission buffers (" + std::to_string(maxE) + " contributions) do not fit";
     return FCG_ENOMEM;
   }
   size_t sort_bytes = 0, scan_bytes = 0;
-  FCG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
-                                    (size_t)std::max<int64_t>(maxE, 1), KeyDecomposer{}, 0u,
-                                    128u, s),
-          ctx);
+  if (direct)
+    FCG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ekey.p, d_skey.p, d_eval.p,
+                                      d_sval.p, (size_t)std::max<int64_t>(maxE, 1),
+                                      KeyDecomposer{}, 0u, 128u, s),
+            ctx);
+  else
+    FCG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ekey.p, d_skey.p, d_idx.p,
+                                      d_perm.p, (size_t)std::max<int64_t>(maxE, 1),
+                                      KeyDecomposer{}, 0u, 128u, s),
+            ctx);
   FCG_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, d_head.p, d_segid.p, (uint64_t)0,
                                   (size_t)std::max<int64_t>(maxE, 1), rocprim::plus<uint64_t>(),
                                   s),
@@ -1478,8 +1525,10 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   sa.h0_op = h0_op;
   sa.fbits = fbits;
   sa.sbits = sbits;
+  sa.sN = sN;
   sa.seg_start = d_seg.p;
-  sa.perm = d_perm.p;
+  sa.perm = direct ? nullptr : d_perm.p;
+  sa.sval = direct ? d_sval.p : nullptr;
   sa.skey = d_skey.p;
   sa.eval = d_eval.p;
   sa.vkey = d_vkey.p;
@@ -1515,12 +1564,20 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
     FCG_HIP(hipStreamSynchronize(s), ctx);
     emit_s += seconds_since(tf);
     const auto ts = std::chrono::steady_clock::now();
-    // stable sort of the chunk's contributions by x
-    hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, Ec, d_idx.p);
+    // stable sort of the chunk's contributions by x over the key bits in use
+    const unsigned end_bit = (unsigned)(sN + bits_for(tstart[ch + 1] - tstart[ch]));
+    sa.tkeys = d_tkeys.p + tstart[ch];
     size_t tb = sort_bytes;
-    FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
-                                      (size_t)Ec, KeyDecomposer{}, 0u, 128u, s),
-            ctx);
+    if (direct) {
+      FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_ekey.p, d_skey.p, d_eval.p, d_sval.p,
+                                        (size_t)Ec, KeyDecomposer{}, 0u, end_bit, s),
+              ctx);
+    } else {
+      hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, s, Ec, d_idx.p);
+      FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_ekey.p, d_skey.p, d_idx.p, d_perm.p,
+                                        (size_t)Ec, KeyDecomposer{}, 0u, end_bit, s),
+              ctx);
+    }
     // segments of equal keys: heads, ids by scan, starts
     hipLaunchKernelGGL(k_heads, dim3(2048), dim3(256), 0, s, Ec, d_skey.p, d_head.p);
     size_t sb = scan_bytes;
