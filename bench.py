@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="sparse",
-                    choices=["bst", "tpsci", "sparse", "serve", "pt2"])
+                    choices=["bst", "tpsci", "sparse", "serve", "pt2", "pt2_12c"])
     ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
     ap.add_argument("--rate", type=float, default=50.0, help="serve workload: arrivals per s")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
@@ -96,6 +96,8 @@ def main():
         return serve(args)
     if args.workload == "pt2":
         return pt2(args)
+    if args.workload == "pt2_12c":
+        return pt2_12c(args)
     import torch
     from fermicg_amd import _lib
     from fermicg_amd.marshal import flatten_ham, flatten_ops
@@ -365,6 +367,105 @@ def pt2(args):
                      "kernel": "k_build_blocks of the first-order pair plan (FP64 VALU, whose peak "
                                "equals the FP64 MFMA peak on MI355X); time includes the host join",
                      "flops_algorithmic": bf, "seconds": bs},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def pt2_12c(args):
+    """Second-order energy of the 48-orbital, 12-cluster, M=32 workload
+    (fermicg_amd.workloads.pt2_12c_M32: 9,020,329 rank <= 1 variational
+    configurations over the 26,533 Fock configurations within 2 transferred
+    electrons) at thresh_foi = 1e-5, over the sparse first-order space
+    (fcg_tpsci_pt2_sparse).  State: the CMF reference plus its regularised
+    first-order amplitudes above 0.1 (cmf_first_order_state, select=0.1:
+    1,923 nonzero coefficients), E0 = <c|Hcmf|c>.  One step = one
+    compute_pt2_energy call (items, screening, contraction count and fill,
+    sort, per-configuration sums), inputs on the host as the reference's call
+    takes them.  CPU baseline: the oracle's restatement over the
+    configurations reached (orc_tpsci_pt2_sparse) on the same state cut to
+    its largest coefficients, in kept contributions per second."""
+    import torch
+    from fermicg_amd import _lib  # noqa: F401  (fails loudly when the library is missing)
+    from fermicg_amd.tpsci import Engine, compute_pt2_energy
+    from fermicg_amd.workloads import cmf_first_order_state, pt2_12c_M32
+    thresh, select = 1e-5, 0.1
+    t0 = time.time()
+    p, st = pt2_12c_M32(M=32)
+    meta = dict(p.meta)
+    log("workload", meta, "built in %.1fs" % (time.time() - t0))
+    eng = Engine(0)
+    t1 = time.time()
+    st.set_vector(cmf_first_order_state(p, st, eng, select=select))
+    h0 = eng.local_operator(p.clustered_ham, "Hcmf")
+    E0 = np.einsum("ir,ir->r", st.coeffs,
+                   eng.tpsci_plan(st, p.cluster_ops, h0, matrix_free=True).sigma(st.coeffs))
+    eng.drop_plans()
+    nnz = int(np.count_nonzero(st.coeffs))
+    log("state: %d nonzero, E0 %s, %.1fs" % (nnz, E0, time.time() - t1))
+    times, e2, stats = [], None, None
+    for k in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        e2 = compute_pt2_energy(st, p.cluster_ops, p.clustered_ham, E0=E0, thresh_foi=thresh,
+                                verbose=0, method="sparse", engine=eng)
+        dt = time.perf_counter() - ta
+        stats = dict(compute_pt2_energy.last_stats)
+        if k >= args.warmup:
+            times.append(dt)
+        log("pt2 step %d: %.3f s" % (k, dt), stats)
+    sec = float(np.mean(times))
+    E = stats["n_contributions"]
+    cpu = None
+    if not args.no_cpu:
+        from fermicg_amd.marshal import flatten_ham, flatten_ops
+        from fermicg_amd.tpsci import cluster_sizes_of
+        from oracle.pyoracle import Oracle
+        threads, host = host_cpu()
+        keep = 12
+        c = st.coeffs.copy()
+        order = np.argsort(-np.abs(c[:, 0]))
+        c[order[keep:]] = 0.0
+        orc = Oracle(flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
+                     flatten_ham(p.clustered_ham), st)
+        ta = time.perf_counter()
+        _, n_jobs, n_x = orc.pt2_sparse(c, E0, thresh=thresh, nthreads=threads)
+        wall = time.perf_counter() - ta
+        nc = orc.last_contributions
+        cpu = {"value": nc / wall, "unit": "kept contributions/s", "cores": threads,
+               "kind": "port",
+               "sample": "orc_tpsci_pt2_sparse (C restatement of compute_pt2_energy, "
+                         "src/tpsci_pt2_energy.jl:10, over the configurations reached) on the "
+                         "state cut to its %d largest coefficients: %d contributions, %d "
+                         "first-order configurations, %d target Fock jobs, %d OpenMP threads, "
+                         "%.1f s wall" % (keep, nc, n_x, n_jobs, threads, wall),
+               "host": host}
+        log("cpu", cpu)
+    write_bytes = E * (16 + 8 * int(st.R))   # key + value per kept contribution
+    cs = stats["contraction_seconds"]
+    out = {
+        "metric": "PT2 kept contributions/s (compute_pt2_energy over the sparse first-order "
+                  "space, thresh_foi 1e-5, FP64)",
+        "value": E / sec, "unit": "contributions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded integrals; CMF reference plus selected first-order amplitudes)",
+        "config": {"workload": meta["workload"], "n_var": meta["n_cfg"], "n_fock": meta["n_fock"],
+                   "nonzero_coefficients": nnz, "roots": int(st.R), "thresh_foi": thresh,
+                   "select": select, "n_items": stats["n_items"],
+                   "n_kets_kept": stats["n_kets_kept"], "n_contributions": E,
+                   "n_configs": stats["n_configs"], "n_first_order": stats["n_first_order"],
+                   "n_chunks": stats["n_chunks"], "parallelism": "single GPU"},
+        "pt2_energies_per_s": 1.0 / sec,
+        "e2": [float(x) for x in e2], "E0": [float(x) for x in E0],
+        "phase_seconds": {"contraction": cs, "sort_and_sum": stats["sort_seconds"],
+                          "total": stats["seconds"]},
+        "roofline": {"bound": "hbm", "achieved": write_bytes / cs / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": write_bytes / cs / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "kernel": "k_pt2_emit, count and fill passes together: algorithmic bytes = "
+                               "kept contributions x (16-byte key + 8 R-byte value) written",
+                     "bytes_algorithmic": write_bytes, "seconds": cs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

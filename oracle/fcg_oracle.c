@@ -330,7 +330,7 @@ static int64_t fock_intern(int64_t *slot, int64_t cap, int16_t *keys, int64_t *n
  * configuration's linear index in the block (first cluster fastest);
  * open addressing, insertion order kept in `order` */
 typedef struct {
-  int64_t cap, n, R;
+  int64_t cap, n, R, adds;  /* adds: elements accumulated (kept contributions) */
   int64_t *key;   /* [cap], -1 = empty */
   int64_t *idx;   /* [cap] entry of the slot */
   int64_t *order; /* [n] keys by insertion */
@@ -340,6 +340,7 @@ typedef struct {
 static void lin_map_init(lin_map *m, int R) {
   m->cap = 1024;
   m->n = 0;
+  m->adds = 0;
   m->R = R;
   m->key = (int64_t *)malloc(sizeof(int64_t) * m->cap);
   m->idx = (int64_t *)malloc(sizeof(int64_t) * m->cap);
@@ -480,6 +481,7 @@ static int pt2_term(const orc_ops *o, const orc_ham *h, int t, const int16_t *fx
       r0 /= mb[a];
     }
     if (hm) {
+      hm->adds++;
       double *acc = lin_map_get(hm, lin);
       for (int r = 0; r < R; r++) acc[r] += sgn * v * cket[r];
     } else {
@@ -623,11 +625,12 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
  * too large to form (many clusters) for states with few nonzero
  * coefficients.  Per job the configurations are summed in the order they
  * were first reached.  n_x (nullable): configurations reached outside the
- * variational space.  Returns 0, -3 on a missing 1-body or H0 block, -4 when
+ * variational space; n_contrib (nullable): elements kept.  Returns 0, -3 on a missing 1-body or H0 block, -4 when
  * a target block has more than 2^62 configurations. */
 int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
                          const double *c, const double *e0, int nbody, int h0_op, double thresh,
-                         double *e2, int64_t *n_jobs, int64_t *n_x, int nthreads) {
+                         double *e2, int64_t *n_jobs, int64_t *n_x, int64_t *n_contrib,
+                         int nthreads) {
   const int N = o->n_clusters, S = o->S;
   if (N > MAXN || R < 1 || R > 64) return -1;
   /* ket Fock blocks holding a nonzero coefficient */
@@ -678,11 +681,11 @@ int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp
   fock_map_build(&vm, sp->fock, sp->n_fock, N);
   double *je2 = (double *)calloc((size_t)(nj > 0 ? nj : 1) * R, sizeof(double));
   int err = 0;
-  int64_t nxs = 0;
+  int64_t nxs = 0, ncs = 0;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nxs)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : nxs, ncs)
   for (int64_t j = 0; j < nj; j++) {
     const int16_t *fx = jfock + j * 2 * N;
     int64_t stride[MAXN], nx = 1;
@@ -755,6 +758,7 @@ int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp
         }
       }
     if (rc) err = rc;
+    ncs += hm.adds;
     free(drop);
     lin_map_free(&hm);
     free(w1);
@@ -771,6 +775,7 @@ int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp
   }
   if (n_jobs) *n_jobs = nj;
   if (n_x) *n_x = nxs;
+  if (n_contrib) *n_contrib = ncs;
   free(je2);
   free(vm.slot);
   free(pairs);

@@ -79,11 +79,12 @@ int orc_tpsci_pt2(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R
 /* orc_tpsci_pt2 with each target block's sigma kept over the
  * configurations reached (a map) and zero-coefficient kets skipped: target
  * blocks too large to form are served.  n_x: configurations reached outside
- * the variational space (nullable).  Returns as orc_tpsci_pt2, and -4 when a
+ * the variational space, n_contrib: elements kept (both nullable).  Returns as orc_tpsci_pt2, and -4 when a
  * target block has more than 2^62 configurations. */
 int orc_tpsci_pt2_sparse(const orc_ops *o, const orc_ham *h, const orc_space *sp, int R,
                          const double *c, const double *e0, int nbody, int h0_op, double thresh,
-                         double *e2, int64_t *n_jobs, int64_t *n_x, int nthreads);
+                         double *e2, int64_t *n_jobs, int64_t *n_x, int64_t *n_contrib,
+                         int nthreads);
 
 /* Reference open_matvec_thread (src/tpsci_matvec_thread.jl:11-129,
  * prescreen=false) on the target Fock blocks the caller lists (block j =

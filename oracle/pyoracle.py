@@ -51,7 +51,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, f64, P(f64), P(i64), ctypes.c_int]
         L.orc_tpsci_pt2_sparse.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int,
                                            P(f64), P(f64), ctypes.c_int, ctypes.c_int, f64,
-                                           P(f64), P(i64), P(i64), ctypes.c_int]
+                                           P(f64), P(i64), P(i64), P(i64), ctypes.c_int]
         L.orc_tpsci_open_sigma.argtypes = [P(OrcOps), P(OrcHam), P(OrcSpace), ctypes.c_int, P(f64),
                                            ctypes.c_int, f64, i64, P(ctypes.c_int16), P(i64), P(f64),
                                            P(ctypes.c_uint8), ctypes.c_int]
@@ -150,7 +150,8 @@ class Oracle:
     def pt2_sparse(self, c, e0, nbody=4, h0_op=1, thresh=0.0, nthreads=0):
         """compute_pt2_energy restated over the configurations reached (target
         blocks never formed whole; zero-coefficient kets skipped); returns
-        (e2, n_jobs, n_first_order)."""
+        (e2, n_jobs, n_first_order); the kept elements are in
+        self.last_contributions."""
         c = np.asarray(c, dtype=np.float64)
         if c.ndim == 1:
             c = c[:, None]
@@ -158,13 +159,14 @@ class Oracle:
         cc = np.ascontiguousarray(c.T)
         e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (R,)))
         e2 = np.zeros(R)
-        nj, nx = i64(), i64()
+        nj, nx, nc = i64(), i64(), i64()
         rc = lib().orc_tpsci_pt2_sparse(ctypes.byref(self.ops), ctypes.byref(self.ham),
                                         ctypes.byref(self.space), R, _p(cc, f64), _p(e0, f64),
                                         nbody, h0_op, thresh, _p(e2, f64), ctypes.byref(nj),
-                                        ctypes.byref(nx), nthreads)
+                                        ctypes.byref(nx), ctypes.byref(nc), nthreads)
         if rc:
             raise RuntimeError("oracle pt2_sparse failed: %d" % rc)
+        self.last_contributions = nc.value
         return e2, nj.value, nx.value
 
     def open_sigma(self, c, target_focks, sizes, nbody=4, thresh=0.0, nthreads=0):
