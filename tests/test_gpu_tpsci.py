@@ -100,3 +100,18 @@ def test_davidson_mixed_precision(eng, small):
                                max_iter=400, precision=prec, refresh_k=4, engine=eng)
         print(prec, e - w, v.davidson_stats, flush=True)
         assert np.abs(e - w).max() < 1e-6  # |dE| <= 1 uEh against FP64
+
+
+def test_davidson_large_subspace_host_eigensolver(eng, small):
+    """A subspace of up to 40 x 3 = 120 vectors: past 96 the projected
+    eigenproblem leaves the GPU eigensolver for the host one; the energies
+    still match dense diagonalisation, without a preconditioner so that the
+    subspace does grow past 96 before convergence."""
+    p, st, orc = small
+    H = orc.full_H()
+    w = np.linalg.eigvalsh(H)[:3]
+    e, v = tps_ci_davidson(st, p.cluster_ops, p.clustered_ham, conv_thresh=1e-9, max_iter=400,
+                           max_ss_vecs=40, precond=False, engine=eng)
+    print("davidson", e, w, v.davidson_stats, flush=True)
+    assert v.davidson_stats["iterations"] > 32   # m = 3 + 3 (it - 1) passed 96
+    assert np.abs(e - w).max() < 1e-10
