@@ -36,6 +36,7 @@ EXPORTS = [
     "fcg_sigma_device_noalloc",
     "fcg_tpsci_pt2_sparse",
     "fcg_plan_create_matrix_free",
+    "fcg_serve_start", "fcg_serve_submit", "fcg_serve_poll", "fcg_serve_stop",
 ]
 
 
@@ -49,6 +50,16 @@ class DavidsonStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("sigma_fp64", ctypes.c_int),
                 ("sigma_low", ctypes.c_int), ("converged", ctypes.c_int),
                 ("residual", c_f64 * 16)]
+
+
+class ServeOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int), ("hbm_budget", c_i64),
+                ("nbody", ctypes.c_int), ("davidson", DavidsonOpts)]
+
+
+class ServeTimes(ctypes.Structure):
+    _fields_ = [("submitted", c_f64), ("started", c_f64), ("planned", c_f64),
+                ("finished", c_f64), ("plan_bytes", c_i64)]
 
 
 class PT2Stats(ctypes.Structure):
@@ -122,6 +133,11 @@ def load():
     L.fcg_set_build_class_cap.argtypes = [VP, ctypes.c_int]
     L.fcg_plan_create_matrix_free.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int,
                                               P(VP)]
+    L.fcg_serve_start.argtypes = [P(ServeOpts), P(VP)]
+    L.fcg_serve_submit.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_i64)]
+    L.fcg_serve_poll.argtypes = [VP, c_i64, ctypes.c_int, P(c_f64), P(c_f64), P(DavidsonStats),
+                                 P(ServeTimes)]
+    L.fcg_serve_stop.argtypes = [VP]
     L.fcg_tpsci_pt2_sparse.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64),
                                        ctypes.c_int, ctypes.c_int, c_f64, P(c_f64),
                                        P(PT2SparseStats)]
@@ -438,6 +454,76 @@ class Plan(_Handle):
                  "sigma_low": st.sigma_low, "converged": bool(st.converged),
                  "residual": list(st.residual)[:R]}
         return ev, vec.T.copy(), stats
+
+
+class Server:
+    """The library's serving loop (fcg_serve_*): Davidson solves submitted as
+    they arrive, run with continuous batching on `slots` contexts under an
+    HBM budget for the live plans.  Handles passed to submit() stay
+    referenced until their ticket is polled."""
+
+    def __init__(self, slots=3, hbm_budget=2e8, precision="fp64", conv_thresh=1e-8, max_iter=300,
+                 max_ss_vecs=12, lindep_thresh=1e-12, refresh_k=0, precond=True, nbody=4,
+                 device=0):
+        self.L = load()
+        o = ServeOpts(device, slots, int(hbm_budget), nbody,
+                      DavidsonOpts(conv_thresh, max_iter, max_ss_vecs, lindep_thresh,
+                                   PRECISION[precision], refresh_k, 1 if precond else 0, None))
+        h = VP()
+        rc = self.L.fcg_serve_start(ctypes.byref(o), ctypes.byref(h))
+        if rc:
+            raise FCGError("fcg_serve_start: %s" % ERRORS.get(rc, rc))
+        self.h = h
+        self._live = {}
+
+    def submit(self, ham, ops, space, v0):
+        """Queue a solve from start vectors v0 (n x R); returns its ticket."""
+        v0 = np.asarray(v0, dtype=np.float64)
+        if v0.ndim == 1:
+            v0 = v0[:, None]
+        R = v0.shape[1]
+        if v0.shape[0] != space.nvar():
+            raise ValueError("start vectors do not match the space")
+        vv = np.ascontiguousarray(v0.T)
+        t = c_i64()
+        rc = self.L.fcg_serve_submit(self.h, ham.h, ops.h, space.h, R, _p(vv, c_f64),
+                                     ctypes.byref(t))
+        if rc:
+            raise FCGError("fcg_serve_submit: %s" % ERRORS.get(rc, rc))
+        self._live[t.value] = (ham, ops, space, R, v0.shape[0])
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """None while pending (wait=False); else a dict with the return code,
+        energies, vectors (n x R), Davidson stats and times (seconds since
+        the server started; plan_bytes)."""
+        ham, ops, space, R, n = self._live[ticket]
+        ev = np.zeros(R)
+        vec = np.zeros((R, n))
+        st = DavidsonStats()
+        tm = ServeTimes()
+        rc = self.L.fcg_serve_poll(self.h, ticket, 1 if wait else 0, _p(ev, c_f64),
+                                   _p(vec, c_f64), ctypes.byref(st), ctypes.byref(tm))
+        if rc == 1:
+            return None
+        del self._live[ticket]
+        return {"rc": rc, "energies": ev, "vectors": vec.T.copy(),
+                "stats": {"iterations": st.iterations, "sigma_fp64": st.sigma_fp64,
+                          "sigma_low": st.sigma_low, "converged": bool(st.converged),
+                          "residual": list(st.residual)[:R]},
+                "times": {k: getattr(tm, k) for k, _ in ServeTimes._fields_}}
+
+    def stop(self):
+        if getattr(self, "h", None):
+            self.L.fcg_serve_stop(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
 
 
 def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):

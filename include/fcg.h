@@ -38,7 +38,8 @@ enum {
   FCG_EMISSING_BLOCK = -3, /* a term needs a Γ block that was not uploaded */
   FCG_EOVERFLOW = -4,      /* a size exceeds a library limit */
   FCG_EDEVICE = -5,        /* HIP runtime error */
-  FCG_ENOTCONV = -6        /* Davidson did not converge within max_iter */
+  FCG_ENOTCONV = -6,       /* Davidson did not converge within max_iter */
+  FCG_PENDING = 1          /* fcg_serve_poll: the solve has not finished (not an error) */
 };
 
 /* Stored Hamiltonian-block precisions for the sigma apply.  FCG_FP64 and
@@ -227,6 +228,48 @@ typedef struct {
 int fcg_davidson(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double *v0,
                  const fcg_davidson_opts *opts, double *evals, double *evecs,
                  fcg_davidson_stats *stats);
+
+/* Serving loop: independent Davidson solves submitted as they arrive and run
+ * with continuous batching on one GPU.  `slots` solver threads, each with its
+ * own context (HIP stream), take queued requests oldest first, so solves of
+ * different sizes overlap on the device; each builds its sigma plan under
+ * the process-wide plan budget (set to hbm_budget while the server runs, see
+ * fcg_set_plan_budget) and, when the plan does not fit beside the resident
+ * ones, waits until a running solve frees its plan; a plan larger than the
+ * whole budget fails its request with FCG_ENOMEM.  Every solve runs
+ * fcg_davidson with `davidson` (its precond_diag ignored: the diagonal of
+ * H) and frees its plan.  Solves are deterministic, so a request's results
+ * are the same alone and in the mix.  No counterpart in the reference, whose
+ * Julia host would call tps_ci_davidson (src/tpsci_outer.jl:364) per
+ * request; for the served mix of independent solves. */
+typedef struct fcg_server *fcg_server_t;
+typedef struct {
+  int device;
+  int slots;                   /* concurrent solves (1..64) */
+  int64_t hbm_budget;          /* device bytes all live plans may hold */
+  int nbody;                   /* terms on at most this many clusters (1..4) */
+  fcg_davidson_opts davidson;  /* options of every solve */
+} fcg_serve_opts;
+typedef struct {
+  double submitted, started, planned, finished;  /* seconds since fcg_serve_start */
+  int64_t plan_bytes;                            /* the request's plan footprint */
+} fcg_serve_times;
+int fcg_serve_start(const fcg_serve_opts *opts, fcg_server_t *out);
+/* Queue a solve of the lowest R eigenpairs of the Hamiltonian `ham` over
+ * `sp` from the host start vectors v0 (n_var x R, copied).  The handles
+ * must be uploaded and stay alive until the ticket is polled done; several
+ * requests may share them.  *ticket identifies the request. */
+int fcg_serve_submit(fcg_server_t srv, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp, int R,
+                     const double *v0, int64_t *ticket);
+/* FCG_PENDING while the request is queued or running (wait = 0), else the
+ * solve's return code with evals (R), evecs (n_var x R), stats and times
+ * filled (each nullable) and the ticket released.  wait = 1 blocks until
+ * the request finishes. */
+int fcg_serve_poll(fcg_server_t srv, int64_t ticket, int wait, double *evals, double *evecs,
+                   fcg_davidson_stats *stats, fcg_serve_times *times);
+/* Finish every queued solve, join the slots, restore the plan budget and
+ * free the server (results not polled are dropped). */
+int fcg_serve_stop(fcg_server_t srv);
 
 /* Second-order energy of a TPSCI state over its first-order space (replaces
  * compute_pt2_energy, reference src/tpsci_pt2_energy.jl:10-175, with its
