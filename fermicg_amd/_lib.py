@@ -35,6 +35,7 @@ EXPORTS = [
     "fcg_open_sigma",
     "fcg_sigma_device_noalloc",
     "fcg_tpsci_pt2_sparse",
+    "fcg_tpsci_pt2_sparse_ex",
     "fcg_plan_create_matrix_free",
     "fcg_serve_start", "fcg_serve_submit", "fcg_serve_poll", "fcg_serve_stop",
 ]
@@ -141,6 +142,9 @@ def load():
     L.fcg_tpsci_pt2_sparse.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64),
                                        ctypes.c_int, ctypes.c_int, c_f64, P(c_f64),
                                        P(PT2SparseStats)]
+    L.fcg_tpsci_pt2_sparse_ex.argtypes = [VP, VP, VP, VP, ctypes.c_int, P(c_f64), P(c_f64),
+                                          ctypes.c_int, ctypes.c_int, c_f64, ctypes.c_int,
+                                          P(c_f64), P(PT2SparseStats)]
     L.fcg_open_sigma.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(c_f64), c_f64,
                                  P(c_f64), P(ctypes.c_uint8)]
     L.fcg_plan_destroy.argtypes = [VP]
@@ -543,9 +547,12 @@ def tpsci_pt2(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
     return e2, {k: getattr(st, k) for k, _ in PT2Stats._fields_}
 
 
-def tpsci_pt2_sparse(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
+def tpsci_pt2_sparse(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0, precision="fp64"):
     """Second-order energies over the sparse first-order space
-    (fcg_tpsci_pt2_sparse); returns (e2, stats dict)."""
+    (fcg_tpsci_pt2_sparse_ex; precision "fp64" or "fp32" for the
+    contraction); returns (e2, stats dict)."""
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("the sparse PT2 contracts in fp64 or fp32")
     c = np.asarray(c, dtype=np.float64)
     if c.ndim == 1:
         c = c[:, None]
@@ -554,9 +561,10 @@ def tpsci_pt2_sparse(ctx, ham, ops, space, c, e0, nbody=4, h0_op=1, thresh=0.0):
     e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(e0, np.float64), (R,)))
     e2 = np.zeros(R)
     st = PT2SparseStats()
-    ctx.check(ctx.L.fcg_tpsci_pt2_sparse(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64),
-                                         _p(e0, c_f64), nbody, h0_op, float(thresh),
-                                         _p(e2, c_f64), ctypes.byref(st)),
+    ctx.check(ctx.L.fcg_tpsci_pt2_sparse_ex(ctx.h, ham.h, ops.h, space.h, R, _p(cc, c_f64),
+                                            _p(e0, c_f64), nbody, h0_op, float(thresh),
+                                            PRECISION[precision], _p(e2, c_f64),
+                                            ctypes.byref(st)),
               "fcg_tpsci_pt2_sparse")
     return e2, {k: getattr(st, k) for k, _ in PT2SparseStats._fields_}
 

@@ -449,10 +449,10 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t *sWave, int64_t
 // 4-body term's stage after the second cluster would not fit, over the
 // pairs of surviving states of the last two clusters (T = 2), so that every
 // stored stage fits half the arena and several workgroups share a CU.
-template <bool FILL>
+template <bool FILL, typename TC>
 __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   __shared__ double sV[VMAX];
-  __shared__ double sA[ARENA];
+  __shared__ TC sA[ARENA];           // chain intermediates in the contraction's precision
   __shared__ double sW[4][NPMAX];     // per-cluster bound weights w_q[p]
   __shared__ double sCM[4][NPMAX];    // column maxima of the group's ket states
   __shared__ int16_t sSurv[4][SMAX];  // surviving bra states per active cluster
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
       const double *G0 = a.gdata + a.gb_off[sBlk[0]];
       for (int t0 = 0; t0 < ntile; t0 += tk) {
         const int tn = nb == 1 ? 1 : min(tk, ntile - t0);
-        double *cur = sA, *nxt = sA + ARENA / 2;
+        TC *cur = sA, *nxt = sA + ARENA / 2;
         int cols = 1;
         if (nb > 1) {
           int pre = pre_full;
@@ -740,8 +740,8 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
               const int r = e % pre, i = e / pre;
               const int iL = (t0 + i) / nsA;
               const double *Gi = G + (int64_t)np * (sSurv[q][iL] + (int64_t)mb * Jq);
-              double v = 0.0;
-              for (int p = 0; p < np; p++) v += sV[r + pre * p] * Gi[p];
+              TC v = 0;
+              for (int p = 0; p < np; p++) v += (TC)sV[r + pre * p] * (TC)Gi[p];
               cur[e] = v;
             }
             flops += 2ull * pre * tn * np;
@@ -758,13 +758,13 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
               const int r2 = e % pre2, i = e / pre2;
               const int iA = (t0 + i) % nsA;
               const double *Gi = G + (int64_t)np * (sSurv[q][iA] + (int64_t)mb * Jq);
-              double v = 0.0;
-              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * i] * Gi[p];
+              TC v = 0;
+              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * i] * (TC)Gi[p];
               nxt[e] = v;
             }
             flops += 2ull * pre2 * tn * np;
             __syncthreads();
-            double *tmp = cur;
+            TC *tmp = cur;
             cur = nxt;
             nxt = tmp;
             pre = pre2;
@@ -778,13 +778,13 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
             for (int e = tid; e < pre2 * ns * cols; e += PT) {
               const int r2 = e % pre2, rem = e / pre2, i = rem % ns, col = rem / ns;
               const double *Gi = G + (int64_t)np * (sSurv[q][i] + (int64_t)mb * Jq);
-              double v = 0.0;
-              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * col] * Gi[p];
+              TC v = 0;
+              for (int p = 0; p < np; p++) v += cur[r2 + pre2 * p + pre * col] * (TC)Gi[p];
               nxt[e] = v;
             }
             flops += 2ull * pre2 * ns * cols * np;
             __syncthreads();
-            double *tmp = cur;
+            TC *tmp = cur;
             cur = nxt;
             nxt = tmp;
             pre = pre2;
@@ -804,13 +804,13 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
             const int i0 = e % ns0, col = e / ns0;
             double x;
             if (nb == 1) {
-              x = G0[sSurv[0][i0] + (int64_t)sMb[0] * J0];  // 2-D block (np = 1)
+              x = (double)(TC)G0[sSurv[0][i0] + (int64_t)sMb[0] * J0];  // 2-D block (np = 1)
             } else {
               const int np = sNp[0];
               const double *Gi = G0 + (int64_t)np * (sSurv[0][i0] + (int64_t)sMb[0] * J0);
-              double v = 0.0;
-              for (int p = 0; p < np; p++) v += cur[p + np * col] * Gi[p];
-              x = v;
+              TC v = 0;
+              for (int p = 0; p < np; p++) v += cur[p + np * col] * (TC)Gi[p];
+              x = (double)v;
             }
             const double ax = fabs(x);
             if (pass2 == 0) {
@@ -1071,9 +1071,18 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
                                     int R, const double *c, const double *e0, int nbody,
                                     int h0_op, double thresh, double *e2,
                                     fcg_pt2_sparse_stats *st) {
+  return fcg_tpsci_pt2_sparse_ex(ctx, h, o, var, R, c, e0, nbody, h0_op, thresh, FCG_FP64, e2, st);
+}
+
+extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var,
+                                       int R, const double *c, const double *e0, int nbody,
+                                       int h0_op, double thresh, int precision, double *e2,
+                                       fcg_pt2_sparse_stats *st) {
   if (!ctx || !h || !o || !var || !c || !e0 || !e2 || R < 1 || R > FCG_MAX_R || nbody < 1 ||
-      nbody > 4 || h0_op < 0 || h0_op >= o->n_ops || !(thresh >= 0.0))
+      nbody > 4 || h0_op < 0 || h0_op >= o->n_ops || !(thresh >= 0.0) ||
+      (precision != FCG_FP64 && precision != FCG_FP32))
     return FCG_EINVAL;
+  const bool fp32 = precision == FCG_FP32;
   if (!var->tpsci) {
     ctx->err = "fcg_tpsci_pt2_sparse needs a TPSCI space";
     return FCG_EINVAL;
@@ -1406,8 +1415,12 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
   const auto te = std::chrono::steady_clock::now();
   for (int64_t i0 = 0; i0 < n_slots; i0 += SLOT_CHUNK) {
     a.slot_base = i0;
-    hipLaunchKernelGGL(k_pt2_emit<false>, dim3((unsigned)std::min(SLOT_CHUNK, n_slots - i0)),
-                       dim3(PT), 0, s, a);
+    if (fp32)
+      hipLaunchKernelGGL((k_pt2_emit<false, float>),
+                         dim3((unsigned)std::min(SLOT_CHUNK, n_slots - i0)), dim3(PT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_pt2_emit<false, double>),
+                         dim3((unsigned)std::min(SLOT_CHUNK, n_slots - i0)), dim3(PT), 0, s, a);
     if (verbose()) {
       FCG_HIP(hipStreamSynchronize(s), ctx);
       vlog("contraction count chunk", t0, (double)std::min(i0 + SLOT_CHUNK, n_slots));
@@ -1553,8 +1566,12 @@ extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg
     a.n_slots = s1;
     for (int64_t i0 = s0; i0 < s1; i0 += SLOT_CHUNK) {
       a.slot_base = i0;
-      hipLaunchKernelGGL(k_pt2_emit<true>, dim3((unsigned)std::min(SLOT_CHUNK, s1 - i0)),
-                         dim3(PT), 0, s, a);
+      if (fp32)
+        hipLaunchKernelGGL((k_pt2_emit<true, float>), dim3((unsigned)std::min(SLOT_CHUNK, s1 - i0)),
+                           dim3(PT), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_pt2_emit<true, double>),
+                           dim3((unsigned)std::min(SLOT_CHUNK, s1 - i0)), dim3(PT), 0, s, a);
     }
     FCG_HIP(hipGetLastError(), ctx);
     if (verbose()) {

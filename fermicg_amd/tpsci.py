@@ -113,16 +113,19 @@ class Engine:
                                bra, ket, ket_state.coeffs, nbody=nbody, thresh=thresh)
 
     def pt2(self, ci_vector, cluster_ops, clustered_ham, E0, nbody=4, H0="Hcmf", thresh=0.0,
-            method="dense"):
+            method="dense", precision="fp64"):
         """e2 of an orthonormal TPSCI state over its first-order space:
         method "dense" (fcg_tpsci_pt2, whole target Fock blocks) or "sparse"
         (fcg_tpsci_pt2_sparse, the configurations screened contributions
         reach).  Returns (e2 [R], statistics dict)."""
         from .cluster_ops import OP_NAMES
         space = _lib.DeviceSpace.tpsci(self.ctx, ci_vector)
-        fn = _lib.tpsci_pt2_sparse if method == "sparse" else _lib.tpsci_pt2
-        return fn(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops), space,
-                  ci_vector.coeffs, E0, nbody=nbody, h0_op=OP_NAMES.index(H0), thresh=thresh)
+        args = (self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops), space,
+                ci_vector.coeffs, E0)
+        kw = dict(nbody=nbody, h0_op=OP_NAMES.index(H0), thresh=thresh)
+        if method == "sparse":
+            return _lib.tpsci_pt2_sparse(*args, precision=precision, **kw)
+        return _lib.tpsci_pt2(*args, **kw)
 
     def drop_plans(self, keep=()):
         """Free the cached plans (their device blocks) except those whose
@@ -275,7 +278,8 @@ def compute_expectation_value(ci_vector, cluster_ops, clustered_ham, engine=None
 
 
 def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
-                       thresh_foi=1e-9, prescreen=True, verbose=1, method="auto", engine=None):
+                       thresh_foi=1e-9, prescreen=True, verbose=1, method="auto", engine=None,
+                       precision="fp64"):
     """Second-order energy correction e2[r] of a TPSCI state over its
     first-order space (compute_pt2_energy, src/tpsci_pt2_energy.jl:10-175).
 
@@ -292,7 +296,9 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     target Fock block (few clusters), "sparse" only on the configurations
     screened contributions reach, generated per (ket configuration, term)
     as the reference does (many clusters); "auto" takes "sparse" above 4
-    clusters.  Both give the same e2.  Returns e2 (length R)."""
+    clusters.  Both give the same e2.  `precision` "fp32" runs the sparse
+    engine's contraction in FP32 (every sum after it in FP64).  Returns e2
+    (length R)."""
     eng = engine or default_engine()
     if method == "auto":
         method = "sparse" if ci_vector.N > 4 else "dense"
@@ -302,8 +308,10 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
         E0 = compute_expectation_value(ci, cluster_ops, eng.local_operator(clustered_ham, H0),
                                        engine=eng)
     E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
+    if precision != "fp64" and method != "sparse":
+        raise ValueError("a reduced-precision PT2 contraction needs method='sparse'")
     e2, stats = eng.pt2(ci, cluster_ops, clustered_ham, E0, nbody=nbody, H0=H0,
-                        thresh=thresh_foi, method=method)
+                        thresh=thresh_foi, method=method, precision=precision)
     if verbose:
         evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
         if method == "sparse":

@@ -332,6 +332,16 @@ typedef struct {
 int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
                          const double *c, const double *e0, int nbody, int h0_op, double thresh,
                          double *e2, fcg_pt2_sparse_stats *stats);
+/* The same with the contraction's arithmetic chosen: FCG_FP64, or FCG_FP32
+ * (Γ and integrals rounded to FP32 as read, the chain and each element in
+ * FP32; the threshold test, the values x c, the per-configuration sums and
+ * the compensated e2 sum in FP64, the bounds from the FP64 data): the
+ * mixed-precision PT2, whose |e2(FP32) - e2(FP64)| a caller reports
+ * against this FP64 engine.  No counterpart in the reference (FP64 only). */
+int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t var, int R,
+                            const double *c, const double *e0, int nbody, int h0_op,
+                            double thresh, int precision, double *e2,
+                            fcg_pt2_sparse_stats *stats);
 
 /* Open matvec of a TPSCI state onto the configurations of `bra` (replaces
  * open_matvec_thread, src/tpsci_matvec_thread.jl:11-129, for the target

@@ -15,6 +15,7 @@ Fixtures decoded (reference test/ directory):
   _testdata_cmf_h8.jld2      ints, d1, clusters, init_fspace, e_fci
   _testdata_cmf_h9.jld2      ints, d1, clusters, init_fspace, e_fci (3 states)
   _testdata_cmf_h12_64bit.jld2  ints, d1, clusters, init_fspace, e_cmf
+  _testdata_cmf_cr2_19.jld2  ints, d1, clusters, init_fspace (test/test_qdpt.jl)
 The ``cluster_bases`` entries hold ActiveSpaceSolvers objects whose
 determinant ordering is not documented in the reference; they are skipped.
 """
@@ -65,6 +66,7 @@ def main():
         "_testdata_cmf_h8.jld2": "h8",
         "_testdata_cmf_h9.jld2": "h9",
         "_testdata_cmf_h12_64bit.jld2": "h12",
+        "_testdata_cmf_cr2_19.jld2": "cr2_19",
     }
     for fn, tag in names.items():
         rec = decode(os.path.join(src, fn), os.path.join(here, tag + ".npz"))

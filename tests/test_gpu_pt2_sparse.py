@@ -112,3 +112,27 @@ def test_sparse_pt2_twelve_clusters(eng, M, nnz, thresh):
     print(M, nnz, thresh, e2, ref, nx, stats, flush=True)
     assert stats["n_first_order"] == nx
     assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("case_name,thresh", [("six", 1e-6), ("twelve", 1e-5)])
+def test_sparse_pt2_fp32_contraction(eng, six, case_name, thresh):
+    """The mixed form (FP32 chain and elements, FP64 tests, values and sums)
+    against the FP64 engine and the oracle: |e2(FP32) - e2(FP64)| within
+    1e-5 of |e2| (FP32 rounding of the contractions; elements within that
+    rounding of the threshold may flip), the kept-contribution count within
+    1e-4 of FP64's, and repeated calls bit-identical."""
+    from tests.pt2_cases import twelve_clusters_rank1
+    case = six if case_name == "six" else twelve_clusters_rank1(M=8, nnz=10)
+    st = case.state
+    e0 = _e0(case, st) if case_name == "six" else np.array([-30000.0])
+    kw = dict(E0=e0, thresh_foi=thresh, verbose=0, method="sparse", engine=eng)
+    e64 = compute_pt2_energy(st, case.ops, case.H, **kw)
+    s64 = dict(compute_pt2_energy.last_stats)
+    e32 = compute_pt2_energy(st, case.ops, case.H, precision="fp32", **kw)
+    s32 = dict(compute_pt2_energy.last_stats)
+    again = compute_pt2_energy(st, case.ops, case.H, precision="fp32", **kw)
+    print(case_name, e64, e32, np.abs(e32 - e64) / np.abs(e64), s64["n_contributions"],
+          s32["n_contributions"], flush=True)
+    assert np.array_equal(again, e32)
+    assert np.abs(e32 - e64).max() < 1e-5 * np.abs(e64).max()
+    assert abs(s32["n_contributions"] - s64["n_contributions"]) <= 1e-4 * s64["n_contributions"]
