@@ -12,6 +12,10 @@
   rank-1 space of 580,673 configurations): its sigma against the oracle on
   sampled rows, and its energies in a concurrent mix equal to its energies
   alone.
+* 48 orbitals / 12 clusters / M=32 / rank-1 (pt2_12c_M32, 9,020,329
+  configurations): the sparse PT2 at thresh_foi 1e-5 against the oracle's
+  restatement over the configurations reached, with the first-order state
+  cut to 12 nonzero coefficients so that the oracle finishes in seconds.
 
 Bars: sigma relative to max|sigma| 1e-12 (FP64), 1e-5 (FP32), 1e-4
 (split-BF16); mixed-precision energies within 1 uEh of FP64 (converged on
@@ -124,3 +128,34 @@ def test_served_top_request_8c_M16():
     print("mix", res[0]["energies"], "alone", list(e_alone), flush=True)
     assert np.abs(np.array(res[0]["energies"]) - e_alone).max() < 1e-12
     srv.close()
+
+
+def test_pt2_48o_12c_M32_full_size():
+    """The 12-cluster PT2 workload at its full size (48 orbitals, M=32,
+    9,020,329 rank <= 1 variational configurations, thresh_foi 1e-5) with the
+    first-order state of the benchmark cut to its 12 largest coefficients:
+    the sparse GPU PT2 against the oracle's restatement over the
+    configurations reached (orc_tpsci_pt2_sparse, 16 host threads): e2 to
+    1e-11 relative and the number of first-order configurations exactly;
+    the mixed FP32 contraction within 1e-6 relative of FP64."""
+    from fermicg_amd.tpsci import compute_pt2_energy
+    from fermicg_amd.workloads import cmf_first_order_state, pt2_12c_M32
+    p, st = pt2_12c_M32(M=32)
+    eng = Engine(0)
+    c = cmf_first_order_state(p, st, eng, select=0.1)
+    eng.drop_plans()
+    order = np.argsort(-np.abs(c[:, 0]), kind="stable")
+    c[order[12:]] = 0.0
+    st.set_vector(c / np.linalg.norm(c))
+    e0 = np.array([-30000.0])
+    kw = dict(E0=e0, thresh_foi=1e-5, verbose=0, method="sparse", engine=eng)
+    e2 = compute_pt2_energy(st, p.cluster_ops, p.clustered_ham, **kw)
+    stats = dict(compute_pt2_energy.last_stats)
+    e32 = compute_pt2_energy(st, p.cluster_ops, p.clustered_ham, precision="fp32", **kw)
+    orc = Oracle(flatten_ops(p.cluster_ops, cluster_sizes_of(p.cluster_ops)),
+                 flatten_ham(p.clustered_ham), st)
+    ref, _, nx = orc.pt2_sparse(st.coeffs, e0, thresh=1e-5, nthreads=16)
+    print(e2, e32, ref, nx, stats, flush=True)
+    assert stats["n_first_order"] == nx and nx > 1e7
+    assert np.abs(e2 - ref).max() < 1e-11 * np.abs(ref).max()
+    assert np.abs(e32 - e2).max() < 1e-6 * np.abs(e2).max()
