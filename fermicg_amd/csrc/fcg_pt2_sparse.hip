@@ -526,6 +526,7 @@ __global__ __launch_bounds__(PT) void k_pt2_emit(Pt2Args a) {
   for (int e = tid; e < nv; e += PT) sV[e] = nb > 1 ? a.ints[T.ioff + e] : 1.0;
   KetBound kb;
   ket_bound_setup(a, T, fj, kb);
+  kb.V = sV;  // the staged integrals (read after the batch loop's first barrier)
   const double Bi = a.items_b[it] * SCREEN_SLACK;
   const int64_t k0 = a.fstart[fj], k1 = a.fstart[fj + 1];
   unsigned long long groups_kept = 0, flops = 0;
@@ -908,10 +909,6 @@ __device__ __forceinline__ void neumaier(double &sum, double &comp, double v) {
   sum = t;
 }
 
-__device__ __forceinline__ bool key_less(const Key128 &a, const Key128 &b) {
-  return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
-}
-
 struct SumArgs {
   int N, S, R, n_ops, h0_op, fbits, sbits, sN;
   int64_t n_seg;
@@ -923,6 +920,7 @@ struct SumArgs {
   const double *eval;        // [E][R] in emission order
   const Key128 *vkey;        // variational configurations, sorted
   int64_t n_v;
+  const int64_t *tvar;       // per chunk target: [start, end) of its Fock block in vkey
   const int32_t *gblock;
   const int64_t *gb_off;
   const int32_t *gb_dims;
@@ -947,15 +945,18 @@ __global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
        s += (int64_t)gridDim.x * blockDim.x) {
     const int64_t b = a.seg_start[s], e = a.seg_start[s + 1];
     const Key128 kc = a.skey[b];
-    const Key128 k{a.tkeys[chunk_key_target(kc, a.sN)], chunk_key_states(kc, a.sN)};
-    // variational configurations are projected out
-    int64_t lo = 0, hi = a.n_v;
+    const uint64_t tl = chunk_key_target(kc, a.sN);
+    const Key128 k{a.tkeys[tl], chunk_key_states(kc, a.sN)};
+    // variational configurations are projected out (searched among the
+    // target Fock block's own, an empty range when it holds none)
+    const int64_t v1 = a.tvar[2 * tl + 1];
+    int64_t lo = a.tvar[2 * tl], hi = v1;
     while (lo < hi) {
       const int64_t mid = (lo + hi) / 2;
-      if (key_less(a.vkey[mid], k)) lo = mid + 1;
+      if (a.vkey[mid].lo < k.lo) lo = mid + 1;
       else hi = mid;
     }
-    if (lo < a.n_v && a.vkey[lo].hi == k.hi && a.vkey[lo].lo == k.lo) continue;
+    if (lo < v1 && a.vkey[lo].lo == k.lo) continue;
     nout++;
     double hd = 0.0;
     const int SS = a.S * a.S;
@@ -1089,11 +1090,12 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
   }
   (void)hipSetDevice(ctx->device);
   const auto t0 = std::chrono::steady_clock::now();
-  if (int rc = fcg_form_boxes(ctx, o, var)) return rc;
   const int N = var->N, S = o->S;
   hipStream_t s = ctx->stream;
-  // configurations of the space (the boxes enumerate them in the caller's
-  // order), grouped by Fock block
+  // configurations of the space, grouped by Fock block: read straight from
+  // the uploaded configurations while the space has not been cut into boxes
+  // (the boxes serve plans, not this engine), else enumerated from the boxes
+  // (which keep the caller's order)
   const int64_t n_cfg = var->nvar, n_fock = var->n_fock;
   if (n_cfg >= (int64_t(1) << 31)) {
     ctx->err = "fcg_tpsci_pt2_sparse: more than 2^31 configurations";
@@ -1101,45 +1103,86 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
   }
   std::vector<uint8_t> cst((size_t)std::max<int64_t>(n_cfg, 1) * N);
   std::vector<int64_t> fstart(n_fock + 1, 0);
-  for (int64_t b = 0; b < var->n_box; b++) {
+  const bool unboxed = var->n_box < 0;
+  if (unboxed) {
+    // lo holds the configurations (1-based, n_cfg x N), var_off the Fock
+    // blocks' first configurations
+    if ((int64_t)var->var_off.size() != n_fock + 1 || var->var_off[n_fock] != n_cfg ||
+        (int64_t)var->lo.size() != n_cfg * N) {
+      ctx->err = "fcg_tpsci_pt2_sparse: inconsistent TPSCI space";
+      return FCG_EINVAL;
+    }
+    for (int64_t f = 0; f <= n_fock; f++) {
+      fstart[f] = var->var_off[f];
+      if ((f == 0 && fstart[0] != 0) || (f > 0 && fstart[f] < fstart[f - 1])) {
+        ctx->err = "fcg_tpsci_pt2_sparse: Fock block starts must ascend from 0";
+        return FCG_EINVAL;
+      }
+    }
+    int over = 0;
+#pragma omp parallel for schedule(static) reduction(| : over)
+    for (int64_t e = 0; e < n_cfg * N; e++) {
+      const int st0 = var->lo[e] - 1;
+      over |= st0 < 0 || st0 > 255;
+      cst[e] = (uint8_t)st0;
+    }
+    if (over) {
+      ctx->err = "fcg_tpsci_pt2_sparse: state index outside 1..256";
+      return FCG_EOVERFLOW;
+    }
+  }
+  for (int64_t b = 0; !unboxed && b < var->n_box; b++) {
     const int32_t f = var->box_fock[b];
     if (b > 0 && f < var->box_fock[b - 1]) {
       ctx->err = "fcg_tpsci_pt2_sparse: Fock blocks must be contiguous";
       return FCG_EINVAL;
     }
+    fstart[f + 1] += var->var_off[b + 1] - var->var_off[b];
+  }
+  int over = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : over)
+  for (int64_t b = 0; b < (unboxed ? 0 : var->n_box); b++) {
     const int64_t v0 = var->var_off[b], d = var->var_off[b + 1] - v0;
-    fstart[f + 1] += d;
     for (int64_t e = 0; e < d; e++) {
       int64_t r = e;
       for (int cc = 0; cc < N; cc++) {
         const int rk = var->rank[b * N + cc];
         const int st0 = (int)(r % rk) + var->lo[b * N + cc] - 1;
         r /= rk;
-        if (st0 > 255) {
-          ctx->err = "fcg_tpsci_pt2_sparse: state index above 256";
-          return FCG_EOVERFLOW;
-        }
+        over |= st0 > 255;
         cst[(v0 + e) * N + cc] = (uint8_t)st0;
       }
     }
   }
-  for (int64_t f = 0; f < n_fock; f++) fstart[f + 1] += fstart[f];
+  if (over) {
+    ctx->err = "fcg_tpsci_pt2_sparse: state index above 256";
+    return FCG_EOVERFLOW;
+  }
+  if (!unboxed)
+    for (int64_t f = 0; f < n_fock; f++) fstart[f + 1] += fstart[f];
   // every state inside its sector's basis; every term's Γ blocks shaped as
   // its integrals (the kernels index both without further checks)
+  int bad_sector = 0, bad_state = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad_sector, bad_state)
   for (int64_t f = 0; f < n_fock; f++) {
     for (int cc = 0; cc < N; cc++) {
       const int na = var->fock[(f * N + cc) * 2], nb = var->fock[(f * N + cc) * 2 + 1];
       if (na < 0 || nb < 0 || na >= S || nb >= S) {
-        ctx->err = "fcg_tpsci_pt2_sparse: Fock sector outside the clusters";
-        return FCG_EINVAL;
+        bad_sector = 1;
+        continue;
       }
       const int m = o->nstates[(int64_t)cc * S * S + na * S + nb];
       for (int64_t j = fstart[f]; j < fstart[f + 1]; j++)
-        if (cst[j * N + cc] >= m || cst[j * N + cc] >= 255) {
-          ctx->err = "fcg_tpsci_pt2_sparse: state index outside its sector's basis";
-          return FCG_EINVAL;
-        }
+        bad_state |= cst[j * N + cc] >= m || cst[j * N + cc] >= 255;
     }
+  }
+  if (bad_sector) {
+    ctx->err = "fcg_tpsci_pt2_sparse: Fock sector outside the clusters";
+    return FCG_EINVAL;
+  }
+  if (bad_state) {
+    ctx->err = "fcg_tpsci_pt2_sparse: state index outside its sector's basis";
+    return FCG_EINVAL;
   }
   {
     const int64_t SS = (int64_t)S * S;
@@ -1195,23 +1238,35 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
     for (int64_t j = j0; j < j1; j++) csort[j] = cabs[kord[j]];
     cmaxF[f] = j1 > j0 ? csort[j0] : 0.0;
   }
-  // variational keys
-  std::vector<Key128> vk(std::max<int64_t>(n_cfg, 1));
-#pragma omp parallel for schedule(dynamic, 64)
+  // variational keys, sorted: the Fock blocks by their sector codes, each
+  // block's states sorted in parallel (blocks have distinct sector codes)
+  std::vector<uint64_t> fkey(std::max<int64_t>(n_fock, 1));
   for (int64_t f = 0; f < n_fock; f++) {
     uint64_t hi = 0;
     for (int cc = 0; cc < N; cc++)
       hi |= (uint64_t)(var->fock[(f * N + cc) * 2] * S + var->fock[(f * N + cc) * 2 + 1])
             << (fbits * cc);
+    fkey[f] = hi;
+  }
+  std::vector<int64_t> forder(std::max<int64_t>(n_fock, 1)), vstart(n_fock + 1, 0);
+  std::iota(forder.begin(), forder.begin() + n_fock, (int64_t)0);
+  std::sort(forder.begin(), forder.begin() + n_fock,
+            [&](int64_t x, int64_t y) { return fkey[x] < fkey[y]; });
+  for (int64_t k = 0; k < n_fock; k++)
+    vstart[k + 1] = vstart[k] + (fstart[forder[k] + 1] - fstart[forder[k]]);
+  std::vector<Key128> vk(std::max<int64_t>(n_cfg, 1));
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t k = 0; k < n_fock; k++) {
+    const int64_t f = forder[k];
+    Key128 *out = vk.data() + vstart[k];
     for (int64_t j = fstart[f]; j < fstart[f + 1]; j++) {
       uint64_t lo = 0;
       for (int cc = 0; cc < N; cc++) lo |= (uint64_t)cst[j * N + cc] << (sbits * cc);
-      vk[j] = Key128{hi, lo};
+      out[j - fstart[f]] = Key128{fkey[f], lo};
     }
+    std::sort(out, out + (fstart[f + 1] - fstart[f]),
+              [](const Key128 &x, const Key128 &y) { return x.lo < y.lo; });
   }
-  std::sort(vk.begin(), vk.begin() + n_cfg, [](const Key128 &x, const Key128 &y) {
-    return x.hi < y.hi || (x.hi == y.hi && x.lo < y.lo);
-  });
   // term table
   std::vector<PTerm> terms(h->n_terms);
   for (int g = 0; g < h->n_tc; g++)
@@ -1480,10 +1535,27 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
     }
     tstart.push_back((int64_t)tkeys.size());
   }
+  // each target's Fock block among the sorted variational keys (empty when
+  // the space has no configuration there)
+  std::vector<int64_t> tvar(2 * std::max<size_t>(tkeys.size(), 1), 0);
+  {
+    std::vector<uint64_t> sk(std::max<int64_t>(n_fock, 1));
+    for (int64_t k = 0; k < n_fock; k++) sk[k] = fkey[forder[k]];
+    for (size_t t = 0; t < tkeys.size(); t++) {
+      const auto it = std::lower_bound(sk.begin(), sk.begin() + n_fock, tkeys[t]);
+      const int64_t k = it - sk.begin();
+      if (k < n_fock && sk[k] == tkeys[t]) {
+        tvar[2 * t] = vstart[k];
+        tvar[2 * t + 1] = vstart[k + 1];
+      }
+    }
+  }
   Buf<uint32_t> d_tloc(ctx);
   Buf<uint64_t> d_tkeys(ctx);
+  Buf<int64_t> d_tvar(ctx);
   if ((rc = d_tloc.upload(tloc.data(), n_slots))) return rc;
   if ((rc = d_tkeys.upload(tkeys.data(), (int64_t)tkeys.size()))) return rc;
+  if ((rc = d_tvar.upload(tvar.data(), (int64_t)tvar.size()))) return rc;
   const int sN = sbits * N;
   a.slot_tloc = d_tloc.p;
   a.sN = sN;
@@ -1584,6 +1656,7 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
     // stable sort of the chunk's contributions by x over the key bits in use
     const unsigned end_bit = (unsigned)(sN + bits_for(tstart[ch + 1] - tstart[ch]));
     sa.tkeys = d_tkeys.p + tstart[ch];
+    sa.tvar = d_tvar.p + 2 * tstart[ch];
     size_t tb = sort_bytes;
     if (direct) {
       FCG_HIP(rocprim::radix_sort_pairs(d_tmp.p, tb, d_ekey.p, d_skey.p, d_eval.p, d_sval.p,

@@ -136,3 +136,24 @@ def test_sparse_pt2_fp32_contraction(eng, six, case_name, thresh):
     assert np.array_equal(again, e32)
     assert np.abs(e32 - e64).max() < 1e-5 * np.abs(e64).max()
     assert abs(s32["n_contributions"] - s64["n_contributions"]) <= 1e-4 * s64["n_contributions"]
+
+
+def test_sparse_pt2_boxed_and_unboxed_space(eng, six):
+    """A space already cut into boxes by a plan and a freshly uploaded one
+    (read directly) give the same e2 and statistics bit for bit."""
+    from fermicg_amd import _lib
+    from fermicg_amd.cluster_ops import OP_NAMES
+    st = six.state
+    e0 = _e0(six, st)
+    ham, ops = eng.ham(six.H, 4), eng.ops(six.ops)
+    fresh = _lib.DeviceSpace.tpsci(eng.ctx, st)
+    boxed = _lib.DeviceSpace.tpsci(eng.ctx, st)
+    pl = _lib.Plan.create(eng.ctx, ham, ops, boxed)   # forms the boxes
+    del pl
+    kw = dict(nbody=4, h0_op=OP_NAMES.index("Hcmf"), thresh=1e-6)
+    e_fresh, s_fresh = _lib.tpsci_pt2_sparse(eng.ctx, ham, ops, fresh, st.coeffs, e0, **kw)
+    e_boxed, s_boxed = _lib.tpsci_pt2_sparse(eng.ctx, ham, ops, boxed, st.coeffs, e0, **kw)
+    print(e_fresh, e_boxed, flush=True)
+    assert np.array_equal(e_fresh, e_boxed)
+    for k in ("n_items", "n_contributions", "n_configs", "n_first_order"):
+        assert s_fresh[k] == s_boxed[k], k
