@@ -135,7 +135,8 @@ __device__ __forceinline__ double contract_v(const double *V, const double *cons
   return acc;
 }
 
-constexpr int SV = 2048;   // staged integrals of a TransferConfig group (doubles)
+constexpr int SV = 1536;   // staged integrals of a TransferConfig group (doubles)
+constexpr int NZMAX = 2048;  // nonzeros of one Fock pair listed for the fill (more: dense walk)
 constexpr int SVT = 64;    // staged terms per group
 
 // per (Fock pair, term): where each active cluster's Γ block starts, its
@@ -267,7 +268,7 @@ __device__ __forceinline__ double contract_v1(const double *V, const double *con
 }
 
 template <bool FILL>
-__global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
+__global__ __launch_bounds__(ST, 4) void k_sparse_rows(SparseArgs a) {
   __shared__ uint64_t sPI[MB][2], sPJ[MB][2];
   __shared__ uint64_t sBits[MB][MB / 64];
   __shared__ int32_t sCur[MB];
@@ -276,6 +277,8 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
   __shared__ double sV[FILL ? SV : 1];
   __shared__ int32_t sVoff[FILL ? SVT : 1];
   __shared__ TermPair sTP[FILL ? SVT : 1];
+  __shared__ uint16_t sNZ[FILL ? NZMAX : 1];  // the pair's nonzeros, (I << 8) | J
+  __shared__ int32_t sWave[ST / 64];
   const int tid = threadIdx.x;
   const int N = a.N;
   const int64_t fb = a.fb_base + blockIdx.x;
@@ -322,6 +325,42 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
     }
     __syncthreads();
     if (FILL) {
+      // list the pair's nonzeros (row-major, J ascending), so that the
+      // element loops below give every thread a nonzero instead of walking
+      // all nb x nk candidates; a pair with more than NZMAX walks them
+      int n_list = 0;
+      {
+        int cnt = 0;
+        if (tid < nb)
+          for (int k = 0; k < MB / 64; k++) cnt += __popcll(sBits[tid][k]);
+        const int lane = tid & 63, w = tid >> 6;
+        int x = cnt;
+        for (int off = 1; off < 64; off <<= 1) {
+          const int y = __shfl_up(x, off, 64);
+          if (lane >= off) x += y;
+        }
+        if (lane == 63) sWave[w] = x;
+        __syncthreads();
+        int pre = 0;
+        for (int k = 0; k < ST / 64; k++) {
+          if (k < w) pre += sWave[k];
+          n_list += sWave[k];
+        }
+        if (n_list <= NZMAX && tid < nb) {
+          int at = pre + x - cnt;
+          for (int k = 0; k < MB / 64; k++) {
+            uint64_t wb = sBits[tid][k];
+            while (wb) {
+              const int j = 64 * k + __builtin_ctzll(wb);
+              wb &= wb - 1;
+              sNZ[at++] = (uint16_t)((tid << 8) | j);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      const bool listed = n_list <= NZMAX;
+      const int n_walk = listed ? n_list : nb * nk;
       // the group's terms in chunks that fit the stage (integrals and
       // per-pair Γ block data in LDS); element values accumulate in K over
       // the chunks in term order
@@ -341,8 +380,10 @@ __global__ __launch_bounds__(ST) void k_sparse_rows(SparseArgs a) {
           for (int e = tid; e < sz; e += ST) sV[off + e] = a.ints[T.ioff + e];
         }
         __syncthreads();
-        for (int e = tid; e < nb * nk; e += ST) {
-          const int I = e / nk, J = e % nk;
+#pragma unroll 1
+        for (int e = tid; e < n_walk; e += ST) {
+          const int code = listed ? (int)sNZ[e] : (((e / nk) << 8) | (e % nk));
+          const int I = code >> 8, J = code & 255;
           const uint64_t w = sBits[I][J >> 6];
           if (!((w >> (J & 63)) & 1ull)) continue;
           int below = __popcll(w & ((1ull << (J & 63)) - 1ull));
