@@ -310,8 +310,9 @@ def compute_pt2_energy(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf"
     E0 = np.asarray(E0, dtype=np.float64).reshape(-1)
     if precision != "fp64" and method != "sparse":
         raise ValueError("a reduced-precision PT2 contraction needs method='sparse'")
+    extra = {} if precision == "fp64" else {"precision": precision}
     e2, stats = eng.pt2(ci, cluster_ops, clustered_ham, E0, nbody=nbody, H0=H0,
-                        thresh=thresh_foi, method=method, precision=precision)
+                        thresh=thresh_foi, method=method, **extra)
     if verbose:
         evar = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
         if method == "sparse":
