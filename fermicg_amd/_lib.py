@@ -38,6 +38,7 @@ EXPORTS = [
     "fcg_tpsci_pt2_sparse_ex",
     "fcg_plan_create_matrix_free",
     "fcg_serve_start", "fcg_serve_submit", "fcg_serve_poll", "fcg_serve_stop",
+    "fcg_space_diagonal",
 ]
 
 
@@ -154,6 +155,7 @@ def load():
     L.fcg_sigma_device.argtypes = [VP, VP, ctypes.c_int, VP, VP, ctypes.c_int]
     L.fcg_sigma_device_noalloc.argtypes = [VP, VP, ctypes.c_int, VP, VP, ctypes.c_int]
     L.fcg_diagonal.argtypes = [VP, VP, P(c_f64)]
+    L.fcg_space_diagonal.argtypes = [VP, VP, VP, VP, P(c_f64)]
     L.fcg_davidson.argtypes = [VP, VP, ctypes.c_int, P(c_f64), P(DavidsonOpts), P(c_f64),
                                P(c_f64), P(DavidsonStats)]
     L.fcg_last_sigma_time.argtypes = [VP, P(c_f64), P(ctypes.c_int)]
@@ -587,6 +589,16 @@ def open_sigma(ctx, ham, ops, bra, ket, c, nbody=4, thresh=0.0):
                                    float(thresh), _p(out, c_f64), _p(hit, ctypes.c_uint8)),
               "fcg_open_sigma")
     return out.T.copy(), hit[:nb].astype(bool)
+
+
+def space_diagonal(ctx, ham, ops, space):
+    """Diagonal of H over the configurations of a TPSCI space without a plan
+    (fcg_space_diagonal)."""
+    n = space.nvar()
+    out = np.zeros(max(n, 1))
+    ctx.check(ctx.L.fcg_space_diagonal(ctx.h, ham.h, ops.h, space.h, _p(out, c_f64)),
+              "fcg_space_diagonal")
+    return out[:n].copy()
 
 
 def set_plan_budget(nbytes):

@@ -7,6 +7,7 @@
     compute_pt2_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:10
     open_matvec_thread(ci_vector, cluster_ops, clustered_ham; …) tpsci_matvec_thread.jl:11
     compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt1_wavefunction.jl:11
+    compute_qdpt_energy(ci_vector, cluster_ops, clustered_ham; …) tpsci_pt2_energy.jl:311
     tpsci_ci(ci_vector, cluster_ops, clustered_ham; …)        tpsci.jl:39
 
 Same arguments, same meaning, same output layout (rows in the TPSCIstate's
@@ -102,6 +103,20 @@ class Engine:
             self._local[key] = hit
         return hit[1]
 
+    def space_diagonal(self, state, cluster_ops, clustered_ham):
+        """H's diagonal over the configurations of `state` without a plan
+        (fcg_space_diagonal)."""
+        sp = _lib.DeviceSpace.tpsci(self.ctx, state)
+        return _lib.space_diagonal(self.ctx, self.ham(clustered_ham), self.ops(cluster_ops), sp)
+
+    def pair_plan(self, bra_state, ket_state, cluster_ops, clustered_ham, nbody=4):
+        """Plan of H between two TPSCI spaces (rows: bra configurations,
+        columns: ket configurations), not cached."""
+        bra = _lib.DeviceSpace.tpsci(self.ctx, bra_state)
+        ket = _lib.DeviceSpace.tpsci(self.ctx, ket_state)
+        return _lib.Plan.create_pair(self.ctx, self.ham(clustered_ham, nbody),
+                                     self.ops(cluster_ops), bra, ket, nbody=nbody)
+
     def open_sigma(self, ket_state, target_state, cluster_ops, clustered_ham, nbody=4,
                    thresh=0.0):
         """Screened open matvec of `ket_state` onto every configuration of
@@ -160,8 +175,11 @@ def tps_ci_matvec(ci_vector, cluster_ops, clustered_ham, precision="fp64", engin
 
 
 def compute_diagonal(ci_vector, cluster_ops, clustered_ham, engine=None):
+    """Diagonal of `clustered_ham` over the configurations of `ci_vector`
+    (compute_diagonal, src/tpsci_outer.jl:694-715): the zero-TransferConfig
+    terms at bra = ket, one GPU thread per configuration, no plan built."""
     eng = engine or default_engine()
-    return eng.tpsci_plan(ci_vector, cluster_ops, clustered_ham).diagonal()
+    return eng.space_diagonal(ci_vector, cluster_ops, clustered_ham)
 
 
 def compute_s2(ci_vector, cluster_ops, engine=None):
@@ -431,6 +449,70 @@ def compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham, nbody=4, H0=
     if verbose:
         print(" PT1: %d configurations, e2 = %s" % (len(sig), np.array2string(e2, precision=8)))
     return e2, sig
+
+
+def compute_qdpt_energy(ci_vector_in, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
+                        thresh_foi=1e-8, prescreen=True, threaded=True, verbose=1, engine=None):
+    """Quasi-degenerate second-order energies (compute_qdpt_energy,
+    src/tpsci_pt2_energy.jl:311-361).
+
+    As in the reference: the state is orthonormalised; E0 = <r|H|r> with the
+    full Hamiltonian unless given; the first-order space X is the support of
+    the screened open matvec at thresh_foi (open_matvec_thread) minus the
+    state's own configurations (project_out!); with Hd the full-H diagonal
+    over X (compute_diagonal) and H_ax the Hamiltonian between the model
+    configurations a (those of the state) and X (build_full_H_parallel),
+    the model-space matrix H_model + H_ax pinv(E0[1] I - diag(Hd)) H_ax^T is
+    diagonalised, the denominator taking the first root's E0 for every
+    configuration and the pseudo-inverse zeroing exact zeros.  Returns its
+    eigenvalues, ascending (one per model configuration).
+
+    On the GPU: the open matvec, Hd (fcg_space_diagonal, no plan), H_ax (a
+    plan between X and the model space, applied to the identity) and
+    H_model (build_full_H).  The n_model^2 reduction over X and the
+    n_model-dimensional eigenproblem run on the host; the matrix is
+    symmetrised before eigvalsh (the reference's product is symmetric up to
+    rounding).  H0, prescreen and threaded are accepted for call
+    compatibility (the reference prints H0 and uses it nowhere else)."""
+    from .state import project_out_state
+    eng = engine or default_engine()
+    ci = _copy_state(ci_vector_in)
+    ci.set_vector(orthonormalize(ci.coeffs))
+    if E0 is None:
+        E0 = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
+    E0 = np.asarray(E0, np.float64).reshape(-1)
+    sig = open_matvec_thread(ci, cluster_ops, clustered_ham, nbody=nbody, thresh=thresh_foi,
+                             prescreen=prescreen, engine=eng)
+    sig = project_out_state(sig, ci)
+    n_model = len(ci)
+    h_model = build_full_H(ci, cluster_ops, clustered_ham, engine=eng)
+    if len(sig):
+        hd = compute_diagonal(sig, cluster_ops, clustered_ham, engine=eng)
+        pl = eng.pair_plan(sig, ci, cluster_ops, clustered_ham)
+        h_xa = np.zeros((len(sig), n_model))
+        for j0 in range(0, n_model, 16):
+            j1 = min(n_model, j0 + 16)
+            unit = np.zeros((n_model, j1 - j0))
+            unit[np.arange(j0, j1), np.arange(j1 - j0)] = 1.0
+            h_xa[:, j0:j1] = pl.sigma(unit)
+        del pl
+        den = E0[0] - hd
+        dinv = np.zeros_like(den)
+        nz = den != 0.0
+        dinv[nz] = 1.0 / den[nz]
+        h2 = (h_xa.T * dinv[None, :]) @ h_xa
+    else:
+        h2 = np.zeros((n_model, n_model))
+    h_tot = h2 + h_model
+    energies = np.linalg.eigvalsh(0.5 * (h_tot + h_tot.T))
+    compute_qdpt_energy.last_stats = {"n_first_order": len(sig), "n_model": n_model}
+    if verbose:
+        print(" QDPT: %d first-order configurations, %d model configurations"
+              % (len(sig), n_model))
+        print(" %5s %12s %12s" % ("Root", "E(0)", "E(2)"))
+        for r in range(min(len(E0), len(energies))):
+            print(" %5d %12.8f %12.8f" % (r + 1, E0[r], energies[r]))
+    return energies
 
 
 def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_foi=1e-6,

@@ -193,6 +193,14 @@ int fcg_sigma_device_noalloc(fcg_ctx_t ctx, fcg_plan_t plan, int R, const double
 /* Diagonal of H in the space's basis (replaces compute_diagonal,
  * src/tpsci_outer.jl:694-715). */
 int fcg_diagonal(fcg_ctx_t ctx, fcg_plan_t plan, double *hd);
+/* The same without a plan, for any TPSCI space (replaces compute_diagonal,
+ * src/tpsci_outer.jl:694-715, term by term: every term of the zero
+ * TransferConfig group contracted at bra = ket = each configuration).  For
+ * spaces whose off-diagonal Hamiltonian is not needed, such as the
+ * first-order space of compute_qdpt_energy (src/tpsci_pt2_energy.jl:344).
+ * hd: host (n_var), in the space's configuration order.  FCG_EINVAL for a
+ * space with Tucker factors. */
+int fcg_space_diagonal(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp, double *hd);
 
 /* Davidson for the lowest R eigenpairs (replaces tps_ci_davidson,
  * src/tpsci_outer.jl:364-453, and ci_solve, src/tucker_outer.jl:68-215,

@@ -267,3 +267,72 @@ class BstOracle(Oracle):
     def sigma(self, c, nthreads=0):
         s, idx = self.sigma_rows(c, np.arange(self.n_box), nthreads)
         return s
+
+
+class _Configs:
+    """Configurations of some Fock blocks in the TPSCIstate layout (fock,
+    fock_start, cfg), for Oracle()."""
+
+    def __init__(self, fock, fock_start, cfg):
+        self.fock = np.ascontiguousarray(fock, np.int16)
+        self.fock_start = np.ascontiguousarray(fock_start, np.int64)
+        self.cfg = np.ascontiguousarray(cfg, np.int16)
+        self.n_fock = self.fock.shape[0]
+
+    def __len__(self):
+        return self.cfg.shape[0]
+
+
+def qdpt(fops, fham, state, target, thresh_foi, E0=None, nthreads=0):
+    """compute_qdpt_energy restated (src/tpsci_pt2_energy.jl:311-361) for an
+    orthonormal state, on the C restatements above:
+      E0[r] = <r|H|r> unless given (:330-333);
+      X = configurations of `target` (every configuration of the open
+          matvec's target Fock blocks, first cluster fastest) that the
+          screened open matvec at thresh_foi reaches (open_matvec_thread,
+          :335-339), minus the state's own (project_out!, :340);
+      Hd = diagonal of H over X (compute_diagonal, :344);
+      H_ax = <a|H|x> for model configurations a (build_full_H_parallel,
+          :345), here the open matvec of each unit vector without screening;
+      H2 = H_ax pinv(E0[1] I - diag(Hd)) H_ax^T (:349-352), exact zeros of
+          the diagonal left out as pinv does;
+      eigenvalues of H2 + H_model (:354-357), ascending.
+    Returns (energies, number of configurations in X)."""
+    c = np.asarray(state.coeffs, np.float64)
+    orc = Oracle(fops, fham, state)
+    h_model = orc.full_H(nthreads)
+    if E0 is None:
+        E0 = np.einsum("ir,ij,jr->r", c, h_model, c)
+    E0 = np.asarray(E0, np.float64).reshape(-1)
+    focks = target.fock.reshape(target.n_fock, -1)
+    sizes = np.diff(target.fock_start)
+    _, hit = orc.open_sigma(c, focks, sizes, thresh=thresh_foi, nthreads=nthreads)
+    model = {(state.fock[b].tobytes(), state.cfg[i].tobytes())
+             for b in range(state.n_fock) for i in range(state.fock_start[b],
+                                                         state.fock_start[b + 1])}
+    blk = np.repeat(np.arange(target.n_fock), sizes)
+    keep = hit.copy()
+    for i in np.nonzero(hit)[0]:
+        if (target.fock[blk[i]].tobytes(), target.cfg[i].tobytes()) in model:
+            keep[i] = False
+    xi = np.nonzero(keep)[0]
+    if len(xi) == 0:
+        return np.linalg.eigvalsh(h_model), 0
+    starts = np.searchsorted(xi, target.fock_start)
+    xs = _Configs(target.fock, starts, target.cfg[xi])
+    hd = Oracle(fops, fham, xs).diagonal()
+    n_model = c.shape[0]
+    h_xa = np.zeros((len(xi), n_model))
+    for j0 in range(0, n_model, 64):   # the restatement takes at most 64 vectors
+        j1 = min(n_model, j0 + 64)
+        unit = np.zeros((n_model, j1 - j0))
+        unit[np.arange(j0, j1), np.arange(j1 - j0)] = 1.0
+        h_all, _ = orc.open_sigma(unit, focks, sizes, thresh=0.0, nthreads=nthreads)
+        h_xa[:, j0:j1] = h_all[xi]
+    den = E0[0] - hd
+    dinv = np.zeros_like(den)
+    nz = den != 0.0
+    dinv[nz] = 1.0 / den[nz]
+    h2 = (h_xa.T * dinv[None, :]) @ h_xa
+    h = h2 + h_model
+    return np.linalg.eigvalsh(0.5 * (h + h.T)), len(xi)

@@ -73,6 +73,9 @@ class OracleEngine:
             self._plans[key] = pl
         return pl
 
+    def space_diagonal(self, state, cluster_ops, clustered_ham):
+        return self.oracle(state, cluster_ops, clustered_ham).diagonal()
+
     def local_operator(self, clustered_ham, op_string):
         from fermicg_amd.operator import extract_1body_operator
         key = (id(clustered_ham), op_string)

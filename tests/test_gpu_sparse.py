@@ -97,6 +97,8 @@ def test_sparse_sigma_fp64(eng, name):
     assert rel < 1e-12
     hd = compute_diagonal(st, p.cluster_ops, p.clustered_ham, engine=eng)
     assert np.abs(hd - orc.diagonal()).max() < 1e-12 * np.abs(hd).max()
+    hp = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham).diagonal()
+    assert np.abs(hp - hd).max() < 1e-12 * np.abs(hd).max()
 
 
 # BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2

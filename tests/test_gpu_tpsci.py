@@ -48,6 +48,9 @@ def test_diagonal(eng, small):
     p, st, orc = small
     hd = compute_diagonal(st, p.cluster_ops, p.clustered_ham, engine=eng)
     assert np.abs(hd - orc.diagonal()).max() < 1e-12 * np.abs(hd).max()
+    # the plan's stored diagonal (fcg_diagonal, used by Davidson) agrees
+    hp = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham).diagonal()
+    assert np.abs(hp - hd).max() < 1e-12 * np.abs(hd).max()
 
 
 # BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2
