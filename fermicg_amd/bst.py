@@ -34,6 +34,60 @@ class FlatBST:
     rank: np.ndarray      # int16 [n_box, N]
     foff: np.ndarray      # int64 [n_box, N]  (-1 = identity)
     factors: np.ndarray   # float64 pool (column-major range x rank)
+    var_off: np.ndarray = None  # int64 [n_box + 1] first variable of each box
+
+
+# variables per box of the device engine: a larger Tucker block is cut into
+# boxes along its slowest clusters' ranks (contiguous runs of its variables)
+BOX_MAX = 64
+
+
+def _split_block(rgs, facs):
+    """Boxes of at most BOX_MAX variables covering one block in its variable
+    order (first cluster fastest): (ranges, factors) per box.  The clusters
+    before k keep the whole block's rank, cluster k is cut into runs of
+    columns, every later cluster into single columns; an identity factor is
+    cut by sub-ranges, an explicit factor by column subsets over the same
+    range."""
+    N = len(rgs)
+    ranks = [(f.shape[1] if f is not None else hi - lo + 1) for (lo, hi), f in zip(rgs, facs)]
+    if int(np.prod(ranks)) <= BOX_MAX:
+        return [(list(rgs), list(facs))]
+    k, pre = 0, 1
+    while pre * ranks[k] <= BOX_MAX:
+        pre *= ranks[k]
+        k += 1
+    chunk = BOX_MAX // pre
+
+    def cut(c, a, n):
+        (lo, hi), f = rgs[c], facs[c]
+        if f is None:
+            return (lo + a, lo + a + n - 1), None
+        return (lo, hi), np.asfortranarray(f[:, a:a + n])
+
+    out = []
+    slow = [0] * N
+    while True:
+        for a in range(0, ranks[k], chunk):
+            n = min(chunk, ranks[k] - a)
+            br, bf = list(rgs[:k]), list(facs[:k])
+            r, f = cut(k, a, n)
+            br.append(r)
+            bf.append(f)
+            for c in range(k + 1, N):
+                r, f = cut(c, slow[c], 1)
+                br.append(r)
+                bf.append(f)
+            out.append((br, bf))
+        c = k + 1
+        while c < N:
+            slow[c] += 1
+            if slow[c] < ranks[c]:
+                break
+            slow[c] = 0
+            c += 1
+        if c >= N:
+            return out
 
 
 class BSTstate:
@@ -87,16 +141,23 @@ class BSTstate:
         self.coeffs = np.zeros_like(self.coeffs)
 
     def flat(self):
+        """The device layout: one box per block of at most BOX_MAX variables,
+        larger blocks cut into several boxes (_split_block)."""
         if self._flat is not None:
             return self._flat
         N = self.N
-        nb = len(self.blocks)
+        boxes = []
+        for fi, rgs, facs in self.blocks:
+            for br, bf in _split_block(rgs, facs):
+                boxes.append((fi, br, bf))
+        nb = len(boxes)
         lo = np.zeros((nb, N), np.int16)
         hi = np.zeros((nb, N), np.int16)
         rk = np.zeros((nb, N), np.int16)
         fo = -np.ones((nb, N), np.int64)
+        var_off = np.zeros(nb + 1, np.int64)
         chunks, at = [], 0
-        for b, (fi, rgs, facs) in enumerate(self.blocks):
+        for b, (fi, rgs, facs) in enumerate(boxes):
             for c in range(N):
                 lo[b, c], hi[b, c] = rgs[c]
                 F = facs[c]
@@ -108,9 +169,10 @@ class BSTstate:
                     flat = np.asarray(F, np.float64).ravel(order="F")
                     chunks.append(flat)
                     at += flat.size
+            var_off[b + 1] = var_off[b] + int(np.prod(rk[b].astype(np.int64)))
         self._flat = FlatBST(N, len(self.focks), np.array(self.focks, np.int16).reshape(-1, N, 2),
-                             nb, np.array([b[0] for b in self.blocks], np.int32), lo, hi, rk, fo,
-                             np.concatenate(chunks) if chunks else np.zeros(0))
+                             nb, np.array([b[0] for b in boxes], np.int32), lo, hi, rk, fo,
+                             np.concatenate(chunks) if chunks else np.zeros(0), var_off)
         return self._flat
 
     def signature(self):

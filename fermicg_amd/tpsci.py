@@ -117,6 +117,32 @@ class Engine:
         return _lib.Plan.create_pair(self.ctx, self.ham(clustered_ham, nbody),
                                      self.ops(cluster_ops), bra, ket, nbody=nbody)
 
+    def bst_pair_sigma(self, bra, ket, cluster_ops, clustered_ham, nbody=4):
+        """sigma over the blocks of BSTstate `bra` of H applied to BSTstate
+        `ket` (n_bra x R), through a pair plan built for the call."""
+        sb = _lib.DeviceSpace.bst(self.ctx, bra.flat())
+        sk = _lib.DeviceSpace.bst(self.ctx, ket.flat())
+        pl = _lib.Plan.create_pair(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
+                                   sb, sk, nbody=nbody)
+        c = ket.coeffs
+        out = np.zeros((len(bra), c.shape[1]))
+        for r0 in range(0, c.shape[1], 16):
+            r1 = min(c.shape[1], r0 + 16)
+            out[:, r0:r1] = np.asarray(pl.sigma(c[:, r0:r1])).reshape(len(bra), r1 - r0)
+        del pl
+        return out
+
+    def bst_ci_solve(self, vec, cluster_ops, clustered_ham, **kw):
+        """bst.ci_solve on this engine; the plan it caches for the space is
+        freed afterwards (the outer loop's spaces change every call)."""
+        from .bst import ci_solve
+        before = set(self._plans)
+        try:
+            return ci_solve(vec, cluster_ops, clustered_ham, engine=self, **kw)
+        finally:
+            for k in [k for k in self._plans if k not in before]:
+                del self._plans[k]
+
     def open_sigma(self, ket_state, target_state, cluster_ops, clustered_ham, nbody=4,
                    thresh=0.0):
         """Screened open matvec of `ket_state` onto every configuration of

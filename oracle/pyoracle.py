@@ -228,7 +228,7 @@ class BstOracle(Oracle):
                           _p(k(fham.ints, np.float64), f64), fham.n_tc,
                           _p(k(fham.tc, np.int16), i16), _p(k(fham.tc_start, np.int32), i32))
         fl = bst.flat()
-        self.var_off = k(bst.var_offsets(), np.int64)
+        self.var_off = k(fl.var_off, np.int64)
         self.nvar = int(self.var_off[-1])
         self.n_box = fl.n_box
         self.bst = OrcBst(fl.n_fock, _p(k(fl.fock, np.int16), i16), fl.n_box,

@@ -110,6 +110,44 @@ class OracleEngine:
                              thresh=thresh, nthreads=self.nthreads)
         return e2, {"n_target_fock": n_jobs, "n_first_order": -1, "seconds": 0.0}
 
+    def bst_pair_sigma(self, bra, ket, cluster_ops, clustered_ham, nbody=4):
+        """H c_ket on bra's blocks: the C restatement of build_sigma! over the
+        union of both block lists, ket coefficients placed after zeros."""
+        from fermicg_amd.bst import BSTstate
+        from oracle.pyoracle import BstOracle
+        u = BSTstate(bra.N, ket.coeffs.shape[1])
+        for st in (bra, ket):
+            for fi, rgs, facs in st.blocks:
+                u.add_block(st.focks[fi], rgs, facs)
+        n_bra = len(bra)
+        u.coeffs = np.vstack([np.zeros((n_bra, ket.coeffs.shape[1])), ket.coeffs])
+        fo, fh = self._flat_inputs(cluster_ops, clustered_ham, nbody)
+        orc = BstOracle(fo, fh, u)
+        n_bra_boxes = len(bra.flat().box_fock)
+        s, idx = orc.sigma_rows(u.coeffs, np.arange(n_bra_boxes), nthreads=self.nthreads)
+        assert len(idx) == n_bra
+        return s
+
+    def bst_ci_solve(self, vec, cluster_ops, clustered_ham, conv_thresh=1e-5, max_iter=40,
+                     max_ss_vecs=12, lindep_thresh=1e-10, nbody=4):
+        """ci_solve's result from the dense H of the space (columns from the
+        restated sigma), in the pseudo-canonical basis ci_solve returns."""
+        from fermicg_amd.bst import form_1body_operator_diagonal
+        rot, _ = form_1body_operator_diagonal(vec, cluster_ops, H0="Hcmf", pseudo_canon=True)
+        n, R = len(rot), rot.coeffs.shape[1]
+        H = np.zeros((n, n))
+        probe = rot.copy_structure(R=1)
+        for j0 in range(0, n, 64):
+            j1 = min(n, j0 + 64)
+            probe.R = j1 - j0
+            probe.coeffs = np.zeros((n, j1 - j0))
+            probe.coeffs[np.arange(j0, j1), np.arange(j1 - j0)] = 1.0
+            H[:, j0:j1] = self.bst_pair_sigma(rot, probe, cluster_ops, clustered_ham, nbody)
+        w, v = np.linalg.eigh(0.5 * (H + H.T))
+        out = rot.copy_structure(R=R)
+        out.set_vector(v[:, :R])
+        return w[:R].copy(), out
+
     def drop_plans(self, keep=()):
         for k in [k for k in self._plans if k not in keep]:
             del self._plans[k]
