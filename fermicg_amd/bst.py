@@ -395,6 +395,13 @@ def ci_solve(ci_vector, cluster_ops, clustered_ham, conv_thresh=1e-5, max_ss_vec
                               max_ss_vecs=max_ss_vecs, lindep_thresh=lindep_thresh,
                               precision=precision, refresh_k=refresh_k, precond=precond,
                               precond_diag=hd)
+    if not stats["converged"]:
+        # the reference prints its residuals and goes on; so does this, loudly
+        import warnings
+        warnings.warn("ci_solve: not converged after %d iterations (residuals %s, conv_thresh %g)"
+                      % (stats["iterations"], np.array2string(np.asarray(stats["residual"]),
+                                                              precision=3), conv_thresh),
+                      RuntimeWarning, stacklevel=2)
     out = vec.copy_structure()
     out.set_vector(v)
     out.davidson_stats = stats

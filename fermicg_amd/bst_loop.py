@@ -24,7 +24,6 @@ exact sigma of each target block (every term at once, on the GPU) is
 compressed once at thresh, which keeps at least what the term-wise
 truncation keeps.
 """
-import warnings
 from collections import OrderedDict
 
 import numpy as np
@@ -336,13 +335,6 @@ def ci_solve(v, cluster_ops, clustered_ham, conv_thresh=1e-5, max_iter=40, max_s
     e, st = eng.bst_ci_solve(orthonormalize(v).to_bst(), cluster_ops, clustered_ham,
                              conv_thresh=conv_thresh, max_iter=max_iter,
                              max_ss_vecs=max_ss_vecs, lindep_thresh=lindep_thresh, nbody=nbody)
-    stats = getattr(st, "davidson_stats", None)
-    if stats is not None and not stats["converged"]:
-        # the reference prints its residuals and goes on; so does this, loudly
-        warnings.warn("ci_solve: not converged after %d iterations (residuals %s, conv_thresh %g)"
-                      % (stats["iterations"], np.array2string(np.asarray(stats["residual"]),
-                                                              precision=3), conv_thresh),
-                      RuntimeWarning, stacklevel=2)
     return np.asarray(e), TState.from_bst(st)
 
 
