@@ -267,8 +267,26 @@ __device__ __forceinline__ double contract_v1(const double *V, const double *con
   return acc;
 }
 
-template <bool FILL>
-__global__ __launch_bounds__(ST, 4) void k_sparse_rows(SparseArgs a) {
+// Launch shapes of the join and fill: workgroup threads, the largest Fock
+// block (configurations), staged integrals (doubles) and terms per group
+// chunk, and the longest listed nonzero list per Fock pair.  Spaces whose
+// Fock blocks hold at most 64 configurations take one wavefront per bra
+// block with small stages: a Fock pair has ~90 nonzeros, which leave most of
+// a 256-thread workgroup idle, and the smaller LDS footprint (10 KB against
+// 38 KB) fits four times as many workgroups per CU.
+template <int ST_, int MB_, int SV_, int SVT_, int NZ_, int WPE_>
+struct RowsCfg {
+  static constexpr int ST = ST_, MB = MB_, SV = SV_, SVT = SVT_, NZ = NZ_, WPE = WPE_;
+};
+// WPE: the waves per SIMD asked of the register allocator (the small
+// variant then fits 128 VGPRs without scratch: four waves per SIMD, which its
+// 10 KB of LDS also allows)
+using BigRows = RowsCfg<ST, MB, SV, SVT, NZMAX, 4>;
+using SmallRows = RowsCfg<64, 64, 512, 16, 1024, 5>;
+
+template <bool FILL, class C>
+__global__ __launch_bounds__(C::ST, C::WPE) void k_sparse_rows(SparseArgs a) {
+  constexpr int ST = C::ST, MB = C::MB, SV = C::SV, SVT = C::SVT, NZMAX = C::NZ;
   __shared__ uint64_t sPI[MB][2], sPJ[MB][2];
   __shared__ uint64_t sBits[MB][MB / 64];
   __shared__ int32_t sCur[MB];
@@ -739,13 +757,18 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
         return FCG_EOVERFLOW;
       }
   }
+  int64_t max_block = 0;
   for (int64_t f = 0; f < sp->n_fock; f++) {
     if (fstart[f + 1] > FCG_SPARSE_MAX_BLOCK) {
       ctx->err = "sparse plan: more than 256 configurations in a Fock block";
       return FCG_EOVERFLOW;
     }
+    max_block = std::max(max_block, fstart[f + 1]);
     fstart[f + 1] += fstart[f];
   }
+  const bool small = max_block <= SmallRows::MB;
+  const int cfg_sv = small ? SmallRows::SV : BigRows::SV;
+  const int cfg_svt = small ? SmallRows::SVT : BigRows::SVT;
   std::vector<int64_t> fp_ptr;
   std::vector<int32_t> fp_fk, fp_g;
   DBuf<int64_t> d_fp_ptr;
@@ -800,7 +823,7 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
     const int t1 = h->tc_start[g + 1];
     for (int c0 = h->tc_start[g]; c0 < t1;) {
       int off = 0, t = c0;
-      for (; t < t1 && t - c0 < SVT; t++) {
+      for (; t < t1 && t - c0 < cfg_svt; t++) {
         const SparseTerm &T = terms[t];
         int sz = 0;
         if (T.nbody > 1) {
@@ -808,11 +831,11 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
           for (int k = 0; k < T.nbody; k++) sz *= T.dims[k];
         }
         if (T.nbody > nbody) sz = 0;
-        if (sz > SV) {  // larger than the stage: integrals read from global memory
+        if (sz > cfg_sv) {  // larger than the stage: integrals read from global memory
           voff[t] = -2;
           continue;
         }
-        if (off + sz > SV) break;
+        if (off + sz > cfg_sv) break;
         voff[t] = T.nbody > nbody ? -1 : off;
         off += sz;
       }
@@ -856,7 +879,12 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
     a.fb_base = f0;
     const unsigned nblk = (unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0);
-    hipLaunchKernelGGL(k_sparse_rows<false>, dim3(nblk), dim3(ST), 0, ctx->stream, a);
+    if (small)
+      hipLaunchKernelGGL((k_sparse_rows<false, SmallRows>), dim3(nblk), dim3(SmallRows::ST), 0,
+                         ctx->stream, a);
+    else
+      hipLaunchKernelGGL((k_sparse_rows<false, BigRows>), dim3(nblk), dim3(BigRows::ST), 0,
+                         ctx->stream, a);
   }
   FCG_HIP(hipGetLastError(), ctx);
   std::vector<int32_t> nnz(n_row);
@@ -938,7 +966,11 @@ int fcg_create_sparse_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, 
   for (int64_t f0 = 0; f0 < sp->n_fock; f0 += FB_CHUNK) {
     a.fb_base = f0;
     const unsigned nblk = (unsigned)std::min<int64_t>(FB_CHUNK, sp->n_fock - f0);
-    hipLaunchKernelGGL(k_sparse_rows<true>, dim3(nblk), dim3(ST), 0, s, a);
+    if (small)
+      hipLaunchKernelGGL((k_sparse_rows<true, SmallRows>), dim3(nblk), dim3(SmallRows::ST), 0, s,
+                         a);
+    else
+      hipLaunchKernelGGL((k_sparse_rows<true, BigRows>), dim3(nblk), dim3(BigRows::ST), 0, s, a);
   }
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = fcg_launch_convert(pl, s);
