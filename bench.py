@@ -16,7 +16,7 @@ Davidson solves (FP64 vs mixed with FP64 residual refresh at restarts, with
 and without the one-time plan build) and their |dE| against FP64 are
 reported beside it.
 
-Prints ONE JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+Prints ONE JSON line.  --gpus N with N > 1 exits: the library runs on one GPU.
 """
 import argparse
 import json
@@ -85,9 +85,9 @@ def main():
                     help="also run a plain 1 GiB device copy (known bytes) to scale PMC counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the sigma build would shard); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the library runs the sigma build on one "
+              "device (bra Fock blocks would be sharded across GPUs with an all-gather of "
+              "c before each sigma); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.cpu_rows <= 0:

@@ -17,7 +17,7 @@ ci_solve, Tucker bookkeeping on the host.
   (tests/golden/bst_h12_oracle.json, 1e-9 Eh), and the energy within
   3e-5 Eh below the reference's -18.329455496968325.  That gap is the one
   difference of method: the reference truncates each term's sigma block
-  (and the 3- and 4-body Γ factors) before adding, this build compresses
+  (and the 3- and 4-body Γ factors) before adding, bst_loop compresses
   the exact sigma block once, which keeps a larger first-order space.
 """
 import json

@@ -6,7 +6,7 @@ seven slices |Γ[p, (q,) :, :]| of the A, B, AA, aa, Aa, Bb and Ab
 operators to 1e-5 (absolute values: the eigenvectors' signs are
 arbitrary).  The slices are kept as data in tests/golden/tdm_h6.json
 (written from that file by tests/golden/extract_tdm_values.py); here the
-same slices of this build's ClusterOps producer (local FCI per sector,
+same slices of fermicg_amd's ClusterOps producer (local FCI per sector,
 cluster_ops.compute_cluster_ops) must match.  A leading index over several
 orbitals is flattened column-major, p + n q, as the reference's arrays are
 laid out.
