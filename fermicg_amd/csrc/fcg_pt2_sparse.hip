@@ -1653,8 +1653,11 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
     FCG_HIP(hipStreamSynchronize(s), ctx);
     emit_s += seconds_since(tf);
     const auto ts = std::chrono::steady_clock::now();
-    // stable sort of the chunk's contributions by x over the key bits in use
-    const unsigned end_bit = (unsigned)(sN + bits_for(tstart[ch + 1] - tstart[ch]));
+    // stable sort of the chunk's contributions by their state bits alone: the
+    // emission wrote them grouped by target Fock block (slots in target
+    // order), so a stable sort keeps every (target, states) group contiguous
+    // and the target bits above sN need no passes of their own
+    const unsigned end_bit = (unsigned)sN;
     sa.tkeys = d_tkeys.p + tstart[ch];
     sa.tvar = d_tvar.p + 2 * tstart[ch];
     size_t tb = sort_bytes;
