@@ -63,8 +63,11 @@ def host_cpu():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 30; 5 for pt2 and 1 for pt2_12c, whose steps "
+                         "are whole PT2 energies)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 3; 1 for pt2 and pt2_12c)")
     ap.add_argument("--workload", default="sparse",
                     choices=["bst", "tpsci", "sparse", "serve", "pt2", "pt2_12c"])
     ap.add_argument("--requests", type=int, default=32, help="serve workload: requests")
@@ -90,6 +93,10 @@ def main():
               "c before each sigma); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.steps is None:
+        args.steps = {"pt2": 5, "pt2_12c": 1}.get(args.workload, 30)
+    if args.warmup is None:
+        args.warmup = 1 if args.workload in ("pt2", "pt2_12c") else 3
     if args.cpu_rows <= 0:
         args.cpu_rows = 100000 if args.workload == "sparse" else 2000
     if args.workload == "serve":
