@@ -479,10 +479,21 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   const bool store = fused == nullptr;
   int rc = fcg_build_host_plan(h, o, sp, ksp, nbody, store, ctx->build_class_cap, &hp, &ctx->err);
   if (rc) return rc;
+  // fused sigma: each pair's products get a slot of db x R doubles
+  std::vector<int64_t> fyoff;
+  if (fused) {
+    fyoff.resize(hp.pair_B.size() + 1);
+    fyoff[0] = 0;
+    for (size_t q = 0; q < hp.pair_B.size(); q++) {
+      const int32_t B = hp.pair_B[q];
+      fyoff[q + 1] = fyoff[q] + (sp->var_off[B + 1] - sp->var_off[B]) * fused->R;
+    }
+  }
   int64_t reserved = 0;
   {
-    // the pair and task tables live on the device during the build
-    const int64_t bytes = (fused ? 0
+    // the pair and task tables (and a fused sigma's pair slots) live on the
+    // device during the build
+    const int64_t bytes = (fused ? 8 * (fyoff.back() + (int64_t)fyoff.size() + sp->n_box + 1)
                                  : fcg_plan_bytes(hp.n_kel, (int64_t)hp.colidx.size(), sp->n_box,
                                                   precisions | (1 << FCG_FP64))) +
                           (int64_t)hp.pair_B.size() * 44 + (int64_t)hp.tasks.size() * sizeof(TaskDev);
@@ -518,8 +529,25 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
   pl->n_small = hp.n_small;
   pl->n_mid = hp.n_mid;
   pl->holds_blocks = store;
+  PTRY(dev_upload(ctx, &pl->d_row_ptr, hp.row_ptr.data(), sp->n_box + 1));
+  FusedSigma fz{};
+  struct FusedSlots {  // freed on every return (after the build's synchronisation)
+    fcg_ctx *ctx;
+    double *part = nullptr;
+    int64_t *yoff = nullptr;
+    ~FusedSlots() {
+      dev_free(ctx, part);
+      dev_free(ctx, yoff);
+    }
+  } slots{ctx};
+  if (fused) {
+    fz = *fused;
+    PTRY(dev_alloc(ctx, &slots.part, std::max<int64_t>(fyoff.back(), 1)));
+    PTRY(dev_upload(ctx, &slots.yoff, fyoff.data(), (int64_t)fyoff.size()));
+    fz.part = slots.part;
+    fz.yoff = slots.yoff;
+  }
   if (store) {  // the apply's layout: only for a plan that keeps its blocks
-    PTRY(dev_upload(ctx, &pl->d_row_ptr, hp.row_ptr.data(), sp->n_box + 1));
     PTRY(dev_upload(ctx, &pl->d_pair_koff, hp.pair_koff.data(), pl->n_pair));
     PTRY(dev_upload(ctx, &pl->d_pair_col, hp.pair_col.data(), pl->n_pair));
     PTRY(dev_upload(ctx, &pl->d_diag_koff, hp.diag_koff.data(), sp->n_box));
@@ -547,7 +575,8 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
     }
   }
   if (pl->n_pair > 0) {
-    hipError_t e = fcg_launch_build(pl, h, o, ctx->stream, fused);
+    hipError_t e = fcg_launch_build(pl, h, o, ctx->stream, fused ? &fz : nullptr);
+    if (e == hipSuccess && fused) e = fcg_launch_fused_sum(pl, &fz, pl->d_row_ptr, ctx->stream);
     if (e == hipSuccess && store) e = fcg_launch_convert(pl, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {

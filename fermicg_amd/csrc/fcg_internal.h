@@ -321,6 +321,11 @@ struct FusedSigma {
   // that some term's nonzero element reached after screening (the keys the
   // reference's open matvec inserts)
   uint8_t *hit = nullptr;
+  // filled by fcg_create_plan: each pair's products (db x R, row-major by
+  // root) at part + yoff[pair], summed per bra box in pair order afterwards
+  // (fixed order: the result does not depend on how workgroups are scheduled)
+  double *part = nullptr;
+  const int64_t *yoff = nullptr;
 };
 int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_space *ksp,
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes = INT64_MAX,
@@ -338,6 +343,9 @@ void fcg_rank1_free(fcg_ctx *ctx, fcg_plan *pl);
 hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops *o,
                             hipStream_t s, const FusedSigma *fused = nullptr);
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s);
+// fused sigma: sigma[bra var, r] += the bra box's pair products in pair order
+hipError_t fcg_launch_fused_sum(const fcg_plan *pl, const FusedSigma *f, const int64_t *row_ptr,
+                                hipStream_t s);
 // vm / vr: apply variants for multi-row and one-row boxes; parts: bit 0
 // multi-row boxes, bit 1 one-row boxes
 // ci: the interleave buffer (at least R * ket n_var * 8 bytes)

@@ -52,6 +52,9 @@ struct BuildArgs {
   const int64_t *bvar;
   // bra variables reached by a nonzero element that passed the screen
   uint8_t *hit;
+  // fused sigma: this pair's db x fR products go to fpart + fyoff[pair]
+  double *fpart;
+  const int64_t *fyoff;
 };
 
 __device__ __forceinline__ double fac(const double *F, int64_t off, int nrg, int x, int i) {
@@ -347,39 +350,43 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       if ((hm >> k) & 1u) a.hit[bv0 + (tid + NT * k) % db] = 1;
   }
   if (a.fsig) {
-    // fused sigma: the block's products summed per bra row in LDS (the chain
-    // buffers are free now; the ket box's c staged beside them), a thread's
-    // run of elements in one row added once, then one global add per
-    // (row, root)
+    // fused sigma: y[row, r] = sum_q block[row, q] c[q, r] in column order.
+    // The block's columns pass through the free Γ stage (sG) in chunks; each
+    // (row, root) entry of y belongs to one thread, which adds the chunk's
+    // products in order, so y is the same however the threads are
+    // scheduled.  The pair's y goes to its own slot (no atomics); the bra
+    // box's slots are summed in pair order by k_fused_sum.
     double *sY = sA, *sC = sA + LX;
     const int fR = a.fR, nY = db * fR;
     const int64_t kv0 = a.kvar[K];
     for (int e = tid; e < nY; e += NT) sY[e] = 0.0;
     for (int e = tid; e < dk * fR; e += NT)
       sC[e] = a.fc[(int64_t)(e / dk) * a.nk_var + kv0 + e % dk];
-    __syncthreads();
-    for (int rr = 0; rr < fR; rr++) {
-      const double *cq = sC + rr * dk;
-      int r = r0, q = q0, rc = r0;
-      double run = 0.0;
-      for (int k = 0; k < EPT; k++) {
-        if (q >= dk) break;
-        if (r != rc) {
-          if (run != 0.0) atomicAdd(&sY[rc * fR + rr], run);
-          run = 0.0;
-          rc = r;
+    const int CH = LG / db;  // columns per chunk (LG >= FCG_MAX_BOX for the fused classes)
+    for (int c0 = 0; c0 < dk; c0 += CH) {
+      const int c1 = min(dk, c0 + CH);
+      __syncthreads();
+      {
+        int r = r0, q = q0;
+        for (int k = 0; k < EPT; k++) {
+          if (q >= dk) break;
+          if (q >= c0 && q < c1) sG[r + db * (q - c0)] = acc[k];
+          r += dr;
+          q += dq;
+          if (r >= db) { r -= db; q++; }
         }
-        run += acc[k] * cq[q];
-        r += dr;
-        q += dq;
-        if (r >= db) { r -= db; q++; }
       }
-      if (run != 0.0) atomicAdd(&sY[rc * fR + rr], run);
+      __syncthreads();
+      for (int e = tid; e < nY; e += NT) {
+        const int r = e / fR;
+        const double *cq = sC + (e % fR) * dk;
+        double y = sY[e];
+        for (int q = c0; q < c1; q++) y += sG[r + db * (q - c0)] * cq[q];
+        sY[e] = y;
+      }
     }
-    __syncthreads();
-    const int64_t bv0 = a.bvar[B];
-    for (int e = tid; e < nY; e += NT)
-      atomicAdd(&a.fsig[(int64_t)(e % a.fR) * a.nb_var + bv0 + e / a.fR], sY[e]);
+    double *slot = a.fpart + a.fyoff[p];
+    for (int e = tid; e < nY; e += NT) slot[e] = sY[e];
     return;
   }
   double *Kout = a.K + a.pair_koff[p];
@@ -815,6 +822,8 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
   a.fsig = fused ? fused->sigma : nullptr;
   a.fR = fused ? fused->R : 0;
   a.hit = fused ? fused->hit : nullptr;
+  a.fpart = fused ? fused->part : nullptr;
+  a.fyoff = fused ? fused->yoff : nullptr;
   a.nb_var = sp->nvar;
   a.nk_var = kp->nvar;
   a.bvar = sp->d_var;
@@ -846,6 +855,33 @@ hipError_t fcg_launch_build(const fcg_plan *pl, const fcg_ham *h, const fcg_ops 
     base = end;
   }
   return hipSuccess;
+}
+
+// sigma[bra var, r] += sum over the bra box's pairs (pair order, i.e. ket box
+// order) of their fused products; one workgroup per bra box
+__global__ __launch_bounds__(256) void k_fused_sum(const int64_t *__restrict__ row_ptr,
+                                                   const int64_t *__restrict__ yoff,
+                                                   const int64_t *__restrict__ bvar,
+                                                   const double *__restrict__ part, int fR,
+                                                   int64_t nb_var, double *__restrict__ sigma) {
+  const int64_t B = blockIdx.x;
+  const int64_t v0 = bvar[B];
+  const int nY = (int)(bvar[B + 1] - v0) * fR;
+  const int64_t q0 = row_ptr[B], q1 = row_ptr[B + 1];
+  for (int e = threadIdx.x; e < nY; e += blockDim.x) {
+    double y = 0.0;
+    for (int64_t q = q0; q < q1; q++) y += part[yoff[q] + e];
+    sigma[(int64_t)(e % fR) * nb_var + v0 + e / fR] += y;
+  }
+}
+
+hipError_t fcg_launch_fused_sum(const fcg_plan *pl, const FusedSigma *f, const int64_t *row_ptr,
+                                hipStream_t s) {
+  const fcg_space *sp = pl->sp;
+  if (sp->n_box == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fused_sum, dim3((unsigned)sp->n_box), dim3(128), 0, s, row_ptr, f->yoff,
+                     sp->d_var, f->part, f->R, sp->nvar, f->sigma);
+  return hipGetLastError();
 }
 
 hipError_t fcg_launch_convert(const fcg_plan *pl, hipStream_t s) {

@@ -88,3 +88,25 @@ def test_open_sigma_pair_plan(eng):
     print("pair plan", pl.stats(), flush=True)
     assert s.shape == ref.shape
     assert np.abs(s - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_fused_sigma_fixed_order(eng):
+    # the fused sigma (PT2's first-order sigma and the open sigma) sums each
+    # bra row over its block pairs in pair order, with no float atomics: PT2
+    # energies and the open sigma are bit-identical from run to run, and the
+    # open sigma agrees with the stored plan's sigma
+    from fermicg_amd.workloads import pt2_medium
+    p, st = pt2_medium(M=7)
+    print("pt2_medium M=7 n_var", len(st), flush=True)
+    e2s = [compute_pt2_energy(st, p.cluster_ops, p.clustered_ham, thresh_foi=1e-9, verbose=0,
+                              engine=eng) for _ in range(3)]
+    print("e2", e2s[0], flush=True)
+    for e in e2s[1:]:
+        assert np.array_equal(e, e2s[0])
+    sp = _lib.DeviceSpace.tpsci(eng.ctx, st)
+    ham, ops = eng.ham(p.clustered_ham), eng.ops(p.cluster_ops)
+    sig = [_lib.open_sigma(eng.ctx, ham, ops, sp, sp, st.coeffs)[0] for _ in range(3)]
+    for s in sig[1:]:
+        assert np.array_equal(s, sig[0])
+    ref = _lib.Plan.create(eng.ctx, ham, ops, sp).sigma(st.coeffs)
+    assert np.abs(sig[0] - ref).max() < 1e-12 * np.abs(ref).max()
