@@ -2,8 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <sys/mman.h>
+
 #include <cstdint>
+#include <cstdlib>
 #include <memory>
+#include <new>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -193,8 +197,12 @@ struct fcg_plan {
 // Allocator whose resize() leaves new trivial elements unwritten (default-
 // instead of value-initialised): the host plan's large tables are written in
 // full by parallel passes, so a serial zero fill first would only cost time.
+// Tables of 2 MiB and more are 2 MiB-aligned and marked for transparent huge
+// pages (hosts that enable them on request), so first touch faults in 2 MiB
+// pages instead of 4 KiB ones.
 template <class T>
 struct DefaultInitAlloc : std::allocator<T> {
+  static constexpr size_t kHuge = (size_t)2 << 20;
   template <class U>
   struct rebind {
     using other = DefaultInitAlloc<U>;
@@ -202,6 +210,19 @@ struct DefaultInitAlloc : std::allocator<T> {
   DefaultInitAlloc() = default;
   template <class U>
   DefaultInitAlloc(const DefaultInitAlloc<U> &) noexcept {}
+  T *allocate(size_t n) {
+    const size_t bytes = n * sizeof(T);
+    if (bytes < kHuge) return std::allocator<T>::allocate(n);
+    const size_t sz = (bytes + kHuge - 1) / kHuge * kHuge;
+    void *p = std::aligned_alloc(kHuge, sz);
+    if (!p) throw std::bad_alloc();
+    (void)madvise(p, sz, MADV_HUGEPAGE);  // a hint: refused, the table keeps small pages
+    return static_cast<T *>(p);
+  }
+  void deallocate(T *p, size_t n) {
+    if (n * sizeof(T) < kHuge) std::allocator<T>::deallocate(p, n);
+    else std::free(p);
+  }
   template <class U>
   void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
     ::new (static_cast<void *>(p)) U;

@@ -252,14 +252,14 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       else it->terms.push_back(t);
     }
   }
-  std::vector<std::vector<LocalTask>> per_fock(bra->n_fock);
+  std::vector<RawVec<LocalTask>> per_fock(bra->n_fock);
   int errcode = FCG_OK;
   std::string errmsg;
 #pragma omp parallel for schedule(dynamic, 4)
   for (int64_t fb = 0; fb < bra->n_fock; fb++) {
     if (errcode || bboxes[fb].empty()) continue;
     const int16_t *F = &bra->fock[fb * 2 * N];
-    std::vector<LocalTask> loc;
+    RawVec<LocalTask> loc;
     int16_t fk[2 * FCG_MAXN];
     for (int g = 0; g < h->n_tc && !errcode; g++) {
       const int16_t *tc = &h->tc[(int64_t)g * 2 * N];
@@ -370,7 +370,7 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
       out->tasks[tt++] = v[i].t;
       pairs[pp - 1].t1 = tt;
     }
-    std::vector<LocalTask>().swap(v);
+    RawVec<LocalTask>().swap(v);
   }
   auto pair_less = [](const PairRec &a, const PairRec &b) {
     return a.B != b.B ? a.B < b.B : a.K < b.K;
