@@ -416,10 +416,17 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
   RawVec<uint8_t> lds_class(np);  // 3 tiny, 2 small, 1 middle, 0 full
 #pragma omp parallel reduction(+ : flops, elems)
   {
-    // the previous task's chain order, reused while its shape repeats (the
-    // tasks of one pair share their clusters, so mostly it does)
-    int m_nb = -1;
-    int64_t m_np[4] = {0, 0, 0, 0}, m_rr[4] = {0, 0, 0, 0}, m_arena = 0;
+    // chain orders by task shape (nbody, Γ index counts, rank products): a
+    // direct-mapped cache per thread, since the same shapes recur across
+    // pairs and the search walks up to 24 permutations
+    struct ChainOrder {
+      uint64_t k0 = ~0ull, k1 = ~0ull;
+      int64_t arena = 0;
+      double fl = 0.0;
+      int best[4] = {0, 1, 2, 3};
+    };
+    std::vector<ChainOrder> chain_cache(4096);
+    int64_t m_arena = 0;
     int m_best[4] = {0, 1, 2, 3};
     double m_fl = 0.0;
 #pragma omp for schedule(dynamic, 1024)
@@ -481,7 +488,20 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
         }
         // chain order: the permutation whose stages need the smallest arena
         // (input and output of a stage side by side; ties: fewest flops)
-        if (nb != m_nb || !std::equal(np_, np_ + 4, m_np) || !std::equal(rr_, rr_ + 4, m_rr)) {
+        uint64_t k0 = (uint64_t)nb, k1 = 0;
+        bool keyed = true;
+        for (int a = 0; a < 4; a++) {
+          keyed = keyed && np_[a] < 256 && rr_[a] < 65536;
+          k0 |= (uint64_t)(np_[a] & 255) << (8 + 8 * a);
+          k1 |= (uint64_t)(rr_[a] & 65535) << (16 * a);
+        }
+        ChainOrder &co =
+            chain_cache[((k0 * 0x9E3779B97F4A7C15ull) ^ (k1 * 0xC2B2AE3D27D4EB4Full)) >> 52];
+        if (keyed && co.k0 == k0 && co.k1 == k1) {
+          m_arena = co.arena;
+          m_fl = co.fl;
+          std::copy(co.best, co.best + 4, m_best);
+        } else {
           int perm[4] = {0, 1, 2, 3};
           int64_t best_arena = INT64_MAX;
           double best_fl = 0.0;
@@ -501,11 +521,15 @@ int fcg_build_host_plan(const fcg_ham *h, const fcg_ops *o, const fcg_space *bra
               std::copy(perm, perm + 4, m_best);
             }
           } while (std::next_permutation(perm, perm + nb));
-          m_nb = nb;
-          std::copy(np_, np_ + 4, m_np);
-          std::copy(rr_, rr_ + 4, m_rr);
           m_arena = best_arena;
           m_fl = best_fl;
+          if (keyed) {
+            co.k0 = k0;
+            co.k1 = k1;
+            co.arena = m_arena;
+            co.fl = m_fl;
+            std::copy(m_best, m_best + 4, co.best);
+          }
         }
         if (m_arena > 2 * FCG_LDS_X) {
           fail("chain intermediate exceeds LDS budget");
