@@ -93,6 +93,10 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
   __shared__ int2 sRowT[FCG_MAX_BOX], sColT[FCG_MAX_BOX];
   __shared__ double sThr[FCG_MAX_BOX];
   __shared__ int sFast;
+  // per task, per chain stage s: the cluster, Γ index count, Γ block and
+  // integral stride of the term position contracted at stage s (in LDS, so
+  // that loops over stages index them without a private stack array)
+  __shared__ int sCh[4][4];
 
   const int tid = threadIdx.x;
   const int64_t p = a.pair_list[a.pair_base + blockIdx.x];
@@ -157,37 +161,44 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
   __syncthreads();
 
   for (int64_t t = a.pair_t0[p]; t < a.pair_t0[p + 1]; t++) {
-    const TaskDev td = a.tasks[t];
-    const TermDev tm = a.terms[td.term];
-    const int nb = tm.nbody;
+    const int32_t meta = a.tasks[t].meta;
+    const TermDev *tm = a.terms + a.tasks[t].term;
+    const int nb = tm->nbody;
     double *cur = sA;
-    int ord[4];
-    for (int s = 0; s < 4; s++) ord[s] = task_order(td.meta, s);
+    if (tid < nb) {
+      const int tp = task_order(meta, tid);
+      int str = 1;
+      for (int j = 0; j < tp; j++) str *= tm->dims[j];
+      sCh[tid][0] = tm->cl[tp];
+      sCh[tid][1] = nb == 1 ? 1 : tm->dims[tp];
+      sCh[tid][2] = a.tasks[t].blk[tp];
+      sCh[tid][3] = str;
+    }
+    if (tid < N) sAct[tid] = -1;
+    __syncthreads();
     int rest = 1;
     if (nb == 1) {
       if (tid == 0) cur[0] = 1.0;
     } else {
       // integrals with their axes in chain order (stage s contracts term
-      // position ord[s]); original layout is column-major in term order
-      int ostr[4], pd[4];
-      ostr[0] = 1;
-      for (int s = 1; s < nb; s++) ostr[s] = ostr[s - 1] * tm.dims[s - 1];
-      for (int s = 0; s < nb; s++) { pd[s] = tm.dims[ord[s]]; rest *= pd[s]; }
+      // position task_order(meta, s)); original layout is column-major in
+      // term order
+      for (int s = 0; s < nb; s++) rest *= sCh[s][1];
+      const int64_t ioff = tm->ioff;
       for (int e = tid; e < rest; e += NT) {
         int r = e, src = 0;
         for (int s = 0; s < nb; s++) {
-          src += (r % pd[s]) * ostr[ord[s]];
-          r /= pd[s];
+          const int d = sCh[s][1];
+          src += (r % d) * sCh[s][3];
+          r /= d;
         }
-        cur[e] = a.ints[tm.ioff + src];
+        cur[e] = a.ints[ioff + src];
       }
     }
-    if (tid < N) sAct[tid] = -1;
-    __syncthreads();
     if (tid == 0) {
       int st = 1;
       for (int s = 0; s < nb; s++) {
-        const int c = tm.cl[ord[s]];
+        const int c = sCh[s][0];
         sAct[c] = s;
         sStride[c] = st;
         st *= sRB[c] * sRK[c];
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       const uint8_t *dg = row ? sDigB + x * N : sDigK + x * N;
       int off = 0, st = 1;
       for (int s = 0; s < nb; s++) {
-        const int c = tm.cl[ord[s]];
+        const int c = sCh[s][0];
         off += (row ? dg[c] : sRB[c] * dg[c]) * st;
         st *= sRB[c] * sRK[c];
       }
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       bool out = false;
       for (int c = 0; c < N; c++) {
         bool act = false;
-        for (int s = 0; s < nb; s++) act |= tm.cl[s] == c;
+        for (int s = 0; s < nb; s++) act |= sCh[s][0] == c;
         if (act) continue;
         int v = dg[c];
         if (row) {
@@ -230,10 +241,9 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       (row ? sRowT[x] : sColT[x]) = make_int2(off, out ? -1 : key);
     }
     for (int s = 0; s < nb; s++) {
-      const int tp = ord[s];
-      const int c = tm.cl[tp];
-      const int blk = td.blk[tp];
-      const int np = (nb == 1) ? 1 : tm.dims[tp];
+      const int c = sCh[s][0];
+      const int blk = sCh[s][2];
+      const int np = sCh[s][1];
       const int mb = a.gb_dims[3 * blk + 1];
       const double *G = a.gdata + a.gb_off[blk];
       const int rb = sRB[c], rk = sRK[c], lb = sLoB[c], lk = sLoK[c];
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(NT, MINW) void k_build_blocks(BuildArgs a) {
       rest = osz;
     }
     // Kronecker accumulate into this thread's block elements
-    const double sign = (td.meta & 1) ? -1.0 : 1.0;
+    const double sign = (meta & 1) ? -1.0 : 1.0;
     if (sFast) {
       int r = r0, q = q0;
       for (int k = 0; k < EPT; k++) {
