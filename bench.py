@@ -10,11 +10,12 @@ sigma build (24 orbitals, 6 clusters, ranks 1..8, 464,878 variables).
 
 One step = one sigma build H c over the whole space for all roots, inputs
 resident in HBM (device pointers, no PCIe in the timed region).  The headline
-`value` is the mixed-precision sigma of --precision (default FP32 Hamiltonian
-blocks, FP32 accumulation); the FP64, FP32, split-BF16 and BF16 rates,
-Davidson solves (FP64 vs mixed with FP64 residual refresh at restarts, with
-and without the one-time plan build) and their |dE| against FP64 are
-reported beside it.
+`value` is the mixed-precision sigma of --precision (default for the sparse
+workload: FP32 Hamiltonian blocks applied to the trial vectors rounded to
+bf16, FP32 accumulation; FP32 blocks and vectors for the others); the FP64,
+FP32, split-BF16, BF16 and FP32-with-bf16-vector rates, Davidson solves (FP64
+vs mixed with FP64 residual refresh at restarts, with and without the
+one-time plan build) and their |dE| against FP64 are reported beside it.
 
 Prints ONE JSON line.  --gpus N with N > 1 exits: the library runs on one GPU.
 """
@@ -75,7 +76,11 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serve workload: concurrent solves")
     ap.add_argument("--order", default="fifo", choices=("fifo", "size"),
                     help="serve workload: which ready request a free slot takes")
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x2", "bf16"])
+    ap.add_argument("--precision", default=None,
+                    choices=["fp64", "fp32", "bf16x2", "bf16", "fp32_bf16c"],
+                    help="the headline sigma precision (default fp32_bf16c for the sparse "
+                         "workload, whose one-row apply gathers c rounded to bf16; fp32 for "
+                         "the others, whose multi-row boxes stage c in FP32)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the Davidson solves")
     ap.add_argument("--ss-vecs", type=int, default=12,
@@ -93,6 +98,8 @@ def main():
               "c before each sigma); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.precision is None:
+        args.precision = "fp32_bf16c" if args.workload == "sparse" else "fp32"
     if args.steps is None:
         args.steps = {"pt2": 5, "pt2_12c": 1}.get(args.workload, 30)
     if args.warmup is None:
@@ -126,7 +133,7 @@ def main():
 
     eng = Engine(0)
     t1 = time.time()
-    precs = ("fp64", "fp32", "bf16x2", "bf16")
+    precs = ("fp64", "fp32", "bf16x2", "bf16", "fp32_bf16c")
     if args.workload == "bst":
         pl = cache_hamiltonian(st, st, p.cluster_ops, p.clustered_ham, precisions=precs, engine=eng)
     else:
@@ -236,7 +243,7 @@ def main():
     # Davidson solves: FP64 vs mixed (FP64 residual refresh at restarts)
     solves, dEs = {}, {}
     if not args.no_solve:
-        for prec in dict.fromkeys(("fp64", "fp32", "bf16", args.precision)):
+        for prec in dict.fromkeys(("fp64", "fp32", "bf16", "fp32_bf16c", args.precision)):
             ta = time.perf_counter()
             e, v, stt = pl.davidson(st.coeffs, conv_thresh=1e-8, max_iter=600,
                                     max_ss_vecs=args.ss_vecs,
@@ -252,8 +259,10 @@ def main():
 
     # roofline of the dominant kernel (the sigma apply of the headline precision)
     prec = args.precision
-    esz = {"fp64": 8, "fp32": 4, "bf16x2": 4, "bf16": 2}[prec]
-    csz = 8 if prec == "fp64" else 4
+    esz = {"fp64": 8, "fp32": 4, "bf16x2": 4, "bf16": 2, "fp32_bf16c": 4}[prec]
+    # c as the apply reads it: bf16 for fp32_bf16c's one-row boxes (every
+    # box of the sparse workload), FP32 for multi-row boxes
+    csz = 8 if prec == "fp64" else 2 if (prec == "fp32_bf16c" and args.workload == "sparse") else 4
     bytes_alg = ps["block_elems"] * esz + 4.0 * ps["panel_cols"] + n * R * (csz + 8)
     k_ms = kernel_ms[prec]["apply_kernel_ms"]
     achieved = bytes_alg / (k_ms * 1e-3) / 1e9
@@ -278,7 +287,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x2": "bf16x2/f32", "bf16": "bf16/f32"}[prec],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x2": "bf16x2/f32", "bf16": "bf16/f32",
+                  "fp32_bf16c": "f32 (c as bf16)"}[prec],
         "data": {"bst": "synthetic (seeded integrals, random Tucker cores and factors)",
                  "sparse": "synthetic (seeded integrals, hash-selected rank-1 configurations, "
                            "random start vectors)",

@@ -11,8 +11,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfcg_hip.so")
 
-FCG_FP64, FCG_FP32, FCG_BF16X2, FCG_BF16 = 0, 1, 2, 3
-PRECISION = {"fp64": FCG_FP64, "fp32": FCG_FP32, "bf16x2": FCG_BF16X2, "bf16": FCG_BF16}
+FCG_FP64, FCG_FP32, FCG_BF16X2, FCG_BF16, FCG_FP32_BF16C = 0, 1, 2, 3, 4
+# "fp32_bf16c": the FP32 blocks applied to c rounded to bf16
+PRECISION = {"fp64": FCG_FP64, "fp32": FCG_FP32, "bf16x2": FCG_BF16X2, "bf16": FCG_BF16,
+             "fp32_bf16c": FCG_FP32_BF16C}
 ERRORS = {-1: "FCG_EINVAL", -2: "FCG_ENOMEM", -3: "FCG_EMISSING_BLOCK", -4: "FCG_EOVERFLOW",
           -5: "FCG_EDEVICE", -6: "FCG_ENOTCONV"}
 

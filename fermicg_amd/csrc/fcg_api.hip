@@ -463,6 +463,8 @@ int fcg_create_plan(fcg_ctx *ctx, fcg_ham *h, fcg_ops *o, fcg_space *sp, fcg_spa
                     int nbody, int precisions, fcg_plan **out, int64_t max_bytes, int64_t *need,
                     const double *thr, const FusedSigma *fused) {
   if (!ctx || !h || !o || !sp || !ksp || !out || nbody < 1 || nbody > 4) return FCG_EINVAL;
+  // the bf16-c apply reads the FP32 blocks
+  if (precisions & (1 << FCG_FP32_BF16C)) precisions |= 1 << FCG_FP32;
   (void)hipSetDevice(ctx->device);
   auto t0 = std::chrono::steady_clock::now();
   TRY(fcg_form_boxes(ctx, o, sp));

@@ -35,7 +35,7 @@ constexpr int FCG_LDS_X_SMALL = 1024, FCG_LDS_T_SMALL = 512, FCG_LDS_S_SMALL = 2
 // and factor rows are smaller), so that two workgroups still share a CU
 constexpr int FCG_LDS_X_MID = 2048, FCG_LDS_G_MID = 4096, FCG_LDS_T_MID = 768, FCG_LDS_S_MID = 512;
 constexpr int FCG_MAX_R = 16;         // roots per sigma launch
-constexpr int FCG_N_PREC = 4;  // FCG_FP64 .. FCG_BF16
+constexpr int FCG_N_PREC = 5;  // FCG_FP64 .. FCG_FP32_BF16C
 constexpr int FCG_QUAD = 4;           // panel columns per stored column quad
 constexpr int FCG_CSR_WPB = 4;        // one-row boxes: wavefronts (rows) per workgroup
 constexpr int FCG_XCDS = 8;           // MI355X accelerator complex dies (workgroups round-robin)

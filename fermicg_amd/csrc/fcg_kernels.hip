@@ -443,6 +443,13 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
 
+// an element of c rounded to bf16 (FCG_FP32_BF16C): read as the FP32 value
+// whose upper 16 bits it holds
+struct bf16c {
+  uint16_t u;
+  __device__ operator float() const { return __uint_as_float((uint32_t)u << 16); }
+};
+
 // R consecutive values from global memory (16- or 8-byte vector loads when
 // R allows; the interleaved vector keeps a variable's R roots contiguous)
 template <typename TC, int R>
@@ -453,6 +460,25 @@ __device__ __forceinline__ void load_row(const TC *__restrict__ p, TC (&v)[R]) {
       const float4 t = reinterpret_cast<const float4 *>(p)[h];
       v[4 * h] = t.x; v[4 * h + 1] = t.y; v[4 * h + 2] = t.z; v[4 * h + 3] = t.w;
     }
+  } else if constexpr (sizeof(TC) == 2 && R % 4 == 0) {
+#pragma unroll
+    for (int h = 0; h < R / 4; h++) {
+      const uint2 t = reinterpret_cast<const uint2 *>(p)[h];
+      v[4 * h].u = (uint16_t)t.x;
+      v[4 * h + 1].u = (uint16_t)(t.x >> 16);
+      v[4 * h + 2].u = (uint16_t)t.y;
+      v[4 * h + 3].u = (uint16_t)(t.y >> 16);
+    }
+  } else if constexpr (sizeof(TC) == 2 && R % 2 == 0) {
+#pragma unroll
+    for (int h = 0; h < R / 2; h++) {
+      const uint32_t t = reinterpret_cast<const uint32_t *>(p)[h];
+      v[2 * h].u = (uint16_t)t;
+      v[2 * h + 1].u = (uint16_t)(t >> 16);
+    }
+  } else if constexpr (sizeof(TC) == 2) {
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = p[r];
   } else if constexpr (R % 2 == 0) {
     using V2 = typename Vec2<TC>::type;
 #pragma unroll
@@ -784,6 +810,14 @@ __global__ void k_interleave(int64_t n, int R, const double *__restrict__ x, TC 
     y[e] = (TC)x[(e % R) * n + e / R];
 }
 
+// the same, each value rounded to bf16 (round to nearest even)
+__global__ void k_interleave_bf16(int64_t n, int R, const double *__restrict__ x,
+                                  bf16c *__restrict__ y) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * R;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e].u = (uint16_t)(bf16_rne_bits((float)x[(e % R) * n + e / R]) >> 16);
+}
+
 __global__ void k_diag(int64_t n_box, const int64_t *__restrict__ var,
                        const int64_t *__restrict__ dkoff, const int32_t *__restrict__ dcol,
                        const double *__restrict__ K, double *__restrict__ hd) {
@@ -923,7 +957,10 @@ void launch_rows_v(const fcg_plan *pl, const TK *K, const TC *c, double *sigma, 
 template <typename TK, typename TA, typename TC, int R>
 void launch_r(const fcg_plan *pl, int vm, int vr, int parts, const TK *K, const TC *c,
               double *sigma, hipStream_t s) {
-  if ((parts & 1) && pl->n_multi > 0) {
+  if constexpr (sizeof(TC) == 2) {
+    // bf16 c is gathered by the one-row apply only (multi-row boxes stage an
+    // FP32 c, launched separately)
+  } else if ((parts & 1) && pl->n_multi > 0) {
     if constexpr (R <= TUNED_MAX_R) {
       if (vm == 0) launch_multi_v<TK, TA, TC, R, 0>(pl, K, c, sigma, s);
       else if (vm == 1) launch_multi_v<TK, TA, TC, R, 1>(pl, K, c, sigma, s);
@@ -984,6 +1021,30 @@ hipError_t fcg_launch_apply(const fcg_plan *pl, int R, const double *c, double *
     hipLaunchKernelGGL(k_interleave<double>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);
     if (before_apply) (void)hipEventRecord(before_apply, s);
     return launch_by_r<double, double, double>(pl, R, vm, vr, parts, pl->d_K64, ci, sigma, s);
+  }
+  if (precision == FCG_FP32_BF16C) {
+    // FP32 blocks; the one-row apply gathers c rounded to bf16, multi-row
+    // boxes stage an FP32 c.  ci_buf holds R * nk doubles: the FP32 copy
+    // (when multi-row boxes run) at its start, the bf16 copy after it at an
+    // 8-byte boundary (it fits for R * nk >= 2; a one-value c with both kinds
+    // of boxes takes the FP32 copy for both)
+    const int64_t n = nk * R;
+    const bool multi = (parts & 1) && pl->n_multi > 0;
+    const bool rows = (parts & 2) && pl->n_single_slots > 0;
+    char *base = static_cast<char *>(ci_buf);
+    float *cf = reinterpret_cast<float *>(base);
+    const int64_t off = multi ? (4 * n + 7) / 8 * 8 : 0;
+    bf16c *cb = rows && off + 2 * n <= 8 * n ? reinterpret_cast<bf16c *>(base + off) : nullptr;
+    if (multi || (rows && !cb))
+      hipLaunchKernelGGL(k_interleave<float>, dim3(1024), dim3(256), 0, s, nk, R, c, cf);
+    if (cb) hipLaunchKernelGGL(k_interleave_bf16, dim3(1024), dim3(256), 0, s, nk, R, c, cb);
+    if (before_apply) (void)hipEventRecord(before_apply, s);
+    hipError_t e = hipSuccess;
+    if (multi) e = launch_by_r<float, float, float>(pl, R, vm, vr, 1, pl->d_K32, cf, sigma, s);
+    if (e == hipSuccess && rows)
+      e = cb ? launch_by_r<float, float, bf16c>(pl, R, vm, vr, 2, pl->d_K32, cb, sigma, s)
+             : launch_by_r<float, float, float>(pl, R, vm, vr, 2, pl->d_K32, cf, sigma, s);
+    return e;
   }
   float *ci = static_cast<float *>(ci_buf);
   hipLaunchKernelGGL(k_interleave<float>, dim3(1024), dim3(256), 0, s, nk, R, c, ci);

@@ -45,8 +45,11 @@ enum {
 /* Stored Hamiltonian-block precisions for the sigma apply.  FCG_FP64 and
  * FCG_FP32 accumulate in their own precision; FCG_BF16X2 stores each
  * element as hi + lo bf16 (the FP32 value to ~16 bits) and FCG_BF16 as the
- * hi bf16 alone, both accumulated in FP32. */
-enum { FCG_FP64 = 0, FCG_FP32 = 1, FCG_BF16X2 = 2, FCG_BF16 = 3 };
+ * hi bf16 alone, both accumulated in FP32.  FCG_FP32_BF16C applies the FP32
+ * blocks to the vector c rounded to bf16 (round to nearest even), which
+ * halves the bytes each gathered column of c occupies; FP32 accumulation
+ * (a plan built with it keeps the FP32 copy). */
+enum { FCG_FP64 = 0, FCG_FP32 = 1, FCG_BF16X2 = 2, FCG_BF16 = 3, FCG_FP32_BF16C = 4 };
 
 typedef struct fcg_ctx *fcg_ctx_t;
 typedef struct fcg_ops *fcg_ops_t;
@@ -113,7 +116,7 @@ void fcg_space_destroy(fcg_space_t sp);
  * tps_ci_matvec, src/tpsci_outer.jl:462-527).  nbody (1..4) drops terms on
  * more clusters (build_sigma! nbody keyword).  precisions: bitmask of
  * (1<<FCG_FP64)|(1<<FCG_FP32)|(1<<FCG_BF16X2)|(1<<FCG_BF16) copies kept
- * resident (FP64 always). */
+ * resident (FP64 always; (1<<FCG_FP32_BF16C) keeps the FP32 copy). */
 int fcg_plan_create(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t sp, int nbody,
                     int precisions, fcg_plan_t *out);
 /* Hamiltonian between two spaces (rows: bra, columns: ket): sigma_bra =
@@ -205,7 +208,7 @@ int fcg_space_diagonal(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t 
 /* Davidson for the lowest R eigenpairs (replaces tps_ci_davidson,
  * src/tpsci_outer.jl:364-453, and ci_solve, src/tucker_outer.jl:68-215,
  * whose eigensolver is the external BlockDavidson package).  precision
- * FCG_FP64 runs every sigma in FP64; FCG_FP32 / FCG_BF16X2 / FCG_BF16 run the
+ * FCG_FP64 runs every sigma in FP64; the other precisions run the
  * expansion sigmas in that precision and recompute the Ritz vectors'
  * sigma and residual in FP64 at every restart, when the reduced-precision
  * residual falls below conv_thresh, and every refresh_k iterations if
@@ -216,7 +219,7 @@ typedef struct {
   int max_iter;         /* reference max_iter */
   int max_ss_vecs;      /* subspace vectors per root before restart */
   double lindep_thresh; /* drop expansion vectors below this norm */
-  int precision;        /* FCG_FP64 / FCG_FP32 / FCG_BF16X2 / FCG_BF16 */
+  int precision;        /* FCG_FP64 .. FCG_FP32_BF16C */
   int refresh_k;        /* extra FP64 refresh period (0: restarts only) */
   int precond;          /* 1: diagonal (Davidson) preconditioner */
   /* host (n_var) diagonal for the preconditioner, or NULL for the diagonal

@@ -33,8 +33,9 @@ from oracle.pyoracle import BstOracle, Oracle
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ("fp64", "fp32", "bf16x2")
-TOL = {"fp64": 1e-12, "fp32": 1e-5, "bf16x2": 1e-4}
+PRECS = ("fp64", "fp32", "bf16x2", "fp32_bf16c")
+# fp32_bf16c: c rounded to bf16 (unit roundoff 2^-9) in the one-row apply
+TOL = {"fp64": 1e-12, "fp32": 1e-5, "bf16x2": 1e-4, "fp32_bf16c": 1e-2}
 THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
@@ -54,13 +55,14 @@ def _check_sigma(pl, c, idx, ref):
 
 def _check_mixed_solve(pl, c):
     e = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp32_bf16c"):
         ev, _, st = pl.davidson(c, conv_thresh=1e-8, max_iter=600, max_ss_vecs=12,
                                 precision=prec)
         print(prec, ev, st["iterations"], st["converged"], flush=True)
         assert st["converged"]
         e[prec] = ev
     assert np.abs(e["fp32"] - e["fp64"]).max() <= 1e-6
+    assert np.abs(e["fp32_bf16c"] - e["fp64"]).max() <= 1e-6
 
 
 def test_bst_24o_6c_M16_full_size():

@@ -101,8 +101,10 @@ def test_sparse_sigma_fp64(eng, name):
     assert np.abs(hp - hd).max() < 1e-12 * np.abs(hd).max()
 
 
-# BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x2", 1e-4), ("bf16", 1e-2)])
+# BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2,
+# also for fp32_bf16c, whose one-row apply gathers c rounded to bf16
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x2", 1e-4), ("bf16", 1e-2),
+                                      ("fp32_bf16c", 1e-2)])
 def test_sparse_sigma_reduced(eng, prec, tol):
     p, st = CASES["rank1_4c_M8"]()
     s = tps_ci_matvec(st, p.cluster_ops, p.clustered_ham, precision=prec, engine=eng)
@@ -118,7 +120,7 @@ def test_sparse_nbody2(eng):
     assert np.abs(s - ref).max() / np.abs(ref).max() < 1e-12
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp32_bf16c"])
 def test_sparse_davidson(eng, prec):
     p, st = CASES["rank1_5c_M16_full"]()
     H = _oracle(p, st).full_H()

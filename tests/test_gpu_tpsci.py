@@ -53,8 +53,10 @@ def test_diagonal(eng, small):
     assert np.abs(hp - hd).max() < 1e-12 * np.abs(hd).max()
 
 
-# BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x2", 1e-4), ("bf16", 1e-2)])
+# BF16 keeps 8 mantissa bits (unit roundoff 2^-9 = 2e-3 per element): 1e-2,
+# also for fp32_bf16c (c rounded to bf16 in the one-row apply)
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x2", 1e-4), ("bf16", 1e-2),
+                                      ("fp32_bf16c", 1e-2)])
 def test_sigma_reduced_precision(eng, small, prec, tol):
     p, st, orc = small
     s_gpu = tps_ci_matvec(st, p.cluster_ops, p.clustered_ham, precision=prec, engine=eng)
@@ -98,7 +100,7 @@ def test_davidson_matches_dense(eng, small):
 def test_davidson_mixed_precision(eng, small):
     p, st, orc = small
     w = np.linalg.eigvalsh(orc.full_H())[:3]
-    for prec in ("fp32", "bf16x2", "bf16"):
+    for prec in ("fp32", "bf16x2", "bf16", "fp32_bf16c"):
         e, v = tps_ci_davidson(st, p.cluster_ops, p.clustered_ham, conv_thresh=1e-8,
                                max_iter=400, precision=prec, refresh_k=4, engine=eng)
         print(prec, e - w, v.davidson_stats, flush=True)
