@@ -3,10 +3,11 @@ interleave scratch in a fixed device array instead of an allocation.
 
 Through the C ABI, on device-resident operands: two calls in a row (the
 second on a smaller plan and root count) must equal fcg_sigma_device's
-results bit for bit in FP64 and FP32 (same kernels, same order of every
-sum); R x n_var above the fixed 2^23 values is refused with FCG_EOVERFLOW
-and a message, and a call within it on the same large space still equals
-fcg_sigma_device.
+results bit for bit in FP64, FP32 and FP32_BF16C (same kernels, same order
+of every sum; FP32_BF16C's bf16 copy of c shares the scratch with the FP32
+copy that multi-row boxes stage); R x n_var above the fixed 2^23 values is
+refused with FCG_EOVERFLOW and a message, and a call within it on the same
+large space still equals fcg_sigma_device.
 """
 import numpy as np
 import pytest
@@ -38,8 +39,9 @@ def test_noalloc_equals_sigma_device_two_calls():
     eng = Engine(0)
     p, st = tpsci_small(seed=0, R=4)
     st.set_vector(np.random.default_rng(1).normal(size=(len(st), 4)))
-    pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, precisions=("fp64", "fp32"))
-    for prec in ("fp64", "fp32"):
+    precs = ("fp64", "fp32", "fp32_bf16c")
+    pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, precisions=precs)
+    for prec in precs:
         a, b = _both(pl, st.coeffs, prec)
         print("first", len(st), prec, flush=True)
         assert not np.isnan(b).any() and np.array_equal(a, b)
@@ -47,9 +49,9 @@ def test_noalloc_equals_sigma_device_two_calls():
     p2 = build_problem(8, 2, 4, 16, seed=5)
     st2 = TPSCIstate.full_blocks(p2.bases, p2.focks, R=2)
     st2.set_vector(np.random.default_rng(2).normal(size=(len(st2), 2)))
-    pl2 = eng.tpsci_plan(st2, p2.cluster_ops, p2.clustered_ham, precisions=("fp64", "fp32"))
+    pl2 = eng.tpsci_plan(st2, p2.cluster_ops, p2.clustered_ham, precisions=precs)
     other = _lib.Context(0)
-    for prec in ("fp64", "fp32"):
+    for prec in precs:
         a, b = _both(pl2, st2.coeffs, prec, ctx=other)
         print("second", len(st2), prec, flush=True)
         assert not np.isnan(b).any() and np.array_equal(a, b)
@@ -64,14 +66,16 @@ def test_noalloc_limit():
     p = build_problem(32, 8, 16, 64, seed=4242, max_transfer=2)
     st = rank1_space(p, R=4, frac=1.0, seed=4242)
     st.set_vector(np.random.default_rng(3).normal(size=(len(st), 4)))
-    pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, nbody=1)
+    pl = eng.tpsci_plan(st, p.cluster_ops, p.clustered_ham, nbody=1,
+                        precisions=("fp64", "fp32_bf16c"))
     n = len(st)
     assert 16 * n > 2 ** 23 >= 4 * n
     dev = torch.device("cuda:0")
     big = torch.zeros((16, n), dtype=torch.float64, device=dev)
     with pytest.raises(_lib.FCGError, match="FCG_EOVERFLOW"):
         pl.sigma_device_noalloc(big.data_ptr(), big.data_ptr(), 16)
-    a, b = _both(pl, st.coeffs, "fp64")
-    assert np.array_equal(a, b)
+    for prec in ("fp64", "fp32_bf16c"):
+        a, b = _both(pl, st.coeffs, prec)
+        assert not np.isnan(b).any() and np.array_equal(a, b)
     del pl
     eng.drop_plans()
