@@ -925,10 +925,29 @@ struct SumArgs {
   const int64_t *gb_off;
   const int32_t *gb_dims;
   const double *gdata;
+  const double *h0tab;       // [N][S*S][2^sbits]: H0_c[x, x] by (cluster, sector, state)
   const double *e0;
   double *part;              // [blocks][R][2]
   unsigned long long *n_out; // configurations outside the variational space
 };
+
+// the H0 diagonal of every (cluster, sector, state) in one table, so that a
+// configuration's Hd is N table reads instead of N chains of Γ-block lookups
+__global__ void k_h0_table(SumArgs a, double *tab) {
+  const int SS = a.S * a.S, M = 1 << a.sbits;
+  const int64_t n = (int64_t)a.N * SS * M;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i / ((int64_t)SS * M)), code = (int)((i / M) % SS), xs = (int)(i % M);
+    const int blk = a.gblock[(((int64_t)c * a.n_ops + a.h0_op) * SS + code) * SS + code];
+    double v = 0.0;
+    if (blk >= 0) {
+      const int mb = a.gb_dims[3 * blk + 1];
+      if (xs < mb) v = a.gdata[a.gb_off[blk] + xs + (int64_t)mb * xs];
+    }
+    tab[i] = v;
+  }
+}
 
 // per x: sigma = sum of its contributions in sorted order; outside the
 // variational space, sigma^2 / (E0 - Hd) with Hd = sum_c H0_c[x_c, x_c]
@@ -963,9 +982,7 @@ __global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
     for (int c = 0; c < a.N; c++) {
       const int code = (int)((k.hi >> (a.fbits * c)) & ((1ull << a.fbits) - 1));
       const int xs = (int)((k.lo >> (a.sbits * c)) & ((1ull << a.sbits) - 1));
-      const int blk = a.gblock[(((int64_t)c * a.n_ops + a.h0_op) * SS + code) * SS + code];
-      const int mb = a.gb_dims[3 * blk + 1];
-      hd += a.gdata[a.gb_off[blk] + xs + (int64_t)mb * xs];
+      hd += a.h0tab[(((int64_t)c * SS + code) << a.sbits) + xs];
     }
     for (int r = 0; r < R; r++) {
       double v = 0.0;
@@ -1622,6 +1639,13 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
   sa.gb_off = o->d_gb_off;
   sa.gb_dims = o->d_gb_dims;
   sa.gdata = o->d_gdata;
+  Buf<double> d_h0tab(ctx);
+  if (!d_h0tab.alloc(((int64_t)N * S * S) << sbits)) {
+    ctx->err = "PT2 H0 diagonal table allocation failed";
+    return FCG_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_h0_table, dim3(256), dim3(256), 0, s, sa, d_h0tab.p);
+  sa.h0tab = d_h0tab.p;
   sa.e0 = d_e0.p;
   sa.part = d_part.p;
   sa.n_out = d_nout.p;
