@@ -35,6 +35,8 @@ EXPORTS = [
     "fcg_apply_variant_counts",
     "fcg_set_build_class_cap",
     "fcg_open_sigma",
+    "fcg_open_sigma_sparse", "fcg_open_result_size", "fcg_open_result_copy",
+    "fcg_open_result_destroy",
     "fcg_sigma_device_noalloc",
     "fcg_tpsci_pt2_sparse",
     "fcg_tpsci_pt2_sparse_ex",
@@ -150,6 +152,12 @@ def load():
                                           P(c_f64), P(PT2SparseStats)]
     L.fcg_open_sigma.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(c_f64), c_f64,
                                  P(c_f64), P(ctypes.c_uint8)]
+    L.fcg_open_sigma_sparse.argtypes = [VP, VP, VP, VP, ctypes.c_int, ctypes.c_int, P(c_f64),
+                                        c_f64, P(VP)]
+    L.fcg_open_result_size.argtypes = [VP, P(c_i64)]
+    L.fcg_open_result_copy.argtypes = [VP, P(c_i16), P(c_i16), P(c_f64)]
+    L.fcg_open_result_destroy.argtypes = [VP]
+    L.fcg_open_result_destroy.restype = None
     L.fcg_plan_destroy.argtypes = [VP]
     L.fcg_plan_destroy.restype = None
     L.fcg_plan_stats.argtypes = [VP, P(c_f64)]
@@ -591,6 +599,36 @@ def open_sigma(ctx, ham, ops, bra, ket, c, nbody=4, thresh=0.0):
                                    float(thresh), _p(out, c_f64), _p(hit, ctypes.c_uint8)),
               "fcg_open_sigma")
     return out.T.copy(), hit[:nb].astype(bool)
+
+
+def open_sigma_sparse(ctx, ham, ops, ket, c, n_clusters, nbody=4, thresh=0.0):
+    """Screened open matvec generated per (ket configuration, term)
+    (fcg_open_sigma_sparse): returns (fock [n, N, 2], cfg [n, N] 1-based,
+    sigma [n, R]) of the configurations some element reached, in the
+    engine's order."""
+    c = np.asarray(c, dtype=np.float64)
+    if c.ndim == 1:
+        c = c[:, None]
+    R = c.shape[1]
+    if c.shape[0] != ket.nvar():
+        raise ValueError("ket vector length %d != space dimension %d" % (c.shape[0], ket.nvar()))
+    cc = np.ascontiguousarray(c.T)
+    res = VP()
+    ctx.check(ctx.L.fcg_open_sigma_sparse(ctx.h, ham.h, ops.h, ket.h, nbody, R, _p(cc, c_f64),
+                                          float(thresh), ctypes.byref(res)),
+              "fcg_open_sigma_sparse")
+    try:
+        n = c_i64(0)
+        ctx.check(ctx.L.fcg_open_result_size(res, ctypes.byref(n)), "fcg_open_result_size")
+        n = n.value
+        fock = np.zeros((max(n, 1), n_clusters, 2), np.int16)
+        cfg = np.zeros((max(n, 1), n_clusters), np.int16)
+        sig = np.zeros((R, max(n, 1)))
+        ctx.check(ctx.L.fcg_open_result_copy(res, _p(fock, c_i16), _p(cfg, c_i16),
+                                             _p(sig, c_f64)), "fcg_open_result_copy")
+    finally:
+        ctx.L.fcg_open_result_destroy(res)
+    return fock[:n], cfg[:n], sig[:, :n].T.copy()
 
 
 def space_diagonal(ctx, ham, ops, space):

@@ -1012,6 +1012,27 @@ __global__ __launch_bounds__(256) void k_pt2_seg_sum(SumArgs a) {
   if (threadIdx.x == 0 && sOut) atomicAdd(a.n_out, sOut);
 }
 
+// open sigma: per x (segment), its full key and sigma_r = the sum of its
+// contributions in sorted order (the order k_pt2_seg_sum sums them in)
+__global__ __launch_bounds__(256) void k_open_out(SumArgs a, uint64_t *khi, uint64_t *klo,
+                                                  double *sig) {
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.n_seg;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = a.seg_start[s], e = a.seg_start[s + 1];
+    const Key128 kc = a.skey[b];
+    khi[s] = a.tkeys[chunk_key_target(kc, a.sN)];
+    klo[s] = chunk_key_states(kc, a.sN);
+    for (int r = 0; r < a.R; r++) {
+      double v = 0.0;
+      if (a.sval)
+        for (int64_t i = b; i < e; i++) v += a.sval[i];
+      else
+        for (int64_t i = b; i < e; i++) v += a.eval[a.perm[i] * a.R + r];
+      sig[s * a.R + r] = v;
+    }
+  }
+}
+
 __global__ void k_iota(int64_t n, uint64_t *p) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -1085,6 +1106,19 @@ double seconds_since(std::chrono::steady_clock::time_point t) {
 
 }  // namespace
 
+// Configurations reached by the sparse open matvec, in the engine's order:
+// packed keys (Fock sector codes, states) and sigma (R per configuration)
+struct fcg_open_result {
+  int N = 0, R = 0, S = 0, fbits = 0, sbits = 0;
+  std::vector<uint64_t> khi, klo;
+  std::vector<double> sig;
+};
+
+static int pt2_sparse_run(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var, int R,
+                          const double *c, const double *e0, int nbody, int h0_op,
+                          double thresh, int precision, double *e2,
+                          fcg_pt2_sparse_stats *st, fcg_open_result *open);
+
 extern "C" int fcg_tpsci_pt2_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var,
                                     int R, const double *c, const double *e0, int nbody,
                                     int h0_op, double thresh, double *e2,
@@ -1096,6 +1130,54 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
                                        int R, const double *c, const double *e0, int nbody,
                                        int h0_op, double thresh, int precision, double *e2,
                                        fcg_pt2_sparse_stats *st) {
+  return pt2_sparse_run(ctx, h, o, var, R, c, e0, nbody, h0_op, thresh, precision, e2, st,
+                        nullptr);
+}
+
+extern "C" int fcg_open_sigma_sparse(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t ket,
+                                     int nbody, int R, const double *c, double thresh,
+                                     fcg_open_result_t *out) {
+  if (!out || R < 1 || R > FCG_MAX_R) return FCG_EINVAL;
+  *out = nullptr;
+  auto res = std::make_unique<fcg_open_result>();
+  std::vector<double> e0(R, 0.0), e2(R, 0.0);
+  const int rc = pt2_sparse_run(ctx, h, o, ket, R, c, e0.data(), nbody, 0, thresh, FCG_FP64,
+                                e2.data(), nullptr, res.get());
+  if (rc) return rc;
+  *out = res.release();
+  return FCG_OK;
+}
+
+extern "C" int fcg_open_result_size(fcg_open_result_t res, int64_t *n) {
+  if (!res || !n) return FCG_EINVAL;
+  *n = (int64_t)res->klo.size();
+  return FCG_OK;
+}
+
+extern "C" int fcg_open_result_copy(fcg_open_result_t res, int16_t *fock, int16_t *cfg,
+                                    double *sigma) {
+  if (!res || !fock || !cfg || !sigma) return FCG_EINVAL;
+  const int N = res->N, R = res->R, S = res->S;
+  const int64_t n = (int64_t)res->klo.size();
+  const uint64_t fm = (uint64_t(1) << res->fbits) - 1, sm = (uint64_t(1) << res->sbits) - 1;
+  for (int64_t x = 0; x < n; x++) {
+    for (int cc = 0; cc < N; cc++) {
+      const int code = (int)((res->khi[x] >> (res->fbits * cc)) & fm);
+      fock[(x * N + cc) * 2] = (int16_t)(code / S);
+      fock[(x * N + cc) * 2 + 1] = (int16_t)(code % S);
+      cfg[x * N + cc] = (int16_t)(((res->klo[x] >> (res->sbits * cc)) & sm) + 1);
+    }
+    for (int r = 0; r < R; r++) sigma[(int64_t)r * n + x] = res->sig[x * R + r];
+  }
+  return FCG_OK;
+}
+
+extern "C" void fcg_open_result_destroy(fcg_open_result_t res) { delete res; }
+
+static int pt2_sparse_run(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, fcg_space_t var, int R,
+                          const double *c, const double *e0, int nbody, int h0_op,
+                          double thresh, int precision, double *e2,
+                          fcg_pt2_sparse_stats *st, fcg_open_result *open) {
   if (!ctx || !h || !o || !var || !c || !e0 || !e2 || R < 1 || R > FCG_MAX_R || nbody < 1 ||
       nbody > 4 || h0_op < 0 || h0_op >= o->n_ops || !(thresh >= 0.0) ||
       (precision != FCG_FP64 && precision != FCG_FP32))
@@ -1231,7 +1313,14 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
   if (fbits * N > 64 || sbits * N > 64) {
     ctx->err = "fcg_tpsci_pt2_sparse: configuration keys need more than 128 bits";
     return FCG_EOVERFLOW;
+  }  if (open) {
+    open->N = N;
+    open->R = R;
+    open->S = S;
+    open->fbits = fbits;
+    open->sbits = sbits;
   }
+
   // screening thresholds per ket (the reference's thresh / max_r |c|), and
   // each Fock block's kets ordered by max_r |c| descending (ties by position)
   std::vector<double> cabs(std::max<int64_t>(n_cfg, 1), 0.0), thr(std::max<int64_t>(n_cfg, 1));
@@ -1713,6 +1802,33 @@ extern "C" int fcg_tpsci_pt2_sparse_ex(fcg_ctx_t ctx, fcg_ham_t h, fcg_ops_t o, 
                        d_seg.p);
     FCG_HIP(hipMemcpyAsync(d_seg.p + n_seg, &Ec, sizeof(int64_t), hipMemcpyHostToDevice, s), ctx);
     sa.n_seg = n_seg;
+    if (open) {
+      // open sigma: every configuration of the chunk with its key and sigma
+      Buf<uint64_t> d_khi(ctx), d_klo(ctx);
+      Buf<double> d_sig(ctx);
+      if (!d_khi.alloc(std::max<int64_t>(n_seg, 1)) || !d_klo.alloc(std::max<int64_t>(n_seg, 1)) ||
+          !d_sig.alloc(std::max<int64_t>(n_seg * R, 1))) {
+        ctx->err = "open sigma output allocation failed";
+        return FCG_ENOMEM;
+      }
+      hipLaunchKernelGGL(k_open_out, dim3(SUM_BLOCKS), dim3(256), 0, s, sa, d_khi.p, d_klo.p,
+                         d_sig.p);
+      FCG_HIP(hipGetLastError(), ctx);
+      const size_t at = open->klo.size();
+      open->khi.resize(at + n_seg);
+      open->klo.resize(at + n_seg);
+      open->sig.resize((at + n_seg) * R);
+      FCG_HIP(hipMemcpyAsync(open->khi.data() + at, d_khi.p, sizeof(uint64_t) * n_seg,
+                             hipMemcpyDeviceToHost, s), ctx);
+      FCG_HIP(hipMemcpyAsync(open->klo.data() + at, d_klo.p, sizeof(uint64_t) * n_seg,
+                             hipMemcpyDeviceToHost, s), ctx);
+      FCG_HIP(hipMemcpyAsync(open->sig.data() + at * R, d_sig.p, sizeof(double) * n_seg * R,
+                             hipMemcpyDeviceToHost, s), ctx);
+      FCG_HIP(hipStreamSynchronize(s), ctx);
+      sort_s += seconds_since(ts);
+      n_seg_total += n_seg;
+      continue;
+    }
     hipLaunchKernelGGL(k_pt2_seg_sum, dim3(SUM_BLOCKS), dim3(256), 0, s, sa);
     FCG_HIP(hipGetLastError(), ctx);
     FCG_HIP(hipMemcpyAsync(part.data(), d_part.p, sizeof(double) * part.size(),

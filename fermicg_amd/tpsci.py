@@ -153,6 +153,15 @@ class Engine:
         return _lib.open_sigma(self.ctx, self.ham(clustered_ham, nbody), self.ops(cluster_ops),
                                bra, ket, ket_state.coeffs, nbody=nbody, thresh=thresh)
 
+    def open_sigma_sparse(self, ket_state, cluster_ops, clustered_ham, nbody=4, thresh=0.0):
+        """Screened open matvec generated per (ket configuration, term)
+        (fcg_open_sigma_sparse): (fock [n, N, 2], cfg [n, N], sigma [n, R])
+        of the configurations some term's element reached."""
+        ket = _lib.DeviceSpace.tpsci(self.ctx, ket_state)
+        return _lib.open_sigma_sparse(self.ctx, self.ham(clustered_ham, nbody),
+                                      self.ops(cluster_ops), ket, ket_state.coeffs,
+                                      ket_state.N, nbody=nbody, thresh=thresh)
+
     def pt2(self, ci_vector, cluster_ops, clustered_ham, E0, nbody=4, H0="Hcmf", thresh=0.0,
             method="dense", precision="fp64"):
         """e2 of an orthonormal TPSCI state over its first-order space:
@@ -377,22 +386,57 @@ def first_order_focks(ci_vector, cluster_ops, clustered_ham):
     Fock block of the state and every TransferConfig of the operator, kept
     when each cluster's electron counts lie in [0, n_orbitals] and its sector
     has an "H" block (open_matvec_thread job list, src/tpsci_matvec_thread.jl:
-    25-52), in first-appearance order."""
-    sizes = cluster_sizes_of(cluster_ops)
+    25-52), in first-appearance order (ket blocks outer, TransferConfigs
+    inner).  Formed with array operations over ket blocks x TransferConfigs,
+    so that spaces of many clusters (thousands of each) take seconds."""
+    sizes = np.array(cluster_sizes_of(cluster_ops), np.int64)
     N = len(cluster_ops)
-    seen, out = set(), []
-    for fk in ci_vector.focks():
-        for tc in clustered_ham.trans:
-            fb = tuple((fk[c][0] + tc[c][0], fk[c][1] + tc[c][1]) for c in range(N))
-            if fb in seen:
-                continue
-            if not all(0 <= a <= sizes[c] and 0 <= b <= sizes[c] for c, (a, b) in enumerate(fb)):
-                continue
-            if not all((fb[c], fb[c]) in cluster_ops[c]["H"] for c in range(N)):
-                continue
-            seen.add(fb)
-            out.append(fb)
-    return out
+    F = np.asarray(ci_vector.fock, np.int64).reshape(-1, N, 2)
+    T = np.array([[tuple(x) for x in tc] for tc in clustered_ham.trans],
+                 np.int64).reshape(-1, N, 2)
+    if len(F) == 0 or len(T) == 0:
+        return []
+    top = int(sizes.max()) + 1
+    has_h = np.zeros((N, top, top), bool)
+    for c in range(N):
+        for fa, fb in cluster_ops[c]["H"]:
+            if fa == fb and 0 <= fa[0] < top and 0 <= fa[1] < top:
+                has_h[c, fa[0], fa[1]] = True
+    cidx = np.arange(N)
+    # each candidate packed into one integer key (sector code a * top + b per
+    # cluster) when N clusters fit 63 bits, else compared as byte rows
+    packed = (top * top) ** N < (1 << 63)
+    weights = np.array([(top * top) ** c for c in range(N)], np.int64) if packed else None
+    seen = np.zeros(0, np.int64) if packed else None
+    seen_rows = set()
+    keep = []
+    step = max(1, (1 << 22) // (len(T) * N))
+    for i0 in range(0, len(F), step):
+        cand = F[i0:i0 + step, None] + T[None]                      # (f, t, N, 2)
+        inr = ((cand >= 0) & (cand <= sizes[None, None, :, None])).all(axis=3)
+        a = np.clip(cand[..., 0], 0, top - 1)
+        b = np.clip(cand[..., 1], 0, top - 1)
+        ok = (inr & has_h[cidx[None, None, :], a, b]).all(axis=2)   # (f, t)
+        rows = cand[ok].reshape(-1, 2 * N)
+        if len(rows) == 0:
+            continue
+        if packed:
+            key = ((rows[:, 0::2] * top + rows[:, 1::2]) * weights).sum(axis=1)
+            _, first = np.unique(key, return_index=True)
+            first = np.sort(first)
+            new = first[~np.isin(key[first], seen)]
+            seen = np.union1d(seen, key[new])
+            keep.append(rows[new])
+        else:
+            for r in rows:
+                t = r.tobytes()
+                if t not in seen_rows:
+                    seen_rows.add(t)
+                    keep.append(r[None])
+    if not keep:
+        return []
+    keep = np.concatenate(keep)
+    return [tuple((int(r[2 * c]), int(r[2 * c + 1])) for c in range(N)) for r in keep]
 
 
 def _full_space(cluster_ops, focks, R):
@@ -415,7 +459,7 @@ def _full_space(cluster_ops, focks, R):
 
 
 def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, prescreen=True,
-                       nbody=4, engine=None):
+                       nbody=4, engine=None, method="dense"):
     """sigma = H c on the first-order space of a TPSCI state
     (open_matvec_thread, src/tpsci_matvec_thread.jl:11-129), on the GPU.
 
@@ -432,10 +476,22 @@ def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, presc
     order (first cluster fastest) and blocks their first appearance; the
     reference's order is its Dicts', so compare by key.  `prescreen` is
     accepted for call compatibility: the reference's bounds only skip
-    elements that fail the same test."""
+    elements that fail the same test.
+
+    method "sparse" generates the same configurations and sigma per (ket
+    configuration, term) with the reference's bounds
+    (fcg_open_sigma_sparse), for first-order spaces whose target Fock
+    blocks (or whose ket blocks x TransferConfigs job list) are too large to
+    list: its TPSCIstate holds only the Fock blocks something reached, in
+    ascending order of their sector codes, each block's configurations in
+    enumeration order (compare by key, as with the reference's Dicts)."""
     from .state import TPSCIstate
     eng = engine or default_engine()
     R = ci_vector.R
+    if method == "sparse":
+        return _open_sparse_state(eng, ci_vector, cluster_ops, clustered_ham, nbody, thresh)
+    if method != "dense":
+        raise ValueError("open_matvec_thread: method is 'dense' or 'sparse'")
     focks = first_order_focks(ci_vector, cluster_ops, clustered_ham)
     target = _full_space(cluster_ops, focks, R)
     sig, keep = eng.open_sigma(ci_vector, target, cluster_ops, clustered_ham, nbody=nbody,
@@ -446,6 +502,30 @@ def open_matvec_thread(ci_vector, cluster_ops, clustered_ham, thresh=1e-9, presc
     out.fock_start = kept_before[target.fock_start]
     out.cfg = target.cfg[keep].copy()
     out.coeffs = sig[keep].copy()
+    return out
+
+
+def _open_sparse_state(eng, ci_vector, cluster_ops, clustered_ham, nbody, thresh):
+    """The sparse open matvec's configurations as a TPSCIstate: the Fock
+    blocks reached, in ascending order of (alpha, beta) counts cluster by
+    cluster, each block's configurations in enumeration order (first cluster
+    fastest)."""
+    from .state import TPSCIstate
+    N, R = ci_vector.N, ci_vector.R
+    fock, cfg, sig = eng.open_sigma_sparse(ci_vector, cluster_ops, clustered_ham, nbody=nbody,
+                                           thresh=thresh)
+    out = TPSCIstate(N, R)
+    if len(cfg) == 0:
+        out.coeffs = np.zeros((0, R))
+        return out
+    fk, blk = np.unique(fock.reshape(len(cfg), 2 * N), axis=0, return_inverse=True)
+    blk = np.asarray(blk).reshape(-1)
+    order = np.lexsort(tuple(cfg[:, c] for c in range(N)) + (blk,))
+    out.fock = fk.reshape(-1, N, 2).astype(np.int16)
+    out.fock_start = np.concatenate([[0], np.cumsum(np.bincount(blk, minlength=len(fk)))]
+                                    ).astype(np.int64)
+    out.cfg = cfg[order].astype(np.int16)
+    out.coeffs = sig[order].copy()
     return out
 
 

@@ -367,6 +367,29 @@ int fcg_open_sigma(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t bra,
                    int nbody, int R, const double *c, double thresh, double *sigma,
                    uint8_t *hit);
 
+/* Open matvec without a target list (replaces open_matvec_thread,
+ * src/tpsci_matvec_thread.jl:11-129, generated as the reference generates it:
+ * per ket configuration and term, _open_matvec_thread_job and
+ * contract_matvec_thread, :233-818, with its bounds, :897-1283): the
+ * configurations reached, in first-order spaces too large to list whole.
+ * Same element test as fcg_open_sigma; a configuration is returned when some
+ * element reached it (the reference's inserted keys).  The engine of
+ * fcg_tpsci_pt2_sparse, its sums kept per configuration instead of reduced to
+ * e2.  *out: a result handle (read it with fcg_open_result_size / _copy,
+ * free it with fcg_open_result_destroy). */
+typedef struct fcg_open_result *fcg_open_result_t;
+int fcg_open_sigma_sparse(fcg_ctx_t ctx, fcg_ham_t ham, fcg_ops_t ops, fcg_space_t ket,
+                          int nbody, int R, const double *c, double thresh,
+                          fcg_open_result_t *out);
+/* Number of configurations in the result. */
+int fcg_open_result_size(fcg_open_result_t res, int64_t *n);
+/* fock (n x N x 2: alpha, beta electrons per cluster), cfg (n x N, 1-based
+ * cluster states) and sigma (column-major n x R) of the n configurations,
+ * in the engine's deterministic order (chunks of target Fock blocks, within
+ * a chunk by states); the caller groups them by Fock configuration. */
+int fcg_open_result_copy(fcg_open_result_t res, int16_t *fock, int16_t *cfg, double *sigma);
+void fcg_open_result_destroy(fcg_open_result_t res);
+
 /* Apply variants the plan chose for a precision and root count (-1 before
  * its first sigma there).  The first sigma of each (precision, R) times each
  * variant of the apply kernels (workgroup shape, loads in flight) on the

@@ -1,5 +1,8 @@
 """open_matvec_thread (src/tpsci_matvec_thread.jl:11-129) on the GPU through the
-C ABI (a screened pair plan over the first-order Fock blocks).
+C ABI: method "dense" (a screened pair plan over the first-order Fock blocks)
+and method "sparse" (fcg_open_sigma_sparse: generated per ket configuration
+and term with the reference's bounds, summed per configuration after a
+device sort).
 
 Oracle: orc_tpsci_open_sigma (oracle/fcg_oracle.c), pinned at thresh 0 to
 dense H c in tests/test_open_sigma_oracle.py.  Bars: the same target Fock
@@ -11,7 +14,8 @@ import numpy as np
 import pytest
 
 from fermicg_amd.marshal import flatten_ham, flatten_ops
-from fermicg_amd.tpsci import Engine, _full_space, first_order_focks, open_matvec_thread
+from fermicg_amd.tpsci import (Engine, _full_space, compute_pt2_energy, first_order_focks,
+                               open_matvec_thread)
 from oracle.pyoracle import Oracle
 from tests.pt2_cases import four_clusters, three_clusters_m4
 
@@ -33,24 +37,79 @@ def _keyed(st):
                                                ("three", 4, 1e-4), ("three", 4, 1e-2),
                                                ("four", 4, 0.0), ("four", 4, 1e-2),
                                                ("four", 2, 1e-4)])
-def test_open_matvec(eng, name, nbody, thresh):
+@pytest.mark.parametrize("method", ["dense", "sparse"])
+def test_open_matvec(eng, name, nbody, thresh, method):
     case = three_clusters_m4() if name == "three" else four_clusters()
     st = case.state
-    print("case", name, "nbody", nbody, "thresh", thresh, "n_var", len(st), flush=True)
-    out = open_matvec_thread(st, case.ops, case.H, thresh=thresh, nbody=nbody, engine=eng)
+    print("case", name, "nbody", nbody, "thresh", thresh, "method", method, "n_var", len(st),
+          flush=True)
+    out = open_matvec_thread(st, case.ops, case.H, thresh=thresh, nbody=nbody, engine=eng,
+                             method=method)
     focks = first_order_focks(st, case.ops, case.H)
-    assert out.focks() == focks
     target = _full_space(case.ops, focks, st.R)
     orc = Oracle(flatten_ops(case.ops, [len(c) for c in case.clusters]),
                  flatten_ham(case.H, nbody), st)
     sig, hit = orc.open_sigma(st.coeffs, focks, np.diff(target.fock_start), nbody=nbody,
                               thresh=thresh)
-    ref_keys = [k for k, i in _keyed(target).items() if hit[i]]
+    ref_keyed = {k: i for k, i in _keyed(target).items() if hit[i]}
     got = _keyed(out)
-    print("inserted", len(got), "oracle", len(ref_keys), "of", len(target), flush=True)
-    assert list(got) == ref_keys
-    ref = sig[hit]
+    print("inserted", len(got), "oracle", len(ref_keyed), "of", len(target), flush=True)
+    if method == "dense":
+        # every target block, in job order, and the inserted keys in order
+        assert out.focks() == focks
+        assert list(got) == list(ref_keyed)
+    else:
+        # the blocks something reached, and the same inserted keys
+        assert set(out.focks()) == {k[0] for k in ref_keyed}
+        assert set(got) == set(ref_keyed)
+    ref = sig[[ref_keyed[k] for k in got]]
     assert out.coeffs.shape == ref.shape
     err = np.abs(out.coeffs - ref).max() / np.abs(ref).max()
     print("rel", err, flush=True)
     assert err < 1e-12
+
+
+def _h0_diag(ops, fock, cfg):
+    """sum_c Hcmf_c[x_c, x_c] of configurations (fock [n, N, 2], cfg [n, N]
+    1-based), clusters in order."""
+    n, N = cfg.shape
+    hd = np.zeros(n)
+    for c in range(N):
+        keys = fock[:, c, 0].astype(np.int64) * 64 + fock[:, c, 1]
+        for k in np.unique(keys):
+            sel = keys == k
+            f = (int(k // 64), int(k % 64))
+            d = np.diag(np.asarray(ops[c]["Hcmf"][(f, f)]))
+            hd[sel] += d[cfg[sel, c] - 1]
+    return hd
+
+
+def _rows(fock, cfg):
+    a = np.ascontiguousarray(np.concatenate([fock.reshape(len(cfg), -1), cfg], axis=1),
+                             dtype=np.int16)
+    return a.view(np.dtype((np.void, a.dtype.itemsize * a.shape[1]))).ravel()
+
+
+def test_open_matvec_sparse_twelve_clusters(eng):
+    """At 12 clusters (48 orbitals), where whole target Fock blocks are out of
+    reach, the sparse open matvec's configurations and sigma reduce to the
+    sparse PT2: the configurations outside the variational space number
+    n_first_order exactly, and sum sigma^2 / (E0 - Hd) over them (Hd the
+    Hcmf diagonal, on the host) equals compute_pt2_energy's e2 to 1e-11."""
+    from tests.pt2_cases import twelve_clusters_rank1
+    case = twelve_clusters_rank1(M=4, nnz=6)
+    st = case.state
+    thresh, e0 = 1e-3, np.array([-30000.0])
+    out = open_matvec_thread(st, case.ops, case.H, thresh=thresh, engine=eng, method="sparse")
+    e2 = compute_pt2_energy(st, case.ops, case.H, E0=e0, thresh_foi=thresh, verbose=0,
+                            method="sparse", engine=eng)
+    stats = compute_pt2_energy.last_stats
+    fock = np.repeat(out.fock, np.diff(out.fock_start), axis=0)
+    outside = ~np.isin(_rows(fock, out.cfg), _rows(np.repeat(st.fock, np.diff(st.fock_start),
+                                                             axis=0), st.cfg))
+    hd = _h0_diag(case.ops, fock[outside], out.cfg[outside])
+    e2_open = np.sum(out.coeffs[outside, 0] ** 2 / (e0[0] - hd))
+    print("reached", len(out), "outside", int(outside.sum()), "n_first_order",
+          stats["n_first_order"], "e2", e2, e2_open, flush=True)
+    assert int(outside.sum()) == stats["n_first_order"]
+    assert abs(e2_open - e2[0]) < 1e-11 * abs(e2[0])
