@@ -531,17 +531,19 @@ def _open_sparse_state(eng, ci_vector, cluster_ops, clustered_ham, nbody, thresh
 
 def compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
                              thresh_foi=1e-8, prescreen=False, verbose=1, threaded=True,
-                             engine=None):
+                             engine=None, open_method="dense"):
     """First-order wavefunction and e2 (compute_pt1_wavefunction,
     src/tpsci_pt1_wavefunction.jl:11-95): sigma = open_matvec_thread on the
     GPU, the state's own configurations projected out, and per root
     v_pt = sigma / (E0 / |c|^2 - Hd) with Hd the H0 diagonal (GPU) and
-    e2 = sigma . v_pt.  Returns (e2, TPSCIstate holding v_pt)."""
+    e2 = sigma . v_pt.  Returns (e2, TPSCIstate holding v_pt).  open_method
+    selects open_matvec_thread's method ("sparse" for first-order spaces too
+    large to list by target Fock block)."""
     from .state import project_out_state
     eng = engine or default_engine()
     norms = np.linalg.norm(ci_vector.coeffs, axis=0)
     sig = open_matvec_thread(ci_vector, cluster_ops, clustered_ham, nbody=nbody, thresh=thresh_foi,
-                             prescreen=prescreen, engine=eng)
+                             prescreen=prescreen, engine=eng, method=open_method)
     sig = project_out_state(sig, ci_vector)
     h0 = eng.local_operator(clustered_ham, H0)
     if E0 is None:
@@ -625,7 +627,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
              thresh_asci=None, thresh_var=None, thresh_spin=None, max_iter=10, conv_thresh=1e-4,
              nbody=4, incremental=True, ci_conv=1e-5, ci_max_iter=50, ci_max_ss_vecs=12,
              ci_lindep_thresh=1e-12, davidson=False, max_mem_ci=20.0, threaded=True, verbose=1,
-             engine=None):
+             engine=None, open_method="dense"):
     """Selected CI in the TPS basis (tpsci_ci, src/tpsci.jl:39-363): each
     iteration adds the configurations of the clipped first-order
     wavefunction to the variational space, solves for R roots there
@@ -642,7 +644,9 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
     configurations: the open matvec of the extract_S2 operator on the
     variational vector at threshold thresh_spin, its residual
     S^2|v> - |v><v|S^2|v> reported per root, and its configurations added
-    with zero coefficients (src/tpsci.jl:126-161).
+    with zero coefficients (src/tpsci.jl:126-161).  open_method selects
+    open_matvec_thread's method ("sparse": generated per ket configuration
+    and term, for first-order spaces too large to list by Fock block).
     Returns (e0, TPSCIstate of the variational vector); the last PT2 estimate
     is tpsci_ci.last_e2."""
     from .state import (add_states, clip_state, copy_state, overlap_states, project_out_state,
@@ -669,7 +673,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
             # S^2|v> - |v><v|S^2|v>: its configurations spin-complete the space
             s2op = eng.s2_operator(cluster_ops)
             res = open_matvec_thread(vec_var, cluster_ops, s2op, nbody=nbody, thresh=thresh_spin,
-                                     engine=eng)
+                                     engine=eng, method=open_method)
             expval = overlap_states(vec_var, res)
             res = add_states(res, scale_state(vec_var, -expval))
             if verbose:
@@ -699,7 +703,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
             del_v0 = add_states(vec_asci, scale_state(vec_asci_old, -S))
             vec_asci_old = copy_state(vec_asci)
             del_sig = open_matvec_thread(del_v0, cluster_ops, clustered_ham, nbody=nbody,
-                                         thresh=thresh_foi, engine=eng)
+                                         thresh=thresh_foi, engine=eng, method=open_method)
             sig = add_states(sig, del_sig)
             sig_old = copy_state(sig)
             sig = project_out_state(sig, vec_asci)
@@ -713,7 +717,7 @@ def tpsci_ci(ci_vector, cluster_ops, clustered_ham, thresh_cipsi=1e-2, thresh_fo
         else:
             e2, vec_pt = compute_pt1_wavefunction(vec_asci, cluster_ops, clustered_ham, E0=efock,
                                                   thresh_foi=thresh_foi, nbody=nbody, verbose=0,
-                                                  engine=eng)
+                                                  engine=eng, open_method=open_method)
         vec_pt = clip_state(vec_pt, thresh_cipsi)
         done = np.abs(e0_last - e0).max() < conv_thresh
         if verbose:

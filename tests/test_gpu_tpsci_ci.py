@@ -85,3 +85,29 @@ def test_pt1_matches_pt2(eng, case):
     e2, vpt = compute_pt1_wavefunction(st, case.ops, case.H, thresh_foi=0.0, verbose=0, engine=eng)
     print("pt1 e2", e2, "pt2", e2_pt2, "n", len(vpt), flush=True)
     assert np.abs(e2 - e2_pt2).max() < 1e-10 * np.abs(e2_pt2).max()
+
+
+@pytest.mark.parametrize("incremental", [True, False])
+def test_tpsci_ci_sparse_open_matvec(eng, case, incremental):
+    """The selection loop with the sparse open matvec (generated per ket
+    configuration and term) selects the same spaces and reaches the same
+    energies as with the target-block open matvec."""
+    kw = dict(thresh_cipsi=1e-2, thresh_foi=1e-5, thresh_asci=-1, conv_thresh=1e-7, ci_conv=1e-8,
+              incremental=incremental, engine=eng)
+    e_d, v_d = tpsci_ci(_start(case), case.ops, case.H, **kw)
+    e2_d = tpsci_ci.last_e2
+    e_s, v_s = tpsci_ci(_start(case), case.ops, case.H, open_method="sparse", **kw)
+    e2_s = tpsci_ci.last_e2
+    print("dense", len(v_d), e_d, e2_d, "sparse", len(v_s), e_s, e2_s, flush=True)
+    assert len(v_s) == len(v_d)
+    assert np.abs(e_s - e_d).max() < 1e-10
+    assert np.abs(e2_s - e2_d).max() < 1e-10
+    # PT1 of the case's state (the loop's end space is complete: its e2 is 0)
+    st = case.state
+    st.set_vector(orthonormalize(st.coeffs))
+    e2a, _ = compute_pt1_wavefunction(st, case.ops, case.H, thresh_foi=1e-6, verbose=0,
+                                      engine=eng)
+    e2b, _ = compute_pt1_wavefunction(st, case.ops, case.H, thresh_foi=1e-6, verbose=0,
+                                      engine=eng, open_method="sparse")
+    print("pt1", e2a, e2b, flush=True)
+    assert np.abs(e2a - e2b).max() < 1e-10 * np.abs(e2a).max()
