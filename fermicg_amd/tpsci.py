@@ -560,7 +560,8 @@ def compute_pt1_wavefunction(ci_vector, cluster_ops, clustered_ham, nbody=4, H0=
 
 
 def compute_qdpt_energy(ci_vector_in, cluster_ops, clustered_ham, nbody=4, H0="Hcmf", E0=None,
-                        thresh_foi=1e-8, prescreen=True, threaded=True, verbose=1, engine=None):
+                        thresh_foi=1e-8, prescreen=True, threaded=True, verbose=1, engine=None,
+                        open_method="dense"):
     """Quasi-degenerate second-order energies (compute_qdpt_energy,
     src/tpsci_pt2_energy.jl:311-361).
 
@@ -581,7 +582,8 @@ def compute_qdpt_energy(ci_vector_in, cluster_ops, clustered_ham, nbody=4, H0="H
     n_model-dimensional eigenproblem run on the host; the matrix is
     symmetrised before eigvalsh (the reference's product is symmetric up to
     rounding).  H0, prescreen and threaded are accepted for call
-    compatibility (the reference prints H0 and uses it nowhere else)."""
+    compatibility (the reference prints H0 and uses it nowhere else).
+    open_method selects open_matvec_thread's method."""
     from .state import project_out_state
     eng = engine or default_engine()
     ci = _copy_state(ci_vector_in)
@@ -590,7 +592,7 @@ def compute_qdpt_energy(ci_vector_in, cluster_ops, clustered_ham, nbody=4, H0="H
         E0 = compute_expectation_value(ci, cluster_ops, clustered_ham, engine=eng)
     E0 = np.asarray(E0, np.float64).reshape(-1)
     sig = open_matvec_thread(ci, cluster_ops, clustered_ham, nbody=nbody, thresh=thresh_foi,
-                             prescreen=prescreen, engine=eng)
+                             prescreen=prescreen, engine=eng, method=open_method)
     sig = project_out_state(sig, ci)
     n_model = len(ci)
     h_model = build_full_H(ci, cluster_ops, clustered_ham, engine=eng)

@@ -93,15 +93,17 @@ def test_qdpt_cr2_reference_values(eng):
     assert np.abs(e - QDPT_REF).max() < 1e-12
 
 
+@pytest.mark.parametrize("open_method", ["dense", "sparse"])
 @pytest.mark.parametrize("thresh", [0.0, 1e-3])
-def test_qdpt_vs_oracle(eng, thresh):
+def test_qdpt_vs_oracle(eng, thresh, open_method):
     case = four_clusters(seed=6, R=2)
     st = case.state
     fops, fham = _flat(case)
     target = _full_space(case.ops, first_order_focks(st, case.ops, case.H), st.R)
     ref, nx = qdpt(fops, fham, st, target, thresh)
-    e = compute_qdpt_energy(st, case.ops, case.H, thresh_foi=thresh, verbose=0, engine=eng)
-    print("thresh", thresh, "model", len(st), "first-order", nx,
+    e = compute_qdpt_energy(st, case.ops, case.H, thresh_foi=thresh, verbose=0, engine=eng,
+                            open_method=open_method)
+    print("thresh", thresh, open_method, "model", len(st), "first-order", nx,
           compute_qdpt_energy.last_stats, flush=True)
     assert compute_qdpt_energy.last_stats["n_first_order"] == nx
     assert np.abs(e - ref).max() < 1e-11 * np.abs(ref).max()
