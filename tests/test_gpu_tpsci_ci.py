@@ -93,7 +93,7 @@ def test_tpsci_ci_sparse_open_matvec(eng, case, incremental):
     configuration and term) selects the same spaces and reaches the same
     energies as with the target-block open matvec."""
     kw = dict(thresh_cipsi=1e-2, thresh_foi=1e-5, thresh_asci=-1, conv_thresh=1e-7, ci_conv=1e-8,
-              incremental=incremental, engine=eng)
+              ci_max_iter=200, incremental=incremental, engine=eng)
     e_d, v_d = tpsci_ci(_start(case), case.ops, case.H, **kw)
     e2_d = tpsci_ci.last_e2
     e_s, v_s = tpsci_ci(_start(case), case.ops, case.H, open_method="sparse", **kw)
